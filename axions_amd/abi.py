@@ -1,0 +1,106 @@
+"""ctypes mirror of the C ABI structs in include/axions/axions.h.
+
+Only layouts and constants live here; loading libaxions.so is done by
+axions_amd.lattice.  Parameter names are the reference's file-scope consts
+(evolution.cpp:23-59, mpi_evolution.cpp:17-60, mooreEvol.cpp:19-42).
+"""
+import ctypes
+import math
+
+AX_OK = 0
+AX_EINVAL = -1
+AX_ENOMEM = -2
+AX_EBUDGET = -3
+AX_EHIP = -4
+AX_ENONFINITE = -5
+AX_ENOTFOUND = -6
+AX_ENODEV = -7
+
+AX_MODEL_AH = 0
+AX_MODEL_MOORE = 1
+AX_BC_PERIODIC = 0
+AX_BC_FIXED = 1
+
+
+class AxParams(ctypes.Structure):
+    _fields_ = [
+        ("model", ctypes.c_int32),
+        ("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("nz", ctypes.c_int32),
+        ("dx", ctypes.c_double), ("dy", ctypes.c_double), ("dz", ctypes.c_double),
+        ("dt", ctypes.c_double),
+        ("lambda_", ctypes.c_double), ("eta", ctypes.c_double), ("g", ctypes.c_double),
+        ("lambda1", ctypes.c_double), ("lambda2", ctypes.c_double), ("lambda12", ctypes.c_double),
+        ("eta1", ctypes.c_double), ("eta2", ctypes.c_double),
+        ("q1", ctypes.c_int32), ("q2", ctypes.c_int32),
+        ("alpha", ctypes.c_double), ("scaling", ctypes.c_double),
+        ("damped_nt", ctypes.c_int32),
+        ("dampFac", ctypes.c_double),
+        ("theta_damp_form", ctypes.c_int32),
+        ("energy_form", ctypes.c_int32),
+        ("boundary", ctypes.c_int32),
+        ("precision", ctypes.c_int32),
+    ]
+
+
+class AxDiag(ctypes.Structure):
+    _fields_ = [
+        ("timestep", ctypes.c_int32),
+        ("finite", ctypes.c_int32),
+        ("energy", ctypes.c_double),
+        ("gauss_mean", ctypes.c_double),
+        ("wound_plaquettes", ctypes.c_int64),
+        ("string_length", ctypes.c_double),
+        ("xi", ctypes.c_double),
+    ]
+
+
+def ah_params(n=64, ny=None, nz=None, dx=0.5, dt=0.1, lam=2.0, eta=1.0, g=1.0, alpha=0.0,
+              scaling=1.0, damped_nt=0, dampFac=0.5, theta_damp_form=0, energy_form=0,
+              precision=64, boundary=AX_BC_PERIODIC, dy=None, dz=None):
+    """Abelian-Higgs parameters; defaults are the plain BASELINE set
+    (dx=0.5, dt=0.1, lambda=2, g=eta=1, no friction)."""
+    p = AxParams()
+    p.model = AX_MODEL_AH
+    p.nx, p.ny, p.nz = n, (ny or n), (nz or n)
+    p.dx = dx
+    p.dy = dx if dy is None else dy
+    p.dz = dx if dz is None else dz
+    p.dt = dt
+    p.lambda_, p.eta, p.g = lam, eta, g
+    p.lambda1 = p.lambda2 = 2.0
+    p.lambda12 = 0.0
+    p.eta1 = p.eta2 = 1.0
+    p.q1, p.q2 = 2, 1
+    p.alpha, p.scaling = alpha, scaling
+    p.damped_nt, p.dampFac = damped_nt, dampFac
+    p.theta_damp_form, p.energy_form = theta_damp_form, energy_form
+    p.boundary, p.precision = boundary, precision
+    return p
+
+
+def moore_params(n=64, ny=None, nz=None, dx=0.7, dt=0.3, lambda1=2.0, lambda2=2.0, lambda12=0.0,
+                 eta1=1.0, eta2=1.0, q1=2, q2=1, g=None, boundary=AX_BC_PERIODIC, precision=64):
+    """Moore-model parameters at mooreEvol.cpp:23-36 defaults;
+    g = sqrt(0.5*lambda1*eta1^2/(q1^2+q2^2)) as mooreEvol.cpp:36."""
+    p = AxParams()
+    p.model = AX_MODEL_MOORE
+    p.nx, p.ny, p.nz = n, (ny or n), (nz or n)
+    p.dx = p.dy = p.dz = dx
+    p.dt = dt
+    p.lambda_, p.eta = 2.0, 1.0
+    p.lambda1, p.lambda2, p.lambda12 = lambda1, lambda2, lambda12
+    p.eta1, p.eta2 = eta1, eta2
+    p.q1, p.q2 = q1, q2
+    p.g = math.sqrt(0.5 * lambda1 * eta1 ** 2 / (q1 * q1 + q2 * q2)) if g is None else g
+    p.alpha, p.scaling, p.damped_nt, p.dampFac = 0.0, 1.0, 0, 0.0
+    p.theta_damp_form, p.energy_form = 0, 0
+    p.boundary, p.precision = boundary, precision
+    return p
+
+
+def ncomp(p):
+    return 5 if p.model == AX_MODEL_AH else 7
+
+
+def state_len(p):
+    return ncomp(p) * 2 * p.nx * p.ny * p.nz

@@ -1,0 +1,155 @@
+/*
+ * axions.h - C ABI of the MI355X lattice-evolution library (libaxions.so).
+ *
+ * The reference (sjcott/Axions) has no library interface: every program is
+ * one main() whose parameters are file-scope consts and whose hot path is
+ * the body of its TimeStep loop.  Each entry point below names the region of
+ * the reference it replaces, so a host program written like the reference's
+ * main() can call this library instead of running its own site loops.
+ *
+ * Conventions
+ *  - Plain C types only; no exceptions cross the boundary.  Every function
+ *    returns AX_OK (0) or a negative AX_E* code; ax_strerror() names it and
+ *    ax_last_error() gives a one-line detail for the calling thread.
+ *  - Host arrays use the reference's Array layout (array.cpp:66-70):
+ *    [component][level][x][y][z], z fastest.  AH components are
+ *    (Re phi, Im phi, theta_x, theta_y, theta_z) as in Data/ic.txt
+ *    (evolution.cpp:108); Moore components are (Re phi1, Im phi1, Re phi2,
+ *    Im phi2, theta_x, theta_y, theta_z) as in Data/moore_ic.txt
+ *    (mooreEvol.cpp:77).  theta_a = g*A_a*dx_a is the link variable.
+ *  - Time levels follow the reference: during TimeStep n the current level
+ *    is tNow = (n+1)%2 and the update overwrites tPast = n%2
+ *    (mpi_evolution.cpp:370-371).
+ *  - A handle is used by one host thread at a time; distinct handles are
+ *    independent.  ax_step() only enqueues work on the handle's HIP stream.
+ */
+#ifndef AXIONS_AXIONS_H
+#define AXIONS_AXIONS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define AX_OK 0
+#define AX_EINVAL (-1)     /* bad argument or parameter combination */
+#define AX_ENOMEM (-2)     /* device or host allocation failed */
+#define AX_EBUDGET (-3)    /* admission refused: HBM budget exceeded */
+#define AX_EHIP (-4)       /* a HIP runtime call failed */
+#define AX_ENONFINITE (-5) /* a NaN/Inf was found in the lattice state */
+#define AX_ENOTFOUND (-6)  /* unknown request id */
+#define AX_ENODEV (-7)     /* no usable GPU */
+
+#define AX_MODEL_AH 0      /* Abelian-Higgs: one complex scalar + U(1) links */
+#define AX_MODEL_MOORE 1   /* two complex scalars, charges q1,q2, one U(1) */
+
+#define AX_BC_PERIODIC 0   /* periodic in x, y, z (mpi_evolution.cpp:385-405) */
+#define AX_BC_FIXED 1      /* edge sites never updated (mooreEvol.cpp:187-189) */
+
+/* Run parameters.  Names and meanings follow the reference's file-scope
+ * consts (evolution.cpp:23-59, mpi_evolution.cpp:17-60, mooreEvol.cpp:19-42).
+ * ax_default_params() fills the BASELINE "plain" Abelian-Higgs set. */
+typedef struct ax_params {
+  int32_t model;            /* AX_MODEL_AH or AX_MODEL_MOORE */
+  int32_t nx, ny, nz;       /* lattice sites per axis, each >= 3 */
+  double dx, dy, dz, dt;
+  /* Abelian-Higgs: V = lambda/4 (|phi|^2 - eta^2)^2, gauge coupling g */
+  double lambda, eta, g;
+  /* Moore (mooreEvol.cpp:27-35); uses g above as the gauge coupling */
+  double lambda1, lambda2, lambda12, eta1, eta2;
+  int32_t q1, q2;
+  /* friction (AH only): PRS term alpha*scaling/time with time = 1 + n*dt,
+   * plus damping dampFac for TimeStep < damped_nt (mpi_evolution.cpp:31-38,
+   * 356, 367-368).  mooreEvol.cpp has no friction; it is ignored there. */
+  double alpha, scaling;
+  int32_t damped_nt;
+  double dampFac;
+  /* 0: theta friction written damp*((theta-thetaPast)/dt), the fused form of
+   *    mpi_evolution.cpp:490-497;  1: (damp*(theta-thetaPast))/dt, the
+   *    two-pass form of evolution.cpp:371-373.  Identical when damp == 0. */
+  int32_t theta_damp_form;
+  /* 0: energy as evolution.cpp:403-427 (E_i = dtheta/(g dt dx_i), zero gauge
+   *    terms when g == 0);  1: as mpi_evolution.cpp:513-530. */
+  int32_t energy_form;
+  int32_t boundary;         /* AX_BC_PERIODIC or AX_BC_FIXED */
+  int32_t precision;        /* 64: FP64 drop-in; 32: FP32 storage+arithmetic */
+} ax_params;
+
+/* Diagnostics of the current state (level tNow of the next TimeStep),
+ * evaluated as the reference does inside its site loop before the update. */
+typedef struct ax_diag {
+  int32_t timestep;         /* n: number of steps taken so far */
+  int32_t finite;           /* 1 if every stored value is finite */
+  double energy;            /* sum dx dy dz rho (evolution.cpp:425) */
+  double gauss_mean;        /* sum |G| / (nx ny nz) (evolution.cpp:1404-1408) */
+  int64_t wound_plaquettes; /* sum over plaquettes of |winding| */
+  double string_length;     /* (2/3) * dx * wound_plaquettes */
+  double xi;                /* string length density L / V */
+} ax_diag;
+
+typedef struct ax_lattice ax_lattice;
+
+void ax_default_params(ax_params *p);
+const char *ax_strerror(int code);
+const char *ax_last_error(void);
+/* Number of stored components (5 AH, 7 Moore) and the host array length
+ * ncomp * 2 * nx * ny * nz that ax_set_state/ax_get_state exchange. */
+int ax_state_size(const ax_params *p, int32_t *ncomp, size_t *n);
+
+/* Allocate device state (zeroed) on the current HIP device.
+ * Replaces the std::vector/Array allocations of main()
+ * (mpi_evolution.cpp:94, evolution.cpp:63). */
+int ax_create(const ax_params *p, ax_lattice **out);
+void ax_destroy(ax_lattice *lat);
+int ax_get_params(const ax_lattice *lat, ax_params *out);
+
+/* Seeded random-phase initial condition with TimeStep reset to 0:
+ * std::mt19937(seed), uniform_real_distribution<double>(0, 2pi), one draw
+ * per site in x-major order, phi = eta*(cos, sin) at both levels, links 0
+ * (mpi_evolution.cpp:287-312).  Moore: moore_phase_mode 0 draws an
+ * independent phase for phi1 then phi2 at each site; 1 uses one phase for
+ * both (moore_mpi_evolution.cpp:304-313). */
+int ax_init_random_phase(ax_lattice *lat, uint32_t seed, int32_t moore_phase_mode);
+
+/* Copy the full two-level state in (out of) the reference layout; values
+ * are converted to (from) the storage precision.  Replaces the ic.txt
+ * reader (evolution.cpp:102-136) and finalField writer
+ * (evolution.cpp:1447-1459).  n must equal ax_state_size(). */
+int ax_set_state(ax_lattice *lat, const double *host, size_t n);
+int ax_get_state(ax_lattice *lat, double *host, size_t n);
+int ax_set_timestep(ax_lattice *lat, int32_t timestep);
+int ax_get_timestep(const ax_lattice *lat, int32_t *timestep);
+
+/* Advance nsteps TimeSteps: the fused in-place update of
+ * mpi_evolution.cpp:381-501 (AH) or mooreEvol.cpp:184-337 + 647-694
+ * (Moore) over the whole lattice, periodic wrap in the index math instead
+ * of the MPI halo exchange (mpi_evolution.cpp:925-943).  Enqueued only. */
+int ax_step(ax_lattice *lat, int32_t nsteps);
+/* Wait for all work enqueued on the handle. */
+int ax_sync(ax_lattice *lat);
+
+/* Energy, Gauss-law deviation, finiteness and string counts of the current
+ * state (evolution.cpp:386-427 + 1404-1408; Moore mooreEvol.cpp:352-389).
+ * Synchronous.  want_winding = 0 skips the plaquette pass. */
+int ax_diagnostics(ax_lattice *lat, int32_t want_winding, ax_diag *out);
+
+/* Per-plaquette winding numbers of the current state, 3*nx*ny*nz int8 in
+ * [plane][x][y][z] order, plane 0 = xy, 1 = xz, 2 = yz (plaquette spanned
+ * from site s along the two axes).  No counterpart in the reference; see
+ * the library documentation for the definition. */
+int ax_winding(ax_lattice *lat, int8_t *out, size_t n);
+
+/* Face crossings of phi = 0 from the bilinear-interpolation test of
+ * mpi_evolution.cpp:536-789 / evolution.cpp:723-1016, in the reference's
+ * order (site x-major, then faces xy, xz, yz, then + root before - root).
+ * Writes min(count, cap) points as x,y,z triples (3*cap doubles) and the
+ * total count to *count. */
+int ax_crossings(ax_lattice *lat, double *xyz, size_t cap, size_t *count);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* AXIONS_AXIONS_H */

@@ -1,0 +1,696 @@
+/*
+ * oracle.cpp - TEST INFRASTRUCTURE ONLY.  CPU restatement of the reference's
+ * lattice evolution, used as the checker for the HIP library.  Only tests/,
+ * __graft_entry__.smoke() and the cpu_baseline leg of bench.py may load it;
+ * the product (axions_amd/) never links or calls anything under oracle/.
+ *
+ * Every arithmetic expression keeps the reference's operand order and
+ * association (the file must be built with -ffp-contract=off), so results
+ * are bit-identical to the reference given the same sin/cos.  pow(x,2) in
+ * the reference is written x*x here (GCC folds pow(x,2.0) to x*x).
+ *
+ * Sources restated (file:line in sjcott/Axions):
+ *   mt19937 + uniform_real_distribution IC ... mpi_evolution.cpp:287-312
+ *   AH fused in-place step ................... mpi_evolution.cpp:381-501
+ *   AH two-pass theta friction form .......... evolution.cpp:371-373
+ *   AH energy / Gauss (OpenMP form) .......... evolution.cpp:298-427, 1404-1408
+ *   AH energy (MPI form) ..................... mpi_evolution.cpp:509-532
+ *   face crossings ........................... mpi_evolution.cpp:536-789
+ *   Moore step, energy, Gauss ................ mooreEvol.cpp:184-395, 647-697
+ *   periodic neighbour indices ............... mpi_evolution.cpp:385-405
+ * The per-plaquette winding number has no reference counterpart.
+ *
+ * Pinning (what checks this file, see tests/): compiled reference runs in
+ * tests/golden (bit-exact state hashes and sampled sites for AH g=0 and g=1,
+ * and for Moore with fixed edges), the std::mt19937 known-answer value, the
+ * libstdc++ <random> routines themselves, and invariants (vacuum, lattice
+ * translation, gauge covariance, straight string winding, cube sums).
+ */
+#ifndef _GNU_SOURCE
+#define _GNU_SOURCE
+#endif
+#include <math.h>
+#include <stdint.h>
+#include <stddef.h>
+#include <string.h>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#include "axions/axions.h"
+#include "axions/ax_trig.h"
+
+namespace {
+
+struct TrigAx {
+  static double sn(double x) { return ax_sin(x); }
+  static double cs(double x) { return ax_cos(x); }
+};
+struct TrigLibm {
+  static double sn(double x) { return ::sin(x); }
+  static double cs(double x) { return ::cos(x); }
+};
+
+/* ---------------- mt19937 (C++ [rand.eng.mers], 32-bit parameters) -------- */
+struct MT {
+  uint32_t s[624];
+  int idx;
+  explicit MT(uint32_t seed) {
+    s[0] = seed;
+    for (int i = 1; i < 624; ++i)
+      s[i] = 1812433253u * (s[i - 1] ^ (s[i - 1] >> 30)) + (uint32_t)i;
+    idx = 624;
+  }
+  void twist() {
+    for (int i = 0; i < 624; ++i) {
+      uint32_t y = (s[i] & 0x80000000u) | (s[(i + 1) % 624] & 0x7fffffffu);
+      uint32_t v = s[(i + 397) % 624] ^ (y >> 1);
+      if (y & 1u) v ^= 0x9908b0dfu;
+      s[i] = v;
+    }
+    idx = 0;
+  }
+  uint32_t next() {
+    if (idx >= 624) twist();
+    uint32_t y = s[idx++];
+    y ^= (y >> 11);
+    y ^= (y << 7) & 0x9d2c5680u;
+    y ^= (y << 15) & 0xefc60000u;
+    y ^= (y >> 18);
+    return y;
+  }
+  /* generate_canonical<double,53> with a 32-bit engine: two draws,
+   * sum = d0 + d1*2^32 (rounded), result sum/2^64, clamped below 1. */
+  double canonical() {
+    double sum = 0.0, tmp = 1.0;
+    for (int k = 0; k < 2; ++k) {
+      sum += (double)next() * tmp;
+      tmp *= 4294967296.0;
+    }
+    double r = sum / tmp;
+    if (r >= 1.0) r = nextafter(1.0, 0.0);
+    return r;
+  }
+};
+
+struct Lat {
+  int nx, ny, nz;
+  size_t V;
+  explicit Lat(const ax_params *p) : nx(p->nx), ny(p->ny), nz(p->nz) {
+    V = (size_t)nx * ny * nz;
+  }
+  size_t id(int i, int j, int k) const { return ((size_t)i * ny + j) * nz + k; }
+};
+
+inline double *fld(double *st, const Lat &L, int comp, int lev) {
+  return st + ((size_t)comp * 2 + lev) * L.V;
+}
+inline const double *fld(const double *st, const Lat &L, int comp, int lev) {
+  return st + ((size_t)comp * 2 + lev) * L.V;
+}
+
+/* neighbour indices of site (i,j,k); periodic wrap as mpi_evolution.cpp:385-405 */
+struct Nb {
+  int i, j, k, ip, im, jp, jm, kp, km;
+  Nb(const Lat &L, int i_, int j_, int k_) : i(i_), j(j_), k(k_) {
+    ip = (i + 1) % L.nx; im = (i - 1 + L.nx) % L.nx;
+    jp = (j + 1) % L.ny; jm = (j - 1 + L.ny) % L.ny;
+    kp = (k + 1) % L.nz; km = (k - 1 + L.nz) % L.nz;
+  }
+};
+
+/* ------------------------------ AH step ---------------------------------- */
+template <class T>
+void ah_site(const ax_params *p, const Lat &L, double *st, int tN, int tP, const Nb &n,
+             double damp, double fric) {
+  const double *pr = fld(st, L, 0, tN), *pi = fld(st, L, 1, tN);
+  const double *tx = fld(st, L, 2, tN), *ty = fld(st, L, 3, tN), *tz = fld(st, L, 4, tN);
+  double *prP = fld(st, L, 0, tP), *piP = fld(st, L, 1, tP);
+  double *txP = fld(st, L, 2, tP), *tyP = fld(st, L, 3, tP), *tzP = fld(st, L, 4, tP);
+  const size_t s = L.id(n.i, n.j, n.k);
+  const size_t sxp = L.id(n.ip, n.j, n.k), sxm = L.id(n.im, n.j, n.k);
+  const size_t syp = L.id(n.i, n.jp, n.k), sym = L.id(n.i, n.jm, n.k);
+  const size_t szp = L.id(n.i, n.j, n.kp), szm = L.id(n.i, n.j, n.km);
+  const size_t s_xp_ym = L.id(n.ip, n.jm, n.k), s_xp_zm = L.id(n.ip, n.j, n.km);
+  const size_t s_xm_yp = L.id(n.im, n.jp, n.k), s_yp_zm = L.id(n.i, n.jp, n.km);
+  const size_t s_xm_zp = L.id(n.im, n.j, n.kp), s_ym_zp = L.id(n.i, n.jm, n.kp);
+  const double dx = p->dx, dy = p->dy, dz = p->dz, dt = p->dt;
+  const double *ph[2] = {pr, pi};
+  double *phP[2] = {prP, piP};
+  const double sg[2] = {1.0, -1.0};
+
+  double mag2 = pr[s] * pr[s] + pi[s] * pi[s];
+  for (int c = 0; c < 2; ++c) {
+    const double *a = ph[c], *b = ph[1 - c];
+    double lxx = (T::cs(tx[s]) * a[sxp] + sg[c] * T::sn(tx[s]) * b[sxp] - 2 * a[s] +
+                  T::cs(tx[sxm]) * a[sxm] - sg[c] * T::sn(tx[sxm]) * b[sxm]) / (dx * dx);
+    double lyy = (T::cs(ty[s]) * a[syp] + sg[c] * T::sn(ty[s]) * b[syp] - 2 * a[s] +
+                  T::cs(ty[sym]) * a[sym] - sg[c] * T::sn(ty[sym]) * b[sym]) / (dy * dy);
+    double lzz = (T::cs(tz[s]) * a[szp] + sg[c] * T::sn(tz[s]) * b[szp] - 2 * a[s] +
+                  T::cs(tz[szm]) * a[szm] - sg[c] * T::sn(tz[szm]) * b[szm]) / (dz * dz);
+    double vel = (a[s] - phP[c][s]) / dt;
+    double acc = lxx + lyy + lzz - 0.5 * p->lambda * (mag2 - p->eta * p->eta) * a[s] - fric * vel;
+    phP[c][s] = 2 * a[s] - phP[c][s] + dt * dt * acc;
+  }
+  double jx = T::cs(tx[s]) * (pi[s] * pr[sxp] - pr[s] * pi[sxp]) +
+              T::sn(tx[s]) * (pr[s] * pr[sxp] + pi[s] * pi[sxp]);
+  double jy = T::cs(ty[s]) * (pi[s] * pr[syp] - pr[s] * pi[syp]) +
+              T::sn(ty[s]) * (pr[s] * pr[syp] + pi[s] * pi[syp]);
+  double jz = T::cs(tz[s]) * (pi[s] * pr[szp] - pr[s] * pi[szp]) +
+              T::sn(tz[s]) * (pr[s] * pr[szp] + pi[s] * pi[szp]);
+  double fxy = (T::sn(tx[s] + ty[sxp] - tx[syp] - ty[s]) -
+                T::sn(tx[sym] + ty[s_xp_ym] - tx[s] - ty[sym])) / (dy * dy);
+  double fxz = (T::sn(tx[s] + tz[sxp] - tx[szp] - tz[s]) -
+                T::sn(tx[szm] + tz[s_xp_zm] - tx[s] - tz[szm])) / (dz * dz);
+  double fyx = (T::sn(ty[s] + tx[syp] - ty[sxp] - tx[s]) -
+                T::sn(ty[sxm] + tx[s_xm_yp] - ty[s] - tx[sxm])) / (dx * dx);
+  double fyz = (T::sn(ty[s] + tz[syp] - ty[szp] - tz[s]) -
+                T::sn(ty[szm] + tz[s_yp_zm] - ty[s] - tz[szm])) / (dz * dz);
+  double fzx = (T::sn(tz[s] + tx[szp] - tz[sxp] - tx[s]) -
+                T::sn(tz[sxm] + tx[s_xm_zp] - tz[s] - tx[sxm])) / (dx * dx);
+  double fzy = (T::sn(tz[s] + ty[szp] - tz[syp] - ty[s]) -
+                T::sn(tz[sym] + ty[s_ym_zp] - tz[s] - ty[sym])) / (dy * dy);
+  const double g = p->g;
+  double ax_, ay_, az_;
+  if (p->theta_damp_form == 0) {
+    double vx = (tx[s] - txP[s]) / dt, vy = (ty[s] - tyP[s]) / dt, vz = (tz[s] - tzP[s]) / dt;
+    ax_ = -2 * g * g * jx - fxy - fxz - damp * vx;
+    ay_ = -2 * g * g * jy - fyx - fyz - damp * vy;
+    az_ = -2 * g * g * jz - fzx - fzy - damp * vz;
+  } else {
+    ax_ = -2 * g * g * jx - fxy - fxz - damp * (tx[s] - txP[s]) / dt;
+    ay_ = -2 * g * g * jy - fyx - fyz - damp * (ty[s] - tyP[s]) / dt;
+    az_ = -2 * g * g * jz - fzx - fzy - damp * (tz[s] - tzP[s]) / dt;
+  }
+  txP[s] = 2 * tx[s] - txP[s] + dt * dt * ax_;
+  tyP[s] = 2 * ty[s] - tyP[s] + dt * dt * ay_;
+  tzP[s] = 2 * tz[s] - tzP[s] + dt * dt * az_;
+}
+
+/* ------------------------------ Moore step ------------------------------- */
+template <class T>
+void moore_site(const ax_params *p, const Lat &L, double *st, int tN, int tP, const Nb &n) {
+  const double *f1[2] = {fld(st, L, 0, tN), fld(st, L, 1, tN)};
+  const double *f2[2] = {fld(st, L, 2, tN), fld(st, L, 3, tN)};
+  const double *tx = fld(st, L, 4, tN), *ty = fld(st, L, 5, tN), *tz = fld(st, L, 6, tN);
+  double *f1P[2] = {fld(st, L, 0, tP), fld(st, L, 1, tP)};
+  double *f2P[2] = {fld(st, L, 2, tP), fld(st, L, 3, tP)};
+  double *txP = fld(st, L, 4, tP), *tyP = fld(st, L, 5, tP), *tzP = fld(st, L, 6, tP);
+  const size_t s = L.id(n.i, n.j, n.k);
+  const size_t sxp = L.id(n.ip, n.j, n.k), sxm = L.id(n.im, n.j, n.k);
+  const size_t syp = L.id(n.i, n.jp, n.k), sym = L.id(n.i, n.jm, n.k);
+  const size_t szp = L.id(n.i, n.j, n.kp), szm = L.id(n.i, n.j, n.km);
+  const size_t s_xp_ym = L.id(n.ip, n.jm, n.k), s_xp_zm = L.id(n.ip, n.j, n.km);
+  const size_t s_xm_yp = L.id(n.im, n.jp, n.k), s_yp_zm = L.id(n.i, n.jp, n.km);
+  const size_t s_xm_zp = L.id(n.im, n.j, n.kp), s_ym_zp = L.id(n.i, n.jm, n.kp);
+  const double dx = p->dx, dy = p->dy, dz = p->dz, dt = p->dt;
+  const int q1 = p->q1, q2 = p->q2;
+  const double sg[2] = {1.0, -1.0};
+  double m1 = f1[0][s] * f1[0][s] + f1[1][s] * f1[1][s];
+  double m2 = f2[0][s] * f2[0][s] + f2[1][s] * f2[1][s];
+  double acc1[2], acc2[2];
+  for (int c = 0; c < 2; ++c) {
+    const double *a = f1[c], *b = f1[1 - c], *u = f2[c], *w = f2[1 - c];
+    double l1x = (T::cs(q1 * tx[s]) * a[sxp] + sg[c] * T::sn(q1 * tx[s]) * b[sxp] - 2 * a[s] +
+                  T::cs(q1 * tx[sxm]) * a[sxm] - sg[c] * T::sn(q1 * tx[sxm]) * b[sxm]) / (dx * dx);
+    double l2x = (T::cs(q2 * tx[s]) * u[sxp] + sg[c] * T::sn(q2 * tx[s]) * w[sxp] - 2 * u[s] +
+                  T::cs(q2 * tx[sxm]) * u[sxm] - sg[c] * T::sn(q2 * tx[sxm]) * w[sxm]) / (dx * dx);
+    double l1y = (T::cs(q1 * ty[s]) * a[syp] + sg[c] * T::sn(q1 * ty[s]) * b[syp] - 2 * a[s] +
+                  T::cs(q1 * ty[sym]) * a[sym] - sg[c] * T::sn(q1 * ty[sym]) * b[sym]) / (dy * dy);
+    double l2y = (T::cs(q2 * ty[s]) * u[syp] + sg[c] * T::sn(q2 * ty[s]) * w[syp] - 2 * u[s] +
+                  T::cs(q2 * ty[sym]) * u[sym] - sg[c] * T::sn(q2 * ty[sym]) * w[sym]) / (dy * dy);
+    double l1z = (T::cs(q1 * tz[s]) * a[szp] + sg[c] * T::sn(q1 * tz[s]) * b[szp] - 2 * a[s] +
+                  T::cs(q1 * tz[szm]) * a[szm] - sg[c] * T::sn(q1 * tz[szm]) * b[szm]) / (dz * dz);
+    double l2z = (T::cs(q2 * tz[s]) * u[szp] + sg[c] * T::sn(q2 * tz[s]) * w[szp] - 2 * u[s] +
+                  T::cs(q2 * tz[szm]) * u[szm] - sg[c] * T::sn(q2 * tz[szm]) * w[szm]) / (dz * dz);
+    acc1[c] = l1x + l1y + l1z -
+              (0.5 * p->lambda1 * (m1 - p->eta1 * p->eta1) + 0.5 * p->lambda12 * (m2 - p->eta2 * p->eta2)) * a[s];
+    acc2[c] = l2x + l2y + l2z -
+              (0.5 * p->lambda2 * (m2 - p->eta2 * p->eta2) + 0.5 * p->lambda12 * (m1 - p->eta1 * p->eta1)) * u[s];
+  }
+  const double *r1 = f1[0], *i1 = f1[1], *r2 = f2[0], *i2 = f2[1];
+  double j1x = q1 * (T::cs(q1 * tx[s]) * (i1[s] * r1[sxp] - r1[s] * i1[sxp]) +
+                     T::sn(q1 * tx[s]) * (r1[s] * r1[sxp] + i1[s] * i1[sxp]));
+  double j1y = q1 * (T::cs(q1 * ty[s]) * (i1[s] * r1[syp] - r1[s] * i1[syp]) +
+                     T::sn(q1 * ty[s]) * (r1[s] * r1[syp] + i1[s] * i1[syp]));
+  double j1z = q1 * (T::cs(q1 * tz[s]) * (i1[s] * r1[szp] - r1[s] * i1[szp]) +
+                     T::sn(q1 * tz[s]) * (r1[s] * r1[szp] + i1[s] * i1[szp]));
+  double j2x = q2 * (T::cs(q2 * tx[s]) * (i2[s] * r2[sxp] - r2[s] * i2[sxp]) +
+                     T::sn(q2 * tx[s]) * (r2[s] * r2[sxp] + i2[s] * i2[sxp]));
+  double j2y = q2 * (T::cs(q2 * ty[s]) * (i2[s] * r2[syp] - r2[s] * i2[syp]) +
+                     T::sn(q2 * ty[s]) * (r2[s] * r2[syp] + i2[s] * i2[syp]));
+  double j2z = q2 * (T::cs(q2 * tz[s]) * (i2[s] * r2[szp] - r2[s] * i2[szp]) +
+                     T::sn(q2 * tz[s]) * (r2[s] * r2[szp] + i2[s] * i2[szp]));
+  double fxy = (T::sn(tx[s] + ty[sxp] - tx[syp] - ty[s]) -
+                T::sn(tx[sym] + ty[s_xp_ym] - tx[s] - ty[sym])) / (dy * dy);
+  double fxz = (T::sn(tx[s] + tz[sxp] - tx[szp] - tz[s]) -
+                T::sn(tx[szm] + tz[s_xp_zm] - tx[s] - tz[szm])) / (dz * dz);
+  double fyx = (T::sn(ty[s] + tx[syp] - ty[sxp] - tx[s]) -
+                T::sn(ty[sxm] + tx[s_xm_yp] - ty[s] - tx[sxm])) / (dx * dx);
+  double fyz = (T::sn(ty[s] + tz[syp] - ty[szp] - tz[s]) -
+                T::sn(ty[szm] + tz[s_yp_zm] - ty[s] - tz[szm])) / (dz * dz);
+  double fzx = (T::sn(tz[s] + tx[szp] - tz[sxp] - tx[s]) -
+                T::sn(tz[sxm] + tx[s_xm_zp] - tz[s] - tx[sxm])) / (dx * dx);
+  double fzy = (T::sn(tz[s] + ty[szp] - tz[syp] - ty[s]) -
+                T::sn(tz[sym] + ty[s_ym_zp] - tz[s] - ty[sym])) / (dy * dy);
+  const double g = p->g;
+  double atx = -2 * g * g * (j1x + j2x) - fxy - fxz;
+  double aty = -2 * g * g * (j1y + j2y) - fyx - fyz;
+  double atz = -2 * g * g * (j1z + j2z) - fzx - fzy;
+  for (int c = 0; c < 2; ++c) {
+    f1P[c][s] = 2 * f1[c][s] - f1P[c][s] + dt * dt * acc1[c];
+    f2P[c][s] = 2 * f2[c][s] - f2P[c][s] + dt * dt * acc2[c];
+  }
+  txP[s] = 2 * tx[s] - txP[s] + dt * dt * atx;
+  tyP[s] = 2 * ty[s] - tyP[s] + dt * dt * aty;
+  tzP[s] = 2 * tz[s] - tzP[s] + dt * dt * atz;
+}
+
+template <class T>
+void run_steps(const ax_params *p, double *st, int ts0, int nsteps, int nthreads) {
+  Lat L(p);
+  const int e = (p->boundary == AX_BC_FIXED) ? 1 : 0;
+  for (int n = ts0; n < ts0 + nsteps; ++n) {
+    const int tN = (n + 1) % 2, tP = n % 2;
+    const double time = 1 + n * p->dt;
+    const double damp = (n < p->damped_nt) ? p->dampFac : 0;
+    const double fric = damp + p->alpha * p->scaling / time;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static) num_threads(nthreads > 0 ? nthreads : 1)
+#endif
+    for (int i = e; i < L.nx - e; ++i)
+      for (int j = e; j < L.ny - e; ++j)
+        for (int k = e; k < L.nz - e; ++k) {
+          Nb nb(L, i, j, k);
+          if (p->model == AX_MODEL_AH)
+            ah_site<T>(p, L, st, tN, tP, nb, damp, fric);
+          else
+            moore_site<T>(p, L, st, tN, tP, nb);
+        }
+  }
+}
+
+/* ---------------------------- diagnostics -------------------------------- */
+/* out[0] energy, out[1] mean |G| (periodic: sum/(nx*ny)/nz; fixed edges:
+ * mooreEvol.cpp's running sum of |G|/((nx-2)(ny-2)(nz-2)) before its second
+ * division), out[2] 1.0 if every stored value is finite. */
+template <class T>
+void diag_ah(const ax_params *p, const double *st, int n, double *out) {
+  Lat L(p);
+  const int tN = (n + 1) % 2, tP = n % 2;
+  const double *f[2] = {fld(st, L, 0, tN), fld(st, L, 1, tN)};
+  const double *fP[2] = {fld(st, L, 0, tP), fld(st, L, 1, tP)};
+  const double *th[3] = {fld(st, L, 2, tN), fld(st, L, 3, tN), fld(st, L, 4, tN)};
+  const double *thP[3] = {fld(st, L, 2, tP), fld(st, L, 3, tP), fld(st, L, 4, tP)};
+  const double dx = p->dx, dy = p->dy, dz = p->dz, dt = p->dt, g = p->g, lam = p->lambda, eta = p->eta;
+  const double sg[2] = {1.0, -1.0};
+  double energy = 0, dev = 0;
+  for (int i = 0; i < L.nx; ++i)
+    for (int j = 0; j < L.ny; ++j)
+      for (int k = 0; k < L.nz; ++k) {
+        Nb n_(L, i, j, k);
+        const size_t s = L.id(i, j, k);
+        const size_t sxp = L.id(n_.ip, j, k), sxm = L.id(n_.im, j, k);
+        const size_t syp = L.id(i, n_.jp, k), sym = L.id(i, n_.jm, k);
+        const size_t szp = L.id(i, j, n_.kp), szm = L.id(i, j, n_.km);
+        double mag2 = f[0][s] * f[0][s] + f[1][s] * f[1][s];
+        double vel[2], gx[2], gy[2], gz[2];
+        for (int c = 0; c < 2; ++c) {
+          const double *a = f[c], *b = f[1 - c];
+          vel[c] = (a[s] - fP[c][s]) / dt;
+          gx[c] = (a[s] - T::cs(th[0][sxm]) * a[sxm] + sg[c] * T::sn(th[0][sxm]) * b[sxm]) / dx;
+          gy[c] = (a[s] - T::cs(th[1][sym]) * a[sym] + sg[c] * T::sn(th[1][sym]) * b[sym]) / dy;
+          gz[c] = (a[s] - T::cs(th[2][szm]) * a[szm] + sg[c] * T::sn(th[2][szm]) * b[szm]) / dz;
+        }
+        double pxy = th[0][s] + th[1][sxp] - th[0][syp] - th[1][s];
+        double pxz = th[0][s] + th[2][sxp] - th[0][szp] - th[2][s];
+        double pyz = th[1][s] + th[2][syp] - th[1][szp] - th[2][s];
+        double tdc, fcont, G;
+        if (p->energy_form == 0) {
+          if (g == 0) {
+            G = 0; tdc = 0; fcont = 0;
+          } else {
+            double divN = (th[0][s] - th[0][sxm]) / (dx * dx) + (th[1][s] - th[1][sym]) / (dy * dy) +
+                          (th[2][s] - th[2][szm]) / (dz * dz);
+            double divP = (thP[0][s] - thP[0][sxm]) / (dx * dx) + (thP[1][s] - thP[1][sym]) / (dy * dy) +
+                          (thP[2][s] - thP[2][szm]) / (dz * dz);
+            double divT = (divN - divP) / dt;
+            G = divT / g - 2 * g * (f[1][s] * vel[0] - f[0][s] * vel[1]);
+            double ex = (th[0][s] - thP[0][s]) / (g * dt * dx);
+            double ey = (th[1][s] - thP[1][s]) / (g * dt * dy);
+            double ez = (th[2][s] - thP[2][s]) / (g * dt * dz);
+            tdc = 0.5 * (ex * ex + ey * ey + ez * ez);
+            double bxy = (1 - T::cs(pxy)) / ((g * dx * dy) * (g * dx * dy));
+            double bxz = (1 - T::cs(pxz)) / ((g * dx * dz) * (g * dx * dz));
+            double byz = (1 - T::cs(pyz)) / ((g * dy * dz) * (g * dy * dz));
+            fcont = bxy + bxz + byz;
+          }
+          dev += fabs(G);
+          double vv = mag2 - eta * eta;
+          double rho = vel[0] * vel[0] + vel[1] * vel[1] + gx[0] * gx[0] + gx[1] * gx[1] +
+                       gy[0] * gy[0] + gy[1] * gy[1] + gz[0] * gz[0] + gz[1] * gz[1] + tdc + fcont +
+                       0.25 * lam * (vv * vv);
+          energy += dx * dy * dz * rho;
+        } else {
+          double ex = (th[0][s] - thP[0][s]) / dt;
+          double ey = (th[1][s] - thP[1][s]) / dt;
+          double ez = (th[2][s] - thP[2][s]) / dt;
+          tdc = 0.5 * (ex * ex + ey * ey + ez * ez);
+          double bxy = (1 - T::cs(pxy)) / ((g * dx * dy) * (g * dx * dy));
+          double bxz = (1 - T::cs(pxz)) / ((g * dx * dz) * (g * dx * dz));
+          double byz = (1 - T::cs(pyz)) / ((g * dy * dz) * (g * dy * dz));
+          fcont = bxy + bxz + byz;
+          double vv = mag2 - eta * eta;
+          energy += (vel[0] * vel[0] + vel[1] * vel[1] + gx[0] * gx[0] + gx[1] * gx[1] +
+                     gy[0] * gy[0] + gy[1] * gy[1] + gz[0] * gz[0] + gz[1] * gz[1] + tdc + fcont +
+                     0.25 * lam * (vv * vv)) * dx * dy * dz;
+        }
+      }
+  dev = dev / (L.nx * L.ny);
+  dev = dev / L.nz;
+  out[0] = energy;
+  out[1] = (p->energy_form == 0) ? dev : 0.0;
+}
+
+template <class T>
+void diag_moore(const ax_params *p, const double *st, int n, double *out) {
+  Lat L(p);
+  const int tN = (n + 1) % 2, tP = n % 2;
+  const double *f1[2] = {fld(st, L, 0, tN), fld(st, L, 1, tN)};
+  const double *f2[2] = {fld(st, L, 2, tN), fld(st, L, 3, tN)};
+  const double *f1P[2] = {fld(st, L, 0, tP), fld(st, L, 1, tP)};
+  const double *f2P[2] = {fld(st, L, 2, tP), fld(st, L, 3, tP)};
+  const double *th[3] = {fld(st, L, 4, tN), fld(st, L, 5, tN), fld(st, L, 6, tN)};
+  const double *thP[3] = {fld(st, L, 4, tP), fld(st, L, 5, tP), fld(st, L, 6, tP)};
+  const double dx = p->dx, dy = p->dy, dz = p->dz, dt = p->dt, g = p->g;
+  const int q1 = p->q1, q2 = p->q2;
+  const double sg[2] = {1.0, -1.0};
+  const bool fixed = p->boundary == AX_BC_FIXED;
+  const int e = fixed ? 1 : 0;
+  const int inner = (L.nx - 2) * (L.ny - 2) * (L.nz - 2);
+  double energy = 0, dev = 0;
+  for (int i = e; i < L.nx - e; ++i)
+    for (int j = e; j < L.ny - e; ++j)
+      for (int k = e; k < L.nz - e; ++k) {
+        Nb n_(L, i, j, k);
+        const size_t s = L.id(i, j, k);
+        const size_t sxp = L.id(n_.ip, j, k), sxm = L.id(n_.im, j, k);
+        const size_t syp = L.id(i, n_.jp, k), sym = L.id(i, n_.jm, k);
+        const size_t szp = L.id(i, j, n_.kp), szm = L.id(i, j, n_.km);
+        double m1 = f1[0][s] * f1[0][s] + f1[1][s] * f1[1][s];
+        double m2 = f2[0][s] * f2[0][s] + f2[1][s] * f2[1][s];
+        double v1[2], v2[2], g1x[2], g1y[2], g1z[2], g2x[2], g2y[2], g2z[2];
+        for (int c = 0; c < 2; ++c) {
+          const double *a = f1[c], *b = f1[1 - c], *u = f2[c], *w = f2[1 - c];
+          g1x[c] = (a[s] - T::cs(q1 * th[0][sxm]) * a[sxm] + sg[c] * T::sn(q1 * th[0][sxm]) * b[sxm]) / dx;
+          g1y[c] = (a[s] - T::cs(q1 * th[1][sym]) * a[sym] + sg[c] * T::sn(q1 * th[1][sym]) * b[sym]) / dy;
+          g2x[c] = (u[s] - T::cs(q2 * th[0][sxm]) * u[sxm] + sg[c] * T::sn(q2 * th[0][sxm]) * w[sxm]) / dx;
+          g2y[c] = (u[s] - T::cs(q2 * th[1][sym]) * u[sym] + sg[c] * T::sn(q2 * th[1][sym]) * w[sym]) / dy;
+          g1z[c] = (a[s] - T::cs(q1 * th[2][szm]) * a[szm] + sg[c] * T::sn(q1 * th[2][szm]) * b[szm]) / dz;
+          g2z[c] = (u[s] - T::cs(q2 * th[2][szm]) * u[szm] + sg[c] * T::sn(q2 * th[2][szm]) * w[szm]) / dz;
+          v1[c] = (a[s] - f1P[c][s]) / dt;
+          v2[c] = (u[s] - f2P[c][s]) / dt;
+        }
+        double tdc, fcont, G;
+        if (g == 0) {
+          G = 0; tdc = 0; fcont = 0;
+        } else {
+          double divN = (th[0][s] - th[0][sxm]) / (dx * dx) + (th[1][s] - th[1][sym]) / (dy * dy) +
+                        (th[2][s] - th[2][szm]) / (dz * dz);
+          double divP = (thP[0][s] - thP[0][sxm]) / (dx * dx) + (thP[1][s] - thP[1][sym]) / (dy * dy) +
+                        (thP[2][s] - thP[2][szm]) / (dz * dz);
+          double divT = (divN - divP) / dt;
+          G = divT / g - 2 * g * (q1 * (f1[1][s] * v1[0] - f1[0][s] * v1[1]) +
+                                  q2 * (f2[1][s] * v2[0] - f2[0][s] * v2[1]));
+          double ex = (th[0][s] - thP[0][s]) / (g * dt * dx);
+          double ey = (th[1][s] - thP[1][s]) / (g * dt * dy);
+          double ez = (th[2][s] - thP[2][s]) / (g * dt * dz);
+          tdc = 0.5 * (ex * ex + ey * ey + ez * ez);
+          double bxy = (1 - T::cs(th[0][s] + th[1][sxp] - th[0][syp] - th[1][s])) / ((g * dx * dy) * (g * dx * dy));
+          double bxz = (1 - T::cs(th[0][s] + th[2][sxp] - th[0][szp] - th[2][s])) / ((g * dx * dz) * (g * dx * dz));
+          double byz = (1 - T::cs(th[1][s] + th[2][syp] - th[1][szp] - th[2][s])) / ((g * dy * dz) * (g * dy * dz));
+          fcont = bxy + bxz + byz;
+        }
+        if (fixed)
+          dev += fabs(G) / inner;
+        else
+          dev += fabs(G);
+        double w1 = m1 - p->eta1 * p->eta1, w2 = m2 - p->eta2 * p->eta2;
+        double rho = v1[0] * v1[0] + v1[1] * v1[1] + g1x[0] * g1x[0] + g1x[1] * g1x[1] + g1y[0] * g1y[0] +
+                     g1y[1] * g1y[1] + g1z[0] * g1z[0] + g1z[1] * g1z[1] + v2[0] * v2[0] + v2[1] * v2[1] +
+                     g2x[0] * g2x[0] + g2x[1] * g2x[1] + g2y[0] * g2y[0] + g2y[1] * g2y[1] + g2z[0] * g2z[0] +
+                     g2z[1] * g2z[1] + tdc + fcont + 0.25 * p->lambda1 * (w1 * w1) +
+                     0.25 * p->lambda2 * (w2 * w2) + 0.5 * p->lambda12 * w1 * w2;
+        energy += dx * dy * dz * rho;
+      }
+  if (!fixed) {
+    dev = dev / (L.nx * L.ny);
+    dev = dev / L.nz;
+  }
+  out[0] = energy;
+  out[1] = dev;
+}
+
+/* ------------------------------ winding ---------------------------------- */
+/* Gauge-invariant phase across the link s -> s+a:
+ * delta = arg( conj(phi(s)) * exp(-i theta_a(s)) * phi(s+a) ), the transport
+ * used in the covariant Laplacian (evolution.cpp:304-311). */
+inline double link_phase(double cr, double ci, double nr, double ni, double th) {
+  double c = ax_cos(th), sn = ax_sin(th);
+  double wr = c * nr + sn * ni;
+  double wi = c * ni - sn * nr;
+  double re = cr * wr + ci * wi;
+  double im = cr * wi - ci * wr;
+  return ax_atan2(im, re);
+}
+
+/* winding of plaquette (s; a, b): round((d_a(s) + d_b(s+a) - d_a(s+b) - d_b(s)
+ *   + P_ab(s)) / 2pi), P_ab(s) = theta_a(s) + theta_b(s+a) - theta_a(s+b)
+ *   - theta_b(s) in the reference's operand order (mpi_evolution.cpp:471-477). */
+inline int plaq_wind(double da0, double db1, double da2, double db3, double P) {
+  double sum = da0 + db1 - da2 - db3 + P;
+  return (int)rint(sum * 0.15915494309189535);
+}
+
+void winding_impl(const ax_params *p, const double *st, int n, int8_t *out, int field) {
+  Lat L(p);
+  const int tN = (n + 1) % 2;
+  const int c0 = (p->model == AX_MODEL_AH) ? 0 : 2 * field;
+  const int t0 = (p->model == AX_MODEL_AH) ? 2 : 4;
+  const double q = (p->model == AX_MODEL_AH) ? 1.0 : (double)(field == 0 ? p->q1 : p->q2);
+  const double *fr = fld(st, L, c0, tN), *fi = fld(st, L, c0 + 1, tN);
+  const double *th[3] = {fld(st, L, t0, tN), fld(st, L, t0 + 1, tN), fld(st, L, t0 + 2, tN)};
+  for (int i = 0; i < L.nx; ++i)
+    for (int j = 0; j < L.ny; ++j)
+      for (int k = 0; k < L.nz; ++k) {
+        Nb n_(L, i, j, k);
+        size_t s = L.id(i, j, k);
+        size_t sa[3] = {L.id(n_.ip, j, k), L.id(i, n_.jp, k), L.id(i, j, n_.kp)};
+        const int pa[3] = {0, 0, 1}, pb[3] = {1, 2, 2};
+        for (int pl = 0; pl < 3; ++pl) {
+          int a = pa[pl], b = pb[pl];
+          size_t s_a = sa[a], s_b = sa[b];
+          /* s + a + b */
+          int ii = i, jj = j, kk = k;
+          if (a == 0 || b == 0) ii = n_.ip;
+          if (a == 1 || b == 1) jj = n_.jp;
+          if (a == 2 || b == 2) kk = n_.kp;
+          size_t s_ab = L.id(ii, jj, kk);
+          double da0 = link_phase(fr[s], fi[s], fr[s_a], fi[s_a], q * th[a][s]);
+          double db1 = link_phase(fr[s_a], fi[s_a], fr[s_ab], fi[s_ab], q * th[b][s_a]);
+          double da2 = link_phase(fr[s_b], fi[s_b], fr[s_ab], fi[s_ab], q * th[a][s_b]);
+          double db3 = link_phase(fr[s], fi[s], fr[s_b], fi[s_b], q * th[b][s]);
+          double P = q * th[a][s] + q * th[b][s_a] - q * th[a][s_b] - q * th[b][s];
+          out[(size_t)pl * L.V + s] = (int8_t)plaq_wind(da0, db1, da2, db3, P);
+        }
+      }
+}
+
+/* ------------------------------ crossings -------------------------------- */
+struct Pts {
+  double *xyz;
+  size_t cap, n;
+  void push(double a, double b, double c) {
+    if (n < cap) {
+      xyz[3 * n] = a; xyz[3 * n + 1] = b; xyz[3 * n + 2] = c;
+    }
+    ++n;
+  }
+};
+
+/* One face: bilinear zero of (Re, Im) phi on corners v00=(u,w), v10=(u+du,w),
+ * v01=(u,w+dw), v11; returns roots as (sol1 along u, sol2 along w)
+ * (mpi_evolution.cpp:552-623). kind selects how the point is stored. */
+inline void face(const double v00[2], const double v10[2], const double v01[2], const double v11[2],
+                 double u, double du, double w, double dw, int kind, double fixedc, Pts &P) {
+  double c1[2], c2[2], c3[2], c4[2];
+  for (int c = 0; c < 2; ++c) {
+    c1[c] = v11[c] - v10[c] - v01[c] + v00[c];
+    c2[c] = (w + dw) * (v10[c] - v00[c]) + w * (v01[c] - v11[c]);
+    c3[c] = (u + du) * (v01[c] - v00[c]) + u * (v10[c] - v11[c]);
+    c4[c] = (u + du) * ((w + dw) * v00[c] - w * v01[c]) - u * ((w + dw) * v10[c] - w * v11[c]);
+  }
+  double a = c1[1] * c3[0] - c1[0] * c3[1];
+  double b = c1[1] * c4[0] + c2[1] * c3[0] - c1[0] * c4[1] - c2[0] * c3[1];
+  double c = c2[1] * c4[0] - c2[0] * c4[1];
+  double disc = b * b - 4 * a * c;
+  auto emit = [&](double s1, double s2) {
+    if (kind == 0) P.push(s1, s2, fixedc);        /* xy face: (x, y, z) */
+    else if (kind == 1) P.push(s1, fixedc, s2);   /* xz face: (x, y, z) */
+    else P.push(fixedc, s1, s2);                  /* yz face */
+  };
+  if (a == 0) {
+    if (b != 0) {
+      double s2 = -c / b;
+      if (s2 >= w && s2 <= w + dw) {
+        double s1 = -(c3[0] * s2 + c4[0]) / (c1[0] * s2 + c2[0]);
+        if (s1 >= u && s1 <= u + du) emit(s1, s2);
+      }
+    }
+  } else if (disc >= 0) {
+    double s2 = (-b + sqrt(disc)) / (2 * a);
+    if (s2 >= w && s2 <= w + dw) {
+      double s1 = -(c3[0] * s2 + c4[0]) / (c1[0] * s2 + c2[0]);
+      if (s1 >= u && s1 <= u + du) emit(s1, s2);
+    }
+    s2 = (-b - sqrt(disc)) / (2 * a);
+    if (s2 >= w && s2 <= w + dw) {
+      double s1 = -(c3[0] * s2 + c4[0]) / (c1[0] * s2 + c2[0]);
+      if (s1 >= u && s1 <= u + du) emit(s1, s2);
+    }
+  }
+}
+
+void crossings_impl(const ax_params *p, const double *st, int n, Pts &P) {
+  Lat L(p);
+  const int tN = (n + 1) % 2;
+  const double *f[2] = {fld(st, L, 0, tN), fld(st, L, 1, tN)};
+  const int x0 = int(0.5 * (L.nx - 1)), y0 = int(0.5 * (L.ny - 1)), z0 = int(0.5 * (L.nz - 1));
+  const double dx = p->dx, dy = p->dy, dz = p->dz;
+  for (int i = 0; i < L.nx; ++i)
+    for (int j = 0; j < L.ny; ++j)
+      for (int k = 0; k < L.nz; ++k) {
+        Nb n_(L, i, j, k);
+        size_t s = L.id(i, j, k), sx = L.id(n_.ip, j, k), sy = L.id(i, n_.jp, k), sz = L.id(i, j, n_.kp);
+        size_t sxy = L.id(n_.ip, n_.jp, k), sxz = L.id(n_.ip, j, n_.kp), syz = L.id(i, n_.jp, n_.kp);
+        double x = (i - x0) * dx, y = (j - y0) * dy, z = (k - z0) * dz;
+        double v[2], vx[2], vy[2], vz[2], vxy[2], vxz[2], vyz[2];
+        for (int c = 0; c < 2; ++c) {
+          v[c] = f[c][s]; vx[c] = f[c][sx]; vy[c] = f[c][sy]; vz[c] = f[c][sz];
+          vxy[c] = f[c][sxy]; vxz[c] = f[c][sxz]; vyz[c] = f[c][syz];
+        }
+        face(v, vx, vy, vxy, x, dx, y, dy, 0, z, P);
+        face(v, vx, vz, vxz, x, dx, z, dz, 1, y, P);
+        face(v, vy, vz, vyz, y, dy, z, dz, 2, x, P);
+      }
+}
+
+}  // namespace
+
+/* ================================ C ABI =================================== */
+extern "C" {
+
+int orc_version(void) { return 1; }
+
+void orc_mt19937(uint32_t seed, size_t n, uint32_t *out) {
+  MT m(seed);
+  for (size_t i = 0; i < n; ++i) out[i] = m.next();
+}
+
+void orc_phases(uint32_t seed, size_t n, double *out) {
+  MT m(seed);
+  const double pi = 4.0 * atan(1.0);
+  for (size_t i = 0; i < n; ++i) out[i] = m.canonical() * (2 * pi - 0.0) + 0.0;
+}
+
+/* phi = eta*(cos, sin)(phase) as mpi_evolution.cpp:304-305.  GCC at -O1 and
+ * above turns that cos/sin pair of one variable into a single glibc sincos()
+ * call (its results differ from separate cos/sin for ~0.1% of arguments), so
+ * sincos() is called explicitly to reproduce the compiled reference. */
+int orc_init_random_phase(const ax_params *p, uint32_t seed, int mode, double *st) {
+  Lat L(p);
+  MT m(seed);
+  const double pi = 4.0 * atan(1.0);
+  const int nc = (p->model == AX_MODEL_AH) ? 5 : 7;
+  memset(st, 0, sizeof(double) * (size_t)nc * 2 * L.V);
+  for (size_t s = 0; s < L.V; ++s) {
+    if (p->model == AX_MODEL_AH) {
+      double ph = m.canonical() * (2 * pi - 0.0) + 0.0;
+      double sn, cs;
+      sincos(ph, &sn, &cs);
+      double re = p->eta * cs, im = p->eta * sn;
+      for (int t = 0; t < 2; ++t) {
+        fld(st, L, 0, t)[s] = re;
+        fld(st, L, 1, t)[s] = im;
+      }
+    } else {
+      double p1 = m.canonical() * (2 * pi - 0.0) + 0.0;
+      double p2 = (mode == 1) ? p1 : m.canonical() * (2 * pi - 0.0) + 0.0;
+      double s1, c1, s2, c2;
+      sincos(p1, &s1, &c1);
+      sincos(p2, &s2, &c2);
+      double a = p->eta1 * c1, b = p->eta1 * s1;
+      double c = p->eta2 * c2, d = p->eta2 * s2;
+      for (int t = 0; t < 2; ++t) {
+        fld(st, L, 0, t)[s] = a; fld(st, L, 1, t)[s] = b;
+        fld(st, L, 2, t)[s] = c; fld(st, L, 3, t)[s] = d;
+      }
+    }
+  }
+  return 0;
+}
+
+/* trig_mode 0: the shared ax_trig.h functions; 1: libm sin/cos. */
+int orc_step(const ax_params *p, double *st, int ts0, int nsteps, int trig_mode, int nthreads) {
+  if (trig_mode == 1)
+    run_steps<TrigLibm>(p, st, ts0, nsteps, nthreads);
+  else
+    run_steps<TrigAx>(p, st, ts0, nsteps, nthreads);
+  return 0;
+}
+
+int orc_diag(const ax_params *p, const double *st, int n, int trig_mode, double *out) {
+  if (p->model == AX_MODEL_AH) {
+    if (trig_mode == 1) diag_ah<TrigLibm>(p, st, n, out);
+    else diag_ah<TrigAx>(p, st, n, out);
+  } else {
+    if (trig_mode == 1) diag_moore<TrigLibm>(p, st, n, out);
+    else diag_moore<TrigAx>(p, st, n, out);
+  }
+  Lat L(p);
+  const int nc = (p->model == AX_MODEL_AH) ? 5 : 7;
+  int fin = 1;
+  for (size_t i = 0; i < (size_t)nc * 2 * L.V; ++i)
+    if (!isfinite(st[i])) { fin = 0; break; }
+  out[2] = fin;
+  return 0;
+}
+
+/* field: 0 = phi (AH) / phi1 (Moore), 1 = phi2 (Moore) */
+int orc_winding(const ax_params *p, const double *st, int n, int field, int8_t *out) {
+  winding_impl(p, st, n, out, field);
+  return 0;
+}
+
+int orc_crossings(const ax_params *p, const double *st, int n, double *xyz, size_t cap, size_t *count) {
+  Pts P{xyz, cap, 0};
+  crossings_impl(p, st, n, P);
+  *count = P.n;
+  return 0;
+}
+
+/* fn: 0 sin, 1 cos, 2 atan2(x, y), 3 sinf, 4 cosf */
+void orc_trig(int fn, const double *x, const double *y, double *out, size_t n) {
+  for (size_t i = 0; i < n; ++i) {
+    switch (fn) {
+      case 0: out[i] = ax_sin(x[i]); break;
+      case 1: out[i] = ax_cos(x[i]); break;
+      case 2: out[i] = ax_atan2(x[i], y[i]); break;
+      case 3: { float s, c; ax_sincosf((float)x[i], &s, &c); out[i] = s; break; }
+      default: { float s, c; ax_sincosf((float)x[i], &s, &c); out[i] = c; break; }
+    }
+  }
+}
+
+}  // extern "C"

@@ -1,0 +1,96 @@
+"""TEST INFRASTRUCTURE ONLY: ctypes wrapper around oracle/liboracle.so, the
+CPU restatement of the reference (oracle/oracle.cpp).  Imported only by
+tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg."""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+from axions_amd.abi import AxParams, ncomp, state_len
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            build()
+        L = ctypes.CDLL(path)
+        P = ctypes.POINTER(AxParams)
+        vp = ctypes.c_void_p
+        L.orc_mt19937.argtypes = [ctypes.c_uint32, ctypes.c_size_t, vp]
+        L.orc_phases.argtypes = [ctypes.c_uint32, ctypes.c_size_t, vp]
+        L.orc_init_random_phase.argtypes = [P, ctypes.c_uint32, ctypes.c_int, vp]
+        L.orc_step.argtypes = [P, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        L.orc_diag.argtypes = [P, vp, ctypes.c_int, ctypes.c_int, vp]
+        L.orc_winding.argtypes = [P, vp, ctypes.c_int, ctypes.c_int, vp]
+        L.orc_crossings.argtypes = [P, vp, ctypes.c_int, vp, ctypes.c_size_t,
+                                    ctypes.POINTER(ctypes.c_size_t)]
+        L.orc_trig.argtypes = [ctypes.c_int, vp, vp, vp, ctypes.c_size_t]
+        _LIB = L
+    return _LIB
+
+
+def mt19937(seed, n):
+    out = np.empty(n, dtype=np.uint32)
+    lib().orc_mt19937(seed, n, out.ctypes.data)
+    return out
+
+
+def phases(seed, n):
+    out = np.empty(n, dtype=np.float64)
+    lib().orc_phases(seed, n, out.ctypes.data)
+    return out
+
+
+def init_random_phase(p, seed, mode=0):
+    st = np.empty(state_len(p), dtype=np.float64)
+    lib().orc_init_random_phase(ctypes.byref(p), seed, mode, st.ctypes.data)
+    return st
+
+
+def step(p, st, ts0, nsteps, trig_mode=0, nthreads=1):
+    assert st.dtype == np.float64 and st.flags.c_contiguous and st.size == state_len(p)
+    lib().orc_step(ctypes.byref(p), st.ctypes.data, ts0, nsteps, trig_mode, nthreads)
+
+
+def diag(p, st, n, trig_mode=0):
+    out = np.zeros(3)
+    lib().orc_diag(ctypes.byref(p), st.ctypes.data, n, trig_mode, out.ctypes.data)
+    return {"energy": out[0], "gauss": out[1], "finite": bool(out[2])}
+
+
+def winding(p, st, n, field=0):
+    out = np.empty(3 * p.nx * p.ny * p.nz, dtype=np.int8)
+    lib().orc_winding(ctypes.byref(p), st.ctypes.data, n, field, out.ctypes.data)
+    return out
+
+
+def crossings(p, st, n, cap=None):
+    cap = cap if cap is not None else 6 * p.nx * p.ny * p.nz
+    xyz = np.empty((max(cap, 1), 3), dtype=np.float64)
+    cnt = ctypes.c_size_t(0)
+    lib().orc_crossings(ctypes.byref(p), st.ctypes.data, n, xyz.ctypes.data, cap, ctypes.byref(cnt))
+    return xyz[:min(cnt.value, cap)].copy(), cnt.value
+
+
+def trig(fn, x, y=None):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.zeros_like(x) if y is None else np.ascontiguousarray(y, dtype=np.float64)
+    out = np.empty_like(x)
+    lib().orc_trig(fn, x.ctypes.data, y.ctypes.data, out.ctypes.data, x.size)
+    return out
+
+
+def levels(p, st, n):
+    """(current, previous) level views [comp][x][y][z] of the state after n steps."""
+    s = st.reshape(ncomp(p), 2, p.nx, p.ny, p.nz)
+    return s[:, (n + 1) % 2], s[:, n % 2]

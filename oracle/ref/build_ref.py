@@ -1,0 +1,117 @@
+#!/usr/bin/env python3
+"""Build the reference programs (sjcott/Axions) for the oracle.  TEST
+INFRASTRUCTURE ONLY.
+
+The reference keeps its run parameters as file-scope consts
+(evolution.cpp:23-57, mooreEvol.cpp:19-41) that cannot be overridden from
+the command line.  This recipe reads the reference source where it lies,
+substitutes the values of named consts in the text it pipes to g++ (nothing
+is copied to disk), and compiles it with the reference's own recipe
+(g++ -O3 -fopenmp, submit_job.sh:27-31) plus:
+  -fmacro-prefix-map  so the program's __FILE__-relative Data/ and GifData/
+                      paths (evolution.cpp:80-91) resolve in its run folder;
+  -include ref_shim.h full-precision ofstream output; optional shared trig.
+Outputs go to oracle/_ref/<name>/ only.  Every other byte of the compiled
+program is the reference's.
+
+Usage: build_ref.py [--ref /root/reference] [name ...]   (default: all)
+"""
+import argparse
+import os
+import re
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(os.path.dirname(HERE))
+OUT = os.path.join(REPO, "oracle", "_ref")
+
+# name -> (source file, const overrides, shared trig?)
+_AH_SMALL = dict(nx=16, ny=16, nz=16, nt=60, saveFreq=5, stationary_ic="true",
+                 finalOut="true", stringDetect="true", detectBuffer="true",
+                 makeStringPosGif="true")
+CONFIGS = {
+    # global string (g = 0) with PRS friction and early damping; theta stays
+    # exactly 0, so sin/cos only ever see 0 and libm is irrelevant.
+    "evo_ah_g0": ("evolution.cpp", dict(_AH_SMALL, dx=0.5, dy=0.5, dz=0.5, dt=0.05,
+                                        **{"lambda": 1}, eta=1, g=0, damped_nt=20,
+                                        dampFac=0.5, alpha=3, scaling=1), False),
+    # gauged, friction + damping, shared trig and libm trig
+    "evo_ah_g1_shared": ("evolution.cpp", dict(_AH_SMALL, dx=0.5, dy=0.5, dz=0.5, dt=0.1,
+                                               **{"lambda": 2}, eta=1, g=1, damped_nt=20,
+                                               dampFac=0.5, alpha=3, scaling=1), True),
+    "evo_ah_g1_libm": ("evolution.cpp", dict(_AH_SMALL, dx=0.5, dy=0.5, dz=0.5, dt=0.1,
+                                             **{"lambda": 2}, eta=1, g=1, damped_nt=20,
+                                             dampFac=0.5, alpha=3, scaling=1), False),
+    # non-cubic, non-power-of-two spacings (true divisions everywhere)
+    "evo_ah_aniso": ("evolution.cpp", dict(_AH_SMALL, nx=12, ny=10, nz=14, nt=40,
+                                           dx=0.7, dy=0.6, dz=0.55, dt=0.13,
+                                           **{"lambda": 1.5}, eta=0.9, g=0.8,
+                                           damped_nt=10, dampFac=0.3, alpha=2,
+                                           scaling=1), True),
+    # plain AH 64^3, dx=0.5 dt=0.1 lambda=2 g=eta=1, 200 steps
+    "evo_c1_shared": ("evolution.cpp", dict(nx=64, ny=64, nz=64, nt=200, dx=0.5, dy=0.5, dz=0.5,
+                                            dt=0.1, **{"lambda": 2}, eta=1, g=1, damped_nt=0,
+                                            dampFac=0.5, alpha=0, scaling=1, saveFreq=10,
+                                            stationary_ic="true", finalOut="true",
+                                            stringDetect="true", detectBuffer="true",
+                                            makeStringPosGif="true"), True),
+    # Moore model as shipped (fixed edges), reference couplings
+    "moore_shared": ("mooreEvol.cpp", dict(nx=12, ny=12, nz=12, nt=30, makeGif="false"), True),
+    "moore_libm": ("mooreEvol.cpp", dict(nx=12, ny=12, nz=12, nt=30, makeGif="false"), False),
+}
+
+
+def substituted_source(path, overrides):
+    text = open(path).read()
+    for name, val in overrides.items():
+        if isinstance(val, float) and val == int(val):
+            val = repr(val)
+        pat = re.compile(r"^(const\s+(?:int|double|bool)\s+" + re.escape(name) + r"\s*=\s*)([^;]*)(;)",
+                         re.M)
+        text, n = pat.subn(lambda m: m.group(1) + str(val) + m.group(3), text)
+        if n != 1:
+            raise SystemExit(f"{os.path.basename(path)}: const {name} matched {n} times")
+    return text
+
+
+def build(name, ref, cxx="g++"):
+    src, over, shared = CONFIGS[name]
+    path = os.path.join(ref, src)
+    dest = os.path.join(OUT, name)
+    os.makedirs(os.path.join(dest, "Data"), exist_ok=True)
+    os.makedirs(os.path.join(dest, "GifData"), exist_ok=True)
+    text = substituted_source(path, over)
+    cmd = [cxx, "-x", "c++", "-", "-x", "none", os.path.join(ref, "array.cpp"),
+           "-I", ref, "-I", os.path.join(REPO, "include"),
+           "-include", os.path.join(HERE, "ref_shim.h"),
+           f"-fmacro-prefix-map=<stdin>=./{src}",
+           "-O3", "-fopenmp", "-o", os.path.join(dest, "prog")]
+    if shared:
+        cmd.insert(1, "-DAX_REF_SHARED_TRIG")
+    r = subprocess.run(cmd, input=text, text=True, capture_output=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stderr)
+        raise SystemExit(f"build of {name} failed")
+    with open(os.path.join(dest, "overrides.txt"), "w") as f:
+        f.write(f"{src} shared_trig={shared}\n")
+        for k, v in over.items():
+            f.write(f"{k}={v}\n")
+    return dest
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--ref", default="/root/reference")
+    ap.add_argument("names", nargs="*")
+    a = ap.parse_args()
+    if not os.path.isdir(a.ref):
+        print("reference sources not present; nothing to build")
+        return 0
+    for n in (a.names or list(CONFIGS)):
+        print("built", build(n, a.ref))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,129 @@
+#!/usr/bin/env python3
+"""Write the golden fixtures in tests/golden/ from the reference programs.
+
+Runs in the build container only (the reference is not available elsewhere).
+Each program is compiled from sjcott/Axions by oracle/ref/build_ref.py into
+oracle/_ref/<name>/, fed the seeded random-phase initial condition of
+mpi_evolution.cpp:287-312 (written at 17 significant digits as Data/ic.txt,
+the 5-column stationary format of evolution.cpp:102-117, or the 7-column
+Data/moore_ic.txt of mooreEvol.cpp:73-91) and run single-threaded so its
+OpenMP energy/Gauss sums have a fixed order.  The fixture records, per
+program: energy and Gauss-deviation per TimeStep (valsPerLoop), the newest
+field level after the run (finalField.txt) as a SHA-256 plus sampled sites
+, and every string-crossing frame
+(GifData) as count + SHA-256 (+ full list when small).
+
+Usage: python tests/golden/make_golden.py [name ...]
+"""
+import hashlib
+import json
+import os
+import subprocess
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, REPO)
+
+from axions_amd.abi import ah_params, moore_params, AX_BC_FIXED  # noqa: E402
+from oracle import oracle_py as orc  # noqa: E402
+sys.path.insert(0, os.path.join(REPO, "oracle", "ref"))
+import build_ref  # noqa: E402
+
+SEED = 42
+
+# name -> (params builder, nt, saveFreq, detection start step (damped_nt), trig)
+SPECS = {
+    "evo_ah_g0": (lambda: ah_params(16, dx=0.5, dt=0.05, lam=1.0, eta=1.0, g=0.0, alpha=3.0,
+                                    damped_nt=20, dampFac=0.5, theta_damp_form=1), 60, 5, 20, "libm"),
+    "evo_ah_g1_shared": (lambda: ah_params(16, dx=0.5, dt=0.1, lam=2.0, eta=1.0, g=1.0, alpha=3.0,
+                                           damped_nt=20, dampFac=0.5, theta_damp_form=1), 60, 5, 20, "shared"),
+    "evo_ah_g1_libm": (lambda: ah_params(16, dx=0.5, dt=0.1, lam=2.0, eta=1.0, g=1.0, alpha=3.0,
+                                         damped_nt=20, dampFac=0.5, theta_damp_form=1), 60, 5, 20, "libm"),
+    "evo_ah_aniso": (lambda: ah_params(12, ny=10, nz=14, dx=0.7, dy=0.6, dz=0.55, dt=0.13, lam=1.5,
+                                       eta=0.9, g=0.8, alpha=2.0, damped_nt=10, dampFac=0.3,
+                                       theta_damp_form=1), 40, 5, 10, "shared"),
+    "evo_c1_shared": (lambda: ah_params(64, dx=0.5, dt=0.1, lam=2.0, eta=1.0, g=1.0, alpha=0.0,
+                                        damped_nt=0, dampFac=0.5, theta_damp_form=1), 200, 10, 0, "shared"),
+    "moore_shared": (lambda: moore_params(12, boundary=AX_BC_FIXED), 30, None, None, "shared"),
+    "moore_libm": (lambda: moore_params(12, boundary=AX_BC_FIXED), 30, None, None, "libm"),
+}
+
+
+def sha(a):
+    return hashlib.sha256(np.ascontiguousarray(a, dtype=np.float64).tobytes()).hexdigest()
+
+
+def params_dict(p):
+    return {f: getattr(p, f) for f, _ in p._fields_}
+
+
+def write_ic(p, st, path):
+    cur, _ = orc.levels(p, st, 0)
+    cols = cur.reshape(cur.shape[0], -1).T
+    np.savetxt(path, cols, fmt="%.17g")
+
+
+def run(name):
+    builder, nt, save, det0, trig = SPECS[name]
+    p = builder()
+    dest = os.path.join(REPO, "oracle", "_ref", name)
+    if not os.path.exists(os.path.join(dest, "prog")):
+        build_ref.build(name, "/root/reference")
+    moore = p.model == 1
+    st = orc.init_random_phase(p, SEED, 0)
+    write_ic(p, st, os.path.join(dest, "Data", "moore_ic.txt" if moore else "ic.txt"))
+    for d in ("Data", "GifData"):
+        for f in os.listdir(os.path.join(dest, d)):
+            if f.startswith(("valsPerLoop", "gifStringPos", "finalField")):
+                os.remove(os.path.join(dest, d, f))
+    env = dict(os.environ, OMP_NUM_THREADS="1")
+    subprocess.run(["./prog"], cwd=dest, input="golden\n", text=True, env=env, check=True,
+                   stdout=subprocess.DEVNULL)
+    vals = os.path.join(dest, "Data", "valsPerLoop.txt" if moore else "valsPerLoop_golden.txt")
+    energy, gauss = [], []
+    for line in open(vals):
+        f = line.split()
+        energy.append(float(f[0]))
+        gauss.append(float(f[1]))
+    assert len(energy) == nt, (name, len(energy))
+    fin = np.loadtxt(os.path.join(dest, "Data", "finalField.txt"), dtype=np.float64).T.copy()
+    fin = fin.reshape(fin.shape[0], p.nx, p.ny, p.nz)
+    rng = np.random.default_rng(7)
+    V = p.nx * p.ny * p.nz
+    idx = sorted(int(i) for i in rng.choice(V, size=min(64, V), replace=False))
+    flat = fin.reshape(fin.shape[0], -1)
+    fx = {
+        "name": name, "program": build_ref.CONFIGS[name][0], "trig": trig, "seed": SEED,
+        "nt": nt, "params": params_dict(p),
+        "energy": energy, "gauss": gauss,
+        "final_sha256": sha(fin), "final_sample_idx": idx,
+        "final_sample": flat[:, idx].T.tolist(),
+    }
+    frames = []
+    if save is not None:
+        frame = 0
+        for n in range(nt):
+            if n % save == 0 and n >= det0:
+                path = os.path.join(dest, "GifData", f"gifStringPosData_{frame}.txt")
+                pts = np.loadtxt(path, dtype=np.float64, ndmin=2) if os.path.getsize(path) else np.zeros((0, 3))
+                ent = {"step": n, "count": int(pts.shape[0]), "sha256": sha(pts)}
+                if pts.shape[0] <= 400:
+                    ent["points"] = pts.tolist()
+                frames.append(ent)
+                frame += 1
+    fx["frames"] = frames
+    with open(os.path.join(HERE, f"{name}.json"), "w") as f:
+        json.dump(fx, f)
+    print(name, "energy[0..2]", energy[:3], "frames", len(frames), "final", fx["final_sha256"][:12])
+
+
+def main():
+    for n in (sys.argv[1:] or list(SPECS)):
+        run(n)
+
+
+if __name__ == "__main__":
+    main()
