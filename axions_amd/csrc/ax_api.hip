@@ -1,0 +1,565 @@
+/*
+ * ax_api.hip - C ABI of libaxions.so (declared in include/axions/axions.h).
+ *
+ * Host side of one lattice: parameter checks, device allocation, the
+ * seeded initial condition (generated with libstdc++'s own std::mt19937 and
+ * uniform_real_distribution, exactly the calls of mpi_evolution.cpp:290-301),
+ * the TimeStep loop bookkeeping of mpi_evolution.cpp:354-371 and the
+ * diagnostic read-back.  Every kernel is enqueued on the handle's stream.
+ */
+#ifndef _GNU_SOURCE
+#define _GNU_SOURCE
+#endif
+#include <math.h>
+#include <stdarg.h>
+#include <string.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <new>
+#include <random>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "ax_device.h"
+#include "ax_internal.h"
+
+namespace ax {
+thread_local std::string g_last_error;
+
+int fail(int code, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_last_error = buf;
+  return code;
+}
+
+int hip_fail(hipError_t e, const char *what) {
+  return fail(AX_EHIP, "%s: %s", what, hipGetErrorString(e));
+}
+
+int check_params(const ax_params *p) {
+  if (!p) return fail(AX_EINVAL, "params is NULL");
+  if (p->model != AX_MODEL_AH && p->model != AX_MODEL_MOORE) return fail(AX_EINVAL, "model %d", p->model);
+  if (p->nx < 3 || p->ny < 3 || p->nz < 3) return fail(AX_EINVAL, "lattice %dx%dx%d: each axis needs >= 3 sites",
+                                                        p->nx, p->ny, p->nz);
+  if ((long long)p->nx * p->ny * p->nz > 2147483647LL / 2)
+    return fail(AX_EINVAL, "lattice %dx%dx%d exceeds 2^30 sites", p->nx, p->ny, p->nz);
+  if (!(p->dx > 0 && p->dy > 0 && p->dz > 0 && p->dt > 0)) return fail(AX_EINVAL, "spacings must be > 0");
+  if (p->precision != 64 && p->precision != 32) return fail(AX_EINVAL, "precision %d (64 or 32)", p->precision);
+  if (p->boundary != AX_BC_PERIODIC && p->boundary != AX_BC_FIXED) return fail(AX_EINVAL, "boundary %d", p->boundary);
+  if (p->theta_damp_form != 0 && p->theta_damp_form != 1) return fail(AX_EINVAL, "theta_damp_form");
+  if (p->energy_form != 0 && p->energy_form != 1) return fail(AX_EINVAL, "energy_form");
+  return AX_OK;
+}
+
+StepConsts step_consts(const ax_params *p, int n) {
+  StepConsts k;
+  k.nx = p->nx; k.ny = p->ny; k.nz = p->nz;
+  k.edge = p->boundary == AX_BC_FIXED;
+  k.dx2 = p->dx * p->dx; k.dy2 = p->dy * p->dy; k.dz2 = p->dz * p->dz;
+  k.dt = p->dt; k.dt2 = p->dt * p->dt;
+  k.hl = 0.5 * p->lambda; k.eta2 = p->eta * p->eta;
+  k.m2gg = -2 * p->g * p->g;
+  const double time = 1 + n * p->dt;                     /* mpi_evolution.cpp:356 */
+  k.damp = (n < p->damped_nt) ? p->dampFac : 0;         /* :367-368 */
+  k.fric = k.damp + p->alpha * p->scaling / time;
+  k.theta_damp_form = p->theta_damp_form;
+  k.hl1 = 0.5 * p->lambda1; k.hl2 = 0.5 * p->lambda2; k.hl12 = 0.5 * p->lambda12;
+  k.eta1sq = p->eta1 * p->eta1; k.eta2sq = p->eta2 * p->eta2;
+  k.q1 = p->q1; k.q2 = p->q2;
+  return k;
+}
+
+DiagConsts diag_consts(const ax_params *p) {
+  DiagConsts k;
+  k.nx = p->nx; k.ny = p->ny; k.nz = p->nz;
+  k.edge = p->boundary == AX_BC_FIXED;
+  k.model = p->model;
+  k.energy_form = p->energy_form;
+  k.ncomp = p->model == AX_MODEL_AH ? 5 : 7;
+  k.dx = p->dx; k.dy = p->dy; k.dz = p->dz; k.dt = p->dt; k.g = p->g;
+  k.lambda = p->lambda; k.eta = p->eta;
+  k.lambda1 = p->lambda1; k.lambda2 = p->lambda2; k.lambda12 = p->lambda12;
+  k.eta1 = p->eta1; k.eta2 = p->eta2; k.q1 = p->q1; k.q2 = p->q2;
+  k.inner = (double)((p->nx - 2) * (p->ny - 2) * (p->nz - 2));
+  return k;
+}
+
+}  // namespace ax
+
+using namespace ax;
+
+struct ax_lattice {
+  ax_params p;
+  int ncomp = 0;
+  size_t V = 0;
+  size_t esize = 0;
+  int timestep = 0;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  void *state = nullptr;
+  double *part = nullptr;   /* 3 doubles per 256-site block */
+  double *dres = nullptr;   /* small device results */
+  double *hres = nullptr;   /* pinned host copy */
+  int8_t *wind = nullptr;
+  int *bcount = nullptr;
+  long long *boff = nullptr;
+  long long *ctotal = nullptr;
+  double *cxyz = nullptr;
+  size_t cxyz_cap = 0;
+  bool timing = false;                         /* ax_kernel_timer */
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
+  int64_t timed_steps = 0;
+};
+
+namespace ax {
+
+template <typename T>
+Levels<T> levels_of(ax_lattice *L, int n) {
+  Levels<T> lv;
+  T *base = (T *)L->state;
+  const int tN = (n + 1) % 2, tP = n % 2;
+  for (int c = 0; c < 7; ++c) {
+    int cc = c < L->ncomp ? c : 0;
+    lv.cur[c] = base + ((size_t)cc * 2 + tN) * L->V;
+    lv.prv[c] = base + ((size_t)cc * 2 + tP) * L->V;
+  }
+  return lv;
+}
+template Levels<double> levels_of<double>(ax_lattice *, int);
+template Levels<float> levels_of<float>(ax_lattice *, int);
+
+hipStream_t lattice_stream(ax_lattice *L) { return L->stream; }
+const ax_params &lattice_params(const ax_lattice *L) { return L->p; }
+void *lattice_state(ax_lattice *L) { return L->state; }
+size_t lattice_sites(const ax_lattice *L) { return L->V; }
+
+}  // namespace ax
+
+extern "C" {
+
+void ax_default_params(ax_params *p) {
+  if (!p) return;
+  memset(p, 0, sizeof *p);
+  p->model = AX_MODEL_AH;
+  p->nx = p->ny = p->nz = 64;
+  p->dx = p->dy = p->dz = 0.5;
+  p->dt = 0.1;
+  p->lambda = 2; p->eta = 1; p->g = 1;
+  p->lambda1 = 2; p->lambda2 = 2; p->lambda12 = 0; p->eta1 = 1; p->eta2 = 1;
+  p->q1 = 2; p->q2 = 1;
+  p->alpha = 0; p->scaling = 1; p->damped_nt = 0; p->dampFac = 0.5;
+  p->theta_damp_form = 0; p->energy_form = 0;
+  p->boundary = AX_BC_PERIODIC; p->precision = 64;
+}
+
+const char *ax_strerror(int code) {
+  switch (code) {
+    case AX_OK: return "ok";
+    case AX_EINVAL: return "invalid argument";
+    case AX_ENOMEM: return "out of memory";
+    case AX_EBUDGET: return "HBM budget exceeded";
+    case AX_EHIP: return "HIP runtime error";
+    case AX_ENONFINITE: return "non-finite value in lattice state";
+    case AX_ENOTFOUND: return "unknown request";
+    case AX_ENODEV: return "no GPU";
+    default: return "unknown error";
+  }
+}
+
+const char *ax_last_error(void) { return g_last_error.c_str(); }
+
+int ax_state_size(const ax_params *p, int32_t *ncomp, size_t *n) {
+  int rc = check_params(p);
+  if (rc) return rc;
+  int nc = p->model == AX_MODEL_AH ? 5 : 7;
+  if (ncomp) *ncomp = nc;
+  if (n) *n = (size_t)nc * 2 * p->nx * p->ny * p->nz;
+  return AX_OK;
+}
+
+int ax_create(const ax_params *p, ax_lattice **out) {
+  if (!out) return fail(AX_EINVAL, "out is NULL");
+  *out = nullptr;
+  int rc = check_params(p);
+  if (rc) return rc;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(AX_ENODEV, "no HIP device");
+  ax_lattice *L = new (std::nothrow) ax_lattice;
+  if (!L) return fail(AX_ENOMEM, "host allocation");
+  L->p = *p;
+  L->ncomp = p->model == AX_MODEL_AH ? 5 : 7;
+  L->V = (size_t)p->nx * p->ny * p->nz;
+  L->esize = p->precision == 64 ? 8 : 4;
+  hipGetDevice(&L->device);
+  hipError_t e;
+  if ((e = hipStreamCreateWithFlags(&L->stream, hipStreamNonBlocking)) != hipSuccess) {
+    delete L;
+    return hip_fail(e, "hipStreamCreate");
+  }
+  const size_t bytes = (size_t)L->ncomp * 2 * L->V * L->esize;
+  const size_t nb = (L->V + 255) / 256;
+  if (hipMalloc(&L->state, bytes) != hipSuccess || hipMalloc((void **)&L->part, 3 * nb * sizeof(double)) != hipSuccess ||
+      hipMalloc((void **)&L->dres, 16 * sizeof(double)) != hipSuccess ||
+      hipHostMalloc((void **)&L->hres, 16 * sizeof(double), hipHostMallocDefault) != hipSuccess) {
+    hipGetLastError();
+    ax_destroy(L);
+    return fail(AX_ENOMEM, "device allocation of %zu bytes failed", bytes);
+  }
+  if ((e = hipMemsetAsync(L->state, 0, bytes, L->stream)) != hipSuccess) {
+    ax_destroy(L);
+    return hip_fail(e, "hipMemset");
+  }
+  hipStreamSynchronize(L->stream);
+  *out = L;
+  return AX_OK;
+}
+
+void ax_destroy(ax_lattice *L) {
+  if (!L) return;
+  if (L->stream) hipStreamSynchronize(L->stream);
+  for (auto &pr : L->ev) {
+    hipEventDestroy(pr.first);
+    hipEventDestroy(pr.second);
+  }
+  if (L->state) hipFree(L->state);
+  if (L->part) hipFree(L->part);
+  if (L->dres) hipFree(L->dres);
+  if (L->hres) hipHostFree(L->hres);
+  if (L->wind) hipFree(L->wind);
+  if (L->bcount) hipFree(L->bcount);
+  if (L->boff) hipFree(L->boff);
+  if (L->ctotal) hipFree(L->ctotal);
+  if (L->cxyz) hipFree(L->cxyz);
+  if (L->stream) hipStreamDestroy(L->stream);
+  delete L;
+}
+
+int ax_get_params(const ax_lattice *L, ax_params *out) {
+  if (!L || !out) return fail(AX_EINVAL, "NULL argument");
+  *out = L->p;
+  return AX_OK;
+}
+
+int ax_set_timestep(ax_lattice *L, int32_t ts) {
+  if (!L || ts < 0) return fail(AX_EINVAL, "bad timestep");
+  L->timestep = ts;
+  return AX_OK;
+}
+
+int ax_get_timestep(const ax_lattice *L, int32_t *ts) {
+  if (!L || !ts) return fail(AX_EINVAL, "NULL argument");
+  *ts = L->timestep;
+  return AX_OK;
+}
+
+int ax_sync(ax_lattice *L) {
+  if (!L) return fail(AX_EINVAL, "NULL lattice");
+  hipError_t e = hipStreamSynchronize(L->stream);
+  return e == hipSuccess ? AX_OK : hip_fail(e, "hipStreamSynchronize");
+}
+
+/* Upload one field plane (V values) from doubles into level lev of comp. */
+static int upload_plane(ax_lattice *L, int comp, int lev, const double *src, std::vector<float> &tmp) {
+  char *dst = (char *)L->state + ((size_t)comp * 2 + lev) * L->V * L->esize;
+  hipError_t e;
+  if (L->esize == 8) {
+    e = hipMemcpyAsync(dst, src, L->V * 8, hipMemcpyHostToDevice, L->stream);
+  } else {
+    tmp.resize(L->V);
+    for (size_t i = 0; i < L->V; ++i) tmp[i] = (float)src[i];
+    e = hipMemcpyAsync(dst, tmp.data(), L->V * 4, hipMemcpyHostToDevice, L->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(L->stream);
+  }
+  return e == hipSuccess ? AX_OK : hip_fail(e, "upload");
+}
+
+int ax_set_state(ax_lattice *L, const double *host, size_t n) {
+  if (!L || !host) return fail(AX_EINVAL, "NULL argument");
+  if (n != (size_t)L->ncomp * 2 * L->V) return fail(AX_EINVAL, "state length %zu != %zu", n, (size_t)L->ncomp * 2 * L->V);
+  std::vector<float> tmp;
+  for (int c = 0; c < L->ncomp; ++c)
+    for (int t = 0; t < 2; ++t) {
+      int rc = upload_plane(L, c, t, host + ((size_t)c * 2 + t) * L->V, tmp);
+      if (rc) return rc;
+    }
+  hipError_t e = hipStreamSynchronize(L->stream);
+  return e == hipSuccess ? AX_OK : hip_fail(e, "set_state");
+}
+
+int ax_get_state(ax_lattice *L, double *host, size_t n) {
+  if (!L || !host) return fail(AX_EINVAL, "NULL argument");
+  if (n != (size_t)L->ncomp * 2 * L->V) return fail(AX_EINVAL, "state length %zu != %zu", n, (size_t)L->ncomp * 2 * L->V);
+  hipError_t e;
+  if (L->esize == 8) {
+    e = hipMemcpyAsync(host, L->state, n * 8, hipMemcpyDeviceToHost, L->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(L->stream);
+  } else {
+    std::vector<float> tmp(n);
+    e = hipMemcpyAsync(tmp.data(), L->state, n * 4, hipMemcpyDeviceToHost, L->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(L->stream);
+    for (size_t i = 0; i < n; ++i) host[i] = tmp[i];
+  }
+  return e == hipSuccess ? AX_OK : hip_fail(e, "get_state");
+}
+
+/* Fill dst[i] = a*cos(ph[i]), dst2[i] = a*sin(ph[i]) with glibc sincos on
+ * several host threads (the reference's cos/sin pair of one variable,
+ * mpi_evolution.cpp:304-305, is one sincos() call after GCC's -O1+ fusion). */
+static void fill_sincos(const std::vector<double> &ph, size_t stride, size_t off, double amp, std::vector<double> &re,
+                        std::vector<double> &im) {
+  const size_t V = re.size();
+  unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  std::vector<std::thread> th;
+  for (unsigned t = 0; t < nt; ++t)
+    th.emplace_back([&, t] {
+      size_t a = V * t / nt, b = V * (t + 1) / nt;
+      for (size_t i = a; i < b; ++i) {
+        double s, c;
+        sincos(ph[i * stride + off], &s, &c);
+        re[i] = amp * c;
+        im[i] = amp * s;
+      }
+    });
+  for (auto &x : th) x.join();
+}
+
+int ax_random_phases(uint32_t seed, size_t n, double *out) {
+  if (!out && n) return fail(AX_EINVAL, "NULL output");
+  const double pi = 4.0 * atan(1.0);
+  std::mt19937 gen(seed);
+  std::uniform_real_distribution<double> dist(0.0, 2 * pi);
+  for (size_t i = 0; i < n; ++i) out[i] = dist(gen);
+  return AX_OK;
+}
+
+int ax_init_random_phase(ax_lattice *L, uint32_t seed, int32_t mode) {
+  if (!L) return fail(AX_EINVAL, "NULL lattice");
+  if (mode != 0 && mode != 1) return fail(AX_EINVAL, "moore_phase_mode %d", mode);
+  const double pi = 4.0 * atan(1.0);                       /* mpi_evolution.cpp:12 */
+  std::mt19937 gen(seed);
+  std::uniform_real_distribution<double> dist(0.0, 2 * pi);
+  const bool moore = L->p.model == AX_MODEL_MOORE;
+  const size_t per = (moore && mode == 0) ? 2 : 1;
+  std::vector<double> ph(L->V * per);
+  for (size_t i = 0; i < ph.size(); ++i) ph[i] = dist(gen);
+  std::vector<double> re(L->V), im(L->V);
+  std::vector<float> tmp;
+  hipError_t e = hipMemsetAsync(L->state, 0, (size_t)L->ncomp * 2 * L->V * L->esize, L->stream);
+  if (e != hipSuccess) return hip_fail(e, "hipMemset");
+  const int nfield = moore ? 2 : 1;
+  for (int f = 0; f < nfield; ++f) {
+    double amp = !moore ? L->p.eta : (f == 0 ? L->p.eta1 : L->p.eta2);
+    fill_sincos(ph, per, (per == 2) ? (size_t)f : 0, amp, re, im);
+    for (int t = 0; t < 2; ++t) {
+      int rc = upload_plane(L, 2 * f, t, re.data(), tmp);
+      if (!rc) rc = upload_plane(L, 2 * f + 1, t, im.data(), tmp);
+      if (rc) return rc;
+    }
+    e = hipStreamSynchronize(L->stream);  /* re/im are reused for the next field */
+    if (e != hipSuccess) return hip_fail(e, "init");
+  }
+  L->timestep = 0;
+  return AX_OK;
+}
+
+int ax_step(ax_lattice *L, int32_t nsteps) {
+  if (!L || nsteps < 0) return fail(AX_EINVAL, "bad step request");
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (L->timing && nsteps > 0) {
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return fail(AX_EHIP, "event");
+    hipEventRecord(e0, L->stream);
+  }
+  for (int s = 0; s < nsteps; ++s) {
+    int rc = launch_step(L, L->timestep);
+    if (rc) return rc;
+    L->timestep++;
+  }
+  if (e0) {
+    hipEventRecord(e1, L->stream);
+    L->ev.emplace_back(e0, e1);
+    L->timed_steps += nsteps;
+  }
+  return AX_OK;
+}
+
+int ax_kernel_timer(ax_lattice *L, int32_t enable) {
+  if (!L) return fail(AX_EINVAL, "NULL lattice");
+  L->timing = enable != 0;
+  return AX_OK;
+}
+
+int ax_kernel_time(ax_lattice *L, double *ms, int64_t *steps) {
+  if (!L || !ms || !steps) return fail(AX_EINVAL, "NULL argument");
+  hipError_t e = hipStreamSynchronize(L->stream);
+  if (e != hipSuccess) return hip_fail(e, "sync");
+  double tot = 0;
+  for (auto &pr : L->ev) {
+    float t = 0;
+    hipEventElapsedTime(&t, pr.first, pr.second);
+    tot += t;
+    hipEventDestroy(pr.first);
+    hipEventDestroy(pr.second);
+  }
+  L->ev.clear();
+  *ms = tot;
+  *steps = L->timed_steps;
+  L->timed_steps = 0;
+  return AX_OK;
+}
+
+int ax_diagnostics(ax_lattice *L, int32_t want_winding, ax_diag *out) {
+  if (!L || !out) return fail(AX_EINVAL, "NULL argument");
+  const ax_params &p = L->p;
+  DiagConsts k = diag_consts(&p);
+  hipError_t e;
+  if (L->esize == 8)
+    e = launch_diag<double>(k, levels_of<double>(L, L->timestep), L->part, L->dres, L->stream);
+  else
+    e = launch_diag<float>(k, levels_of<float>(L, L->timestep), L->part, L->dres, L->stream);
+  if (e != hipSuccess) return hip_fail(e, "diag launch");
+  if (want_winding) {
+    int rc = ensure_winding_buffer(L);
+    if (rc) return rc;
+    rc = launch_winding_field(L, 0, L->wind, L->part, L->dres + 4);
+    if (rc) return rc;
+  }
+  e = hipMemcpyAsync(L->hres, L->dres, 8 * sizeof(double), hipMemcpyDeviceToHost, L->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(L->stream);
+  if (e != hipSuccess) return hip_fail(e, "diag readback");
+  memset(out, 0, sizeof *out);
+  out->timestep = L->timestep;
+  out->energy = L->hres[0];
+  double dev = L->hres[1];
+  if (p.boundary == AX_BC_PERIODIC) {
+    dev = dev / (p.nx * p.ny);                             /* evolution.cpp:1404-1408 */
+    dev = dev / p.nz;
+  }
+  out->gauss_mean = (p.model == AX_MODEL_AH && p.energy_form == 1) ? 0.0 : dev;
+  out->finite = L->hres[2] == 0 ? 1 : 0;
+  if (want_winding) {
+    const double wxy = L->hres[4], wxz = L->hres[5], wyz = L->hres[6];
+    out->wound_plaquettes = (int64_t)(wxy + wxz + wyz);
+    out->string_length = (2.0 / 3.0) * (wxy * p.dz + wxz * p.dy + wyz * p.dx);
+    out->xi = out->string_length / ((double)L->V * p.dx * p.dy * p.dz);
+  }
+  return AX_OK;
+}
+
+int ax_winding(ax_lattice *L, int8_t *out, size_t n) { return ax_winding_field(L, 0, out, n); }
+
+int ax_winding_field(ax_lattice *L, int32_t field, int8_t *out, size_t n) {
+  if (!L || !out) return fail(AX_EINVAL, "NULL argument");
+  if (n != 3 * L->V) return fail(AX_EINVAL, "winding length %zu != %zu", n, 3 * L->V);
+  if (field < 0 || field > (L->p.model == AX_MODEL_MOORE ? 1 : 0)) return fail(AX_EINVAL, "field %d", field);
+  int rc = ensure_winding_buffer(L);
+  if (rc) return rc;
+  rc = launch_winding_field(L, field, L->wind, L->part, L->dres + 4);
+  if (rc) return rc;
+  hipError_t e = hipMemcpyAsync(out, L->wind, n, hipMemcpyDeviceToHost, L->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(L->stream);
+  return e == hipSuccess ? AX_OK : hip_fail(e, "winding readback");
+}
+
+int ax_crossings(ax_lattice *L, double *xyz, size_t cap, size_t *count) {
+  if (!L || !count || (cap && !xyz)) return fail(AX_EINVAL, "NULL argument");
+  const ax_params &p = L->p;
+  const size_t nb = (L->V + 255) / 256;
+  hipError_t e = hipSuccess;
+  if (!L->bcount) {
+    if (hipMalloc((void **)&L->bcount, nb * sizeof(int)) != hipSuccess ||
+        hipMalloc((void **)&L->boff, nb * sizeof(long long)) != hipSuccess ||
+        hipMalloc((void **)&L->ctotal, sizeof(long long)) != hipSuccess) {
+      hipGetLastError();
+      return fail(AX_ENOMEM, "crossing scratch");
+    }
+  }
+  if (cap > L->cxyz_cap) {
+    if (L->cxyz) hipFree(L->cxyz);
+    L->cxyz = nullptr;
+    L->cxyz_cap = 0;
+    if (hipMalloc((void **)&L->cxyz, cap * 3 * sizeof(double)) != hipSuccess) {
+      hipGetLastError();
+      return fail(AX_ENOMEM, "crossing output of %zu points", cap);
+    }
+    L->cxyz_cap = cap;
+  }
+  const int tN = (L->timestep + 1) % 2;
+  if (L->esize == 8) {
+    const double *b = (const double *)L->state;
+    e = launch_crossings<double>(p.nx, p.ny, p.nz, p.dx, p.dy, p.dz, b + (size_t)tN * L->V, b + (size_t)(2 + tN) * L->V,
+                                 L->bcount, L->boff, L->ctotal, L->cxyz, (long long)cap, L->stream, cap > 0);
+  } else {
+    const float *b = (const float *)L->state;
+    e = launch_crossings<float>(p.nx, p.ny, p.nz, p.dx, p.dy, p.dz, b + (size_t)tN * L->V, b + (size_t)(2 + tN) * L->V,
+                                L->bcount, L->boff, L->ctotal, L->cxyz, (long long)cap, L->stream, cap > 0);
+  }
+  if (e != hipSuccess) return hip_fail(e, "crossing launch");
+  long long tot = 0;
+  e = hipMemcpyAsync(&tot, L->ctotal, sizeof tot, hipMemcpyDeviceToHost, L->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(L->stream);
+  if (e != hipSuccess) return hip_fail(e, "crossing count");
+  *count = (size_t)tot;
+  size_t nw = std::min(cap, (size_t)tot);
+  if (nw) {
+    e = hipMemcpyAsync(xyz, L->cxyz, nw * 3 * sizeof(double), hipMemcpyDeviceToHost, L->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(L->stream);
+    if (e != hipSuccess) return hip_fail(e, "crossing readback");
+  }
+  return AX_OK;
+}
+
+}  // extern "C"
+
+namespace ax {
+
+int ensure_winding_buffer(ax_lattice *L) {
+  if (L->wind) return AX_OK;
+  if (hipMalloc((void **)&L->wind, 3 * L->V) != hipSuccess) {
+    hipGetLastError();
+    return fail(AX_ENOMEM, "winding buffer");
+  }
+  return AX_OK;
+}
+
+int launch_winding_field(ax_lattice *L, int field, int8_t *out, double *part, double *res3) {
+  const ax_params &p = L->p;
+  const int tN = (L->timestep + 1) % 2;
+  const int c0 = p.model == AX_MODEL_AH ? 0 : 2 * field;
+  const int t0 = p.model == AX_MODEL_AH ? 2 : 4;
+  const double q = p.model == AX_MODEL_AH ? 1.0 : (double)(field == 0 ? p.q1 : p.q2);
+  hipError_t e;
+  auto P = [&](int c) { return (size_t)(c * 2 + tN) * L->V; };
+  if (L->esize == 8) {
+    const double *b = (const double *)L->state;
+    e = launch_winding<double>(p.nx, p.ny, p.nz, b + P(c0), b + P(c0 + 1), b + P(t0), b + P(t0 + 1), b + P(t0 + 2), q,
+                               out, part, res3, L->stream);
+  } else {
+    const float *b = (const float *)L->state;
+    e = launch_winding<float>(p.nx, p.ny, p.nz, b + P(c0), b + P(c0 + 1), b + P(t0), b + P(t0 + 1), b + P(t0 + 2), q,
+                              out, part, res3, L->stream);
+  }
+  return e == hipSuccess ? AX_OK : hip_fail(e, "winding launch");
+}
+
+int launch_step(ax_lattice *L, int n) {
+  StepConsts k = step_consts(&L->p, n);
+  hipError_t e;
+  if (L->esize == 8) {
+    Levels<double> lv = levels_of<double>(L, n);
+    if (use_tiled_ah(L->p))
+      e = launch_step_tiled_ah(k, lv, L->stream);
+    else
+      e = launch_step_site<double>(k, lv, L->p.model, L->stream);
+  } else {
+    e = launch_step_site<float>(k, levels_of<float>(L, n), L->p.model, L->stream);
+  }
+  return e == hipSuccess ? AX_OK : hip_fail(e, "step launch");
+}
+
+}  // namespace ax

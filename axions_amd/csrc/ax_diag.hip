@@ -1,0 +1,468 @@
+/*
+ * ax_diag.hip - read-only diagnostic passes over one lattice state.
+ *
+ *  diag_kernel      energy density and Gauss-law deviation per site
+ *                   (evolution.cpp:298-427, Moore mooreEvol.cpp:259-389) in
+ *                   FP64 whatever the storage type, plus a finiteness flag
+ *                   over every stored value; per-block FP64 partial sums are
+ *                   reduced by one block in a fixed order, so the result is
+ *                   deterministic for a given lattice shape.
+ *  winding_kernel   per-plaquette winding numbers (no reference
+ *                   counterpart) and their absolute sum.
+ *  crossing kernels the bilinear face-crossing test of
+ *                   mpi_evolution.cpp:536-789, compacted in site order.
+ *
+ * The diagnostics must read the current level of every neighbour and the
+ * previous level (the Gauss law needs the neighbours' previous level,
+ * evolution.cpp:390-391), so they run as a separate pass before the in-place
+ * update of the same TimeStep.
+ */
+#include "ax_internal.h"
+
+namespace ax {
+
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+/* Block-wide sum of 256 values in a fixed pairing order; valid in thread 0. */
+__device__ __forceinline__ double block_sum256(double v, double *sh) {
+  v = wave_sum(v);
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  __syncthreads();
+  if (l == 0) sh[w] = v;
+  __syncthreads();
+  double r = 0;
+  if (threadIdx.x == 0) r = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+  return r;
+}
+
+template <typename T>
+__device__ __forceinline__ void site_energy_ah(const DiagConsts &k, const Levels<T> &L, int i, int j, int kk,
+                                               double &rho, double &absG) {
+  const int nx = k.nx, ny = k.ny, nz = k.nz;
+  auto id = [&](int a, int b, int c) { return (a * ny + b) * nz + c; };
+  const int s = id(i, j, kk);
+  const int sxp = id(wrapp(i, nx), j, kk), sxm = id(wrapm(i, nx), j, kk);
+  const int syp = id(i, wrapp(j, ny), kk), sym = id(i, wrapm(j, ny), kk);
+  const int szp = id(i, j, wrapp(kk, nz)), szm = id(i, j, wrapm(kk, nz));
+  const double dx = k.dx, dy = k.dy, dz = k.dz, dt = k.dt, g = k.g;
+  const double pr = ld(L.cur[0], s), pi = ld(L.cur[1], s);
+  const double mag2 = pr * pr + pi * pi;
+  double vel[2], gx[2], gy[2], gz[2];
+  double sxm_, cxm_, sym_, cym_, szm_, czm_;
+  ax_sincos(ld(L.cur[2], sxm), &sxm_, &cxm_);
+  ax_sincos(ld(L.cur[3], sym), &sym_, &cym_);
+  ax_sincos(ld(L.cur[4], szm), &szm_, &czm_);
+  {
+    vel[0] = (pr - ld(L.prv[0], s)) / dt;
+    gx[0] = (pr - cxm_ * ld(L.cur[0], sxm) + sxm_ * ld(L.cur[1], sxm)) / dx;
+    gy[0] = (pr - cym_ * ld(L.cur[0], sym) + sym_ * ld(L.cur[1], sym)) / dy;
+    gz[0] = (pr - czm_ * ld(L.cur[0], szm) + szm_ * ld(L.cur[1], szm)) / dz;
+    vel[1] = (pi - ld(L.prv[1], s)) / dt;
+    gx[1] = (pi - cxm_ * ld(L.cur[1], sxm) + (-sxm_) * ld(L.cur[0], sxm)) / dx;
+    gy[1] = (pi - cym_ * ld(L.cur[1], sym) + (-sym_) * ld(L.cur[0], sym)) / dy;
+    gz[1] = (pi - czm_ * ld(L.cur[1], szm) + (-szm_) * ld(L.cur[0], szm)) / dz;
+  }
+  const double X = ld(L.cur[2], s), Y = ld(L.cur[3], s), Z = ld(L.cur[4], s);
+  const double pxy = X + ld(L.cur[3], sxp) - ld(L.cur[2], syp) - Y;
+  const double pxz = X + ld(L.cur[4], sxp) - ld(L.cur[2], szp) - Z;
+  const double pyz = Y + ld(L.cur[4], syp) - ld(L.cur[3], szp) - Z;
+  const double oX = ld(L.prv[2], s), oY = ld(L.prv[3], s), oZ = ld(L.prv[4], s);
+  double tdc, fcont, G = 0;
+  const double vv = mag2 - k.eta * k.eta;
+  if (k.energy_form == 0) {
+    if (g == 0) {
+      tdc = 0;
+      fcont = 0;
+    } else {
+      double divN = (X - ld(L.cur[2], sxm)) / (dx * dx) + (Y - ld(L.cur[3], sym)) / (dy * dy) +
+                    (Z - ld(L.cur[4], szm)) / (dz * dz);
+      double divP = (oX - ld(L.prv[2], sxm)) / (dx * dx) + (oY - ld(L.prv[3], sym)) / (dy * dy) +
+                    (oZ - ld(L.prv[4], szm)) / (dz * dz);
+      double divT = (divN - divP) / dt;
+      G = divT / g - 2 * g * (pi * vel[0] - pr * vel[1]);
+      double ex = (X - oX) / (g * dt * dx), ey = (Y - oY) / (g * dt * dy), ez = (Z - oZ) / (g * dt * dz);
+      tdc = 0.5 * (ex * ex + ey * ey + ez * ez);
+      double bxy = (1 - ax_cos(pxy)) / ((g * dx * dy) * (g * dx * dy));
+      double bxz = (1 - ax_cos(pxz)) / ((g * dx * dz) * (g * dx * dz));
+      double byz = (1 - ax_cos(pyz)) / ((g * dy * dz) * (g * dy * dz));
+      fcont = bxy + bxz + byz;
+    }
+    double r = vel[0] * vel[0] + vel[1] * vel[1] + gx[0] * gx[0] + gx[1] * gx[1] + gy[0] * gy[0] +
+               gy[1] * gy[1] + gz[0] * gz[0] + gz[1] * gz[1] + tdc + fcont + 0.25 * k.lambda * (vv * vv);
+    rho = dx * dy * dz * r;
+  } else {
+    double ex = (X - oX) / dt, ey = (Y - oY) / dt, ez = (Z - oZ) / dt;
+    tdc = 0.5 * (ex * ex + ey * ey + ez * ez);
+    double bxy = (1 - ax_cos(pxy)) / ((g * dx * dy) * (g * dx * dy));
+    double bxz = (1 - ax_cos(pxz)) / ((g * dx * dz) * (g * dx * dz));
+    double byz = (1 - ax_cos(pyz)) / ((g * dy * dz) * (g * dy * dz));
+    fcont = bxy + bxz + byz;
+    rho = (vel[0] * vel[0] + vel[1] * vel[1] + gx[0] * gx[0] + gx[1] * gx[1] + gy[0] * gy[0] +
+           gy[1] * gy[1] + gz[0] * gz[0] + gz[1] * gz[1] + tdc + fcont + 0.25 * k.lambda * (vv * vv)) *
+          dx * dy * dz;
+  }
+  absG = fabs(G);
+}
+
+template <typename T>
+__device__ __forceinline__ void site_energy_moore(const DiagConsts &k, const Levels<T> &L, int i, int j, int kk,
+                                                  double &rho, double &absG) {
+  const int nx = k.nx, ny = k.ny, nz = k.nz;
+  auto id = [&](int a, int b, int c) { return (a * ny + b) * nz + c; };
+  const int s = id(i, j, kk);
+  const int sxp = id(wrapp(i, nx), j, kk), sxm = id(wrapm(i, nx), j, kk);
+  const int syp = id(i, wrapp(j, ny), kk), sym = id(i, wrapm(j, ny), kk);
+  const int szp = id(i, j, wrapp(kk, nz)), szm = id(i, j, wrapm(kk, nz));
+  const double dx = k.dx, dy = k.dy, dz = k.dz, dt = k.dt, g = k.g;
+  const double q[2] = {k.q1, k.q2};
+  double v[2][2], gx[2][2], gy[2][2], gz[2][2], m[2];
+  for (int f = 0; f < 2; ++f) {
+    const T *re = L.cur[2 * f], *im = L.cur[2 * f + 1];
+    const double ar = ld(re, s), ai = ld(im, s);
+    m[f] = ar * ar + ai * ai;
+    double sxm_, cxm_, sym_, cym_, szm_, czm_;
+    ax_sincos(q[f] * ld(L.cur[4], sxm), &sxm_, &cxm_);
+    ax_sincos(q[f] * ld(L.cur[5], sym), &sym_, &cym_);
+    ax_sincos(q[f] * ld(L.cur[6], szm), &szm_, &czm_);
+    gx[f][0] = (ar - cxm_ * ld(re, sxm) + sxm_ * ld(im, sxm)) / dx;
+    gy[f][0] = (ar - cym_ * ld(re, sym) + sym_ * ld(im, sym)) / dy;
+    gz[f][0] = (ar - czm_ * ld(re, szm) + szm_ * ld(im, szm)) / dz;
+    gx[f][1] = (ai - cxm_ * ld(im, sxm) + (-sxm_) * ld(re, sxm)) / dx;
+    gy[f][1] = (ai - cym_ * ld(im, sym) + (-sym_) * ld(re, sym)) / dy;
+    gz[f][1] = (ai - czm_ * ld(im, szm) + (-szm_) * ld(re, szm)) / dz;
+    v[f][0] = (ar - ld(L.prv[2 * f], s)) / dt;
+    v[f][1] = (ai - ld(L.prv[2 * f + 1], s)) / dt;
+  }
+  const double X = ld(L.cur[4], s), Y = ld(L.cur[5], s), Z = ld(L.cur[6], s);
+  const double oX = ld(L.prv[4], s), oY = ld(L.prv[5], s), oZ = ld(L.prv[6], s);
+  double tdc = 0, fcont = 0, G = 0;
+  if (g != 0) {
+    double divN = (X - ld(L.cur[4], sxm)) / (dx * dx) + (Y - ld(L.cur[5], sym)) / (dy * dy) +
+                  (Z - ld(L.cur[6], szm)) / (dz * dz);
+    double divP = (oX - ld(L.prv[4], sxm)) / (dx * dx) + (oY - ld(L.prv[5], sym)) / (dy * dy) +
+                  (oZ - ld(L.prv[6], szm)) / (dz * dz);
+    double divT = (divN - divP) / dt;
+    G = divT / g - 2 * g * (q[0] * (ld(L.cur[1], s) * v[0][0] - ld(L.cur[0], s) * v[0][1]) +
+                            q[1] * (ld(L.cur[3], s) * v[1][0] - ld(L.cur[2], s) * v[1][1]));
+    double ex = (X - oX) / (g * dt * dx), ey = (Y - oY) / (g * dt * dy), ez = (Z - oZ) / (g * dt * dz);
+    tdc = 0.5 * (ex * ex + ey * ey + ez * ez);
+    double bxy = (1 - ax_cos(X + ld(L.cur[5], sxp) - ld(L.cur[4], syp) - Y)) / ((g * dx * dy) * (g * dx * dy));
+    double bxz = (1 - ax_cos(X + ld(L.cur[6], sxp) - ld(L.cur[4], szp) - Z)) / ((g * dx * dz) * (g * dx * dz));
+    double byz = (1 - ax_cos(Y + ld(L.cur[6], syp) - ld(L.cur[5], szp) - Z)) / ((g * dy * dz) * (g * dy * dz));
+    fcont = bxy + bxz + byz;
+  }
+  const double w1 = m[0] - k.eta1 * k.eta1, w2 = m[1] - k.eta2 * k.eta2;
+  double r = v[0][0] * v[0][0] + v[0][1] * v[0][1] + gx[0][0] * gx[0][0] + gx[0][1] * gx[0][1] +
+             gy[0][0] * gy[0][0] + gy[0][1] * gy[0][1] + gz[0][0] * gz[0][0] + gz[0][1] * gz[0][1] +
+             v[1][0] * v[1][0] + v[1][1] * v[1][1] + gx[1][0] * gx[1][0] + gx[1][1] * gx[1][1] +
+             gy[1][0] * gy[1][0] + gy[1][1] * gy[1][1] + gz[1][0] * gz[1][0] + gz[1][1] * gz[1][1] + tdc +
+             fcont + 0.25 * k.lambda1 * (w1 * w1) + 0.25 * k.lambda2 * (w2 * w2) + 0.5 * k.lambda12 * w1 * w2;
+  rho = dx * dy * dz * r;
+  absG = k.edge ? fabs(G) / k.inner : fabs(G);
+}
+
+/* out: per block [energy, sum|G|, nonfinite count] */
+template <typename T>
+__global__ void __launch_bounds__(256) diag_kernel(DiagConsts k, Levels<T> L, int V, double *part) {
+  __shared__ double sh[4];
+  const int s = blockIdx.x * 256 + threadIdx.x;
+  double rho = 0, aG = 0, bad = 0;
+  if (s < V) {
+    const int kk = s % k.nz, r = s / k.nz, j = r % k.ny, i = r / k.ny;
+    const bool inner = !k.edge || (i > 0 && i < k.nx - 1 && j > 0 && j < k.ny - 1 && kk > 0 && kk < k.nz - 1);
+    if (inner) {
+      if (k.model == AX_MODEL_AH)
+        site_energy_ah<T>(k, L, i, j, kk, rho, aG);
+      else
+        site_energy_moore<T>(k, L, i, j, kk, rho, aG);
+    }
+    for (int c = 0; c < k.ncomp; ++c) {
+      if (!isfinite(ld(L.cur[c], s))) bad += 1;
+      if (!isfinite(ld(L.prv[c], s))) bad += 1;
+    }
+  }
+  double e = block_sum256(rho, sh);
+  double g = block_sum256(aG, sh);
+  double b = block_sum256(bad, sh);
+  if (threadIdx.x == 0) {
+    part[3 * blockIdx.x] = e;
+    part[3 * blockIdx.x + 1] = g;
+    part[3 * blockIdx.x + 2] = b;
+  }
+}
+
+/* One block sums nb triples: thread t adds blocks t, t+256, ... in order,
+ * then a fixed tree; deterministic for a given nb. */
+__global__ void __launch_bounds__(256) sum3_kernel(const double *part, int nb, double *out) {
+  __shared__ double sh[4];
+  double a = 0, b = 0, c = 0;
+  for (int i = threadIdx.x; i < nb; i += 256) {
+    a += part[3 * i];
+    b += part[3 * i + 1];
+    c += part[3 * i + 2];
+  }
+  a = block_sum256(a, sh);
+  b = block_sum256(b, sh);
+  c = block_sum256(c, sh);
+  if (threadIdx.x == 0) {
+    out[0] = a;
+    out[1] = b;
+    out[2] = c;
+  }
+}
+
+/* ------------------------------- winding -------------------------------- */
+/* arg( conj(phi(s)) exp(-i theta) phi(s+a) ): the covariant transport of the
+ * reference's Laplacian (evolution.cpp:304-311) made gauge invariant. */
+__device__ __forceinline__ double link_phase(double cr, double ci, double nr, double ni, double th) {
+  double sn, c;
+  ax_sincos(th, &sn, &c);
+  double wr = c * nr + sn * ni;
+  double wi = c * ni - sn * nr;
+  double re = cr * wr + ci * wi;
+  double im = cr * wi - ci * wr;
+  return ax_atan2(im, re);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) winding_kernel(int nx, int ny, int nz, const T *fr, const T *fi,
+                                                      const T *tx, const T *ty, const T *tz, double q,
+                                                      int8_t *out, double *part) {
+  __shared__ double sh[4];
+  const int V = nx * ny * nz;
+  const int s = blockIdx.x * 256 + threadIdx.x;
+  double tot[3] = {0, 0, 0};
+  if (s < V) {
+    const int kk = s % nz, r = s / nz, j = r % ny, i = r / ny;
+    const int ip = wrapp(i, nx), jp = wrapp(j, ny), kp = wrapp(kk, nz);
+    auto id = [&](int a, int b, int c) { return (a * ny + b) * nz + c; };
+    const int sx = id(ip, j, kk), sy = id(i, jp, kk), sz = id(i, j, kp);
+    const T *th[3] = {tx, ty, tz};
+    const int sa[3] = {sx, sy, sz};
+    const int pa[3] = {0, 0, 1}, pb[3] = {1, 2, 2};
+    const double c0r = ld(fr, s), c0i = ld(fi, s);
+#pragma unroll
+    for (int pl = 0; pl < 3; ++pl) {
+      const int a = pa[pl], b = pb[pl];
+      const int s_a = sa[a], s_b = sa[b];
+      const int ii = (a == 0 || b == 0) ? ip : i;
+      const int jj = (a == 1 || b == 1) ? jp : j;
+      const int kq = (a == 2 || b == 2) ? kp : kk;
+      const int s_ab = id(ii, jj, kq);
+      const double ar = ld(fr, s_a), ai = ld(fi, s_a), br = ld(fr, s_b), bi = ld(fi, s_b);
+      const double abr = ld(fr, s_ab), abi = ld(fi, s_ab);
+      const double A0 = q * ld(th[a], s), B1 = q * ld(th[b], s_a), A2 = q * ld(th[a], s_b), B3 = q * ld(th[b], s);
+      double da0 = link_phase(c0r, c0i, ar, ai, A0);
+      double db1 = link_phase(ar, ai, abr, abi, B1);
+      double da2 = link_phase(br, bi, abr, abi, A2);
+      double db3 = link_phase(c0r, c0i, br, bi, B3);
+      double P = A0 + B1 - A2 - B3;
+      double sum = da0 + db1 - da2 - db3 + P;
+      int w = (int)rint(sum * 0.15915494309189535);
+      out[(size_t)pl * V + s] = (int8_t)w;
+      tot[pl] = (double)(w < 0 ? -w : w);
+    }
+  }
+  double a = block_sum256(tot[0], sh);
+  double b = block_sum256(tot[1], sh);
+  double c = block_sum256(tot[2], sh);
+  if (threadIdx.x == 0) {
+    part[3 * blockIdx.x] = a;
+    part[3 * blockIdx.x + 1] = b;
+    part[3 * blockIdx.x + 2] = c;
+  }
+}
+
+
+/* ------------------------------ crossings ------------------------------- */
+/* Solve the face of mpi_evolution.cpp:552-623; returns the number of roots
+ * inside the face (0..2) and writes them as (sol1, sol2) pairs. */
+__device__ __forceinline__ int face_roots(const double v00[2], const double v10[2], const double v01[2],
+                                          const double v11[2], double u, double du, double w, double dw,
+                                          double r[4]) {
+  double c1[2], c2[2], c3[2], c4[2];
+#pragma unroll
+  for (int c = 0; c < 2; ++c) {
+    c1[c] = v11[c] - v10[c] - v01[c] + v00[c];
+    c2[c] = (w + dw) * (v10[c] - v00[c]) + w * (v01[c] - v11[c]);
+    c3[c] = (u + du) * (v01[c] - v00[c]) + u * (v10[c] - v11[c]);
+    c4[c] = (u + du) * ((w + dw) * v00[c] - w * v01[c]) - u * ((w + dw) * v10[c] - w * v11[c]);
+  }
+  double a = c1[1] * c3[0] - c1[0] * c3[1];
+  double b = c1[1] * c4[0] + c2[1] * c3[0] - c1[0] * c4[1] - c2[0] * c3[1];
+  double c = c2[1] * c4[0] - c2[0] * c4[1];
+  double disc = b * b - 4 * a * c;
+  int n = 0;
+  if (a == 0) {
+    if (b != 0) {
+      double s2 = -c / b;
+      if (s2 >= w && s2 <= w + dw) {
+        double s1 = -(c3[0] * s2 + c4[0]) / (c1[0] * s2 + c2[0]);
+        if (s1 >= u && s1 <= u + du) { r[0] = s1; r[1] = s2; n = 1; }
+      }
+    }
+  } else if (disc >= 0) {
+    double s2 = (-b + sqrt(disc)) / (2 * a);
+    if (s2 >= w && s2 <= w + dw) {
+      double s1 = -(c3[0] * s2 + c4[0]) / (c1[0] * s2 + c2[0]);
+      if (s1 >= u && s1 <= u + du) { r[2 * n] = s1; r[2 * n + 1] = s2; ++n; }
+    }
+    s2 = (-b - sqrt(disc)) / (2 * a);
+    if (s2 >= w && s2 <= w + dw) {
+      double s1 = -(c3[0] * s2 + c4[0]) / (c1[0] * s2 + c2[0]);
+      if (s1 >= u && s1 <= u + du) { r[2 * n] = s1; r[2 * n + 1] = s2; ++n; }
+    }
+  }
+  return n;
+}
+
+/* All crossings of site s, in the reference's order; returns count (<= 6). */
+template <typename T>
+__device__ int site_crossings(int nx, int ny, int nz, double dx, double dy, double dz, const T *fr, const T *fi,
+                              int s, double pts[18]) {
+  const int kk = s % nz, r = s / nz, j = r % ny, i = r / ny;
+  const int ip = wrapp(i, nx), jp = wrapp(j, ny), kp = wrapp(kk, nz);
+  auto id = [&](int a, int b, int c) { return (a * ny + b) * nz + c; };
+  const int sx = id(ip, j, kk), sy = id(i, jp, kk), sz = id(i, j, kp);
+  const int sxy = id(ip, jp, kk), sxz = id(ip, j, kp), syz = id(i, jp, kp);
+  const int x0 = int(0.5 * (nx - 1)), y0 = int(0.5 * (ny - 1)), z0 = int(0.5 * (nz - 1));
+  const double x = (i - x0) * dx, y = (j - y0) * dy, z = (kk - z0) * dz;
+  const double v[2] = {ld(fr, s), ld(fi, s)}, vx[2] = {ld(fr, sx), ld(fi, sx)};
+  const double vy[2] = {ld(fr, sy), ld(fi, sy)}, vz[2] = {ld(fr, sz), ld(fi, sz)};
+  const double vxy[2] = {ld(fr, sxy), ld(fi, sxy)}, vxz[2] = {ld(fr, sxz), ld(fi, sxz)};
+  const double vyz[2] = {ld(fr, syz), ld(fi, syz)};
+  double rr[4];
+  int n = 0;
+  int m = face_roots(v, vx, vy, vxy, x, dx, y, dy, rr);
+  for (int q = 0; q < m; ++q) { pts[3 * n] = rr[2 * q]; pts[3 * n + 1] = rr[2 * q + 1]; pts[3 * n + 2] = z; ++n; }
+  m = face_roots(v, vx, vz, vxz, x, dx, z, dz, rr);
+  for (int q = 0; q < m; ++q) { pts[3 * n] = rr[2 * q]; pts[3 * n + 1] = y; pts[3 * n + 2] = rr[2 * q + 1]; ++n; }
+  m = face_roots(v, vy, vz, vyz, y, dy, z, dz, rr);
+  for (int q = 0; q < m; ++q) { pts[3 * n] = x; pts[3 * n + 1] = rr[2 * q]; pts[3 * n + 2] = rr[2 * q + 1]; ++n; }
+  return n;
+}
+
+/* pass 1: per-block crossing counts */
+template <typename T>
+__global__ void __launch_bounds__(256) cross_count_kernel(int nx, int ny, int nz, double dx, double dy, double dz,
+                                                          const T *fr, const T *fi, int *bcount) {
+  __shared__ int sh[4];
+  const int V = nx * ny * nz;
+  const int s = blockIdx.x * 256 + threadIdx.x;
+  int n = 0;
+  if (s < V) {
+    double pts[18];
+    n = site_crossings<T>(nx, ny, nz, dx, dy, dz, fr, fi, s, pts);
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) n += __shfl_xor(n, off, 64);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = n;
+  __syncthreads();
+  if (threadIdx.x == 0) bcount[blockIdx.x] = sh[0] + sh[1] + sh[2] + sh[3];
+}
+
+/* pass 2: exclusive scan of the block counts (one block). */
+__global__ void __launch_bounds__(1024) scan_kernel(const int *in, long long *out, int n, long long *total) {
+  __shared__ long long sh[1024];
+  const int t = threadIdx.x;
+  const int per = (n + 1023) / 1024;
+  const int b0 = t * per, b1 = min(n, b0 + per);
+  long long loc = 0;
+  for (int i = b0; i < b1; ++i) loc += in[i];
+  sh[t] = loc;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {
+    long long v = (t >= off) ? sh[t - off] : 0;
+    __syncthreads();
+    sh[t] += v;
+    __syncthreads();
+  }
+  long long run = sh[t] - loc;
+  for (int i = b0; i < b1; ++i) {
+    out[i] = run;
+    run += in[i];
+  }
+  if (t == 1023) *total = sh[1023];
+}
+
+/* pass 3: recompute and write points at their global offsets. */
+template <typename T>
+__global__ void __launch_bounds__(256) cross_write_kernel(int nx, int ny, int nz, double dx, double dy, double dz,
+                                                          const T *fr, const T *fi, const long long *boff,
+                                                          double *xyz, long long cap) {
+  __shared__ int sh[256];
+  const int V = nx * ny * nz;
+  const int s = blockIdx.x * 256 + threadIdx.x;
+  double pts[18];
+  int n = 0;
+  if (s < V) n = site_crossings<T>(nx, ny, nz, dx, dy, dz, fr, fi, s, pts);
+  sh[threadIdx.x] = n;
+  __syncthreads();
+  for (int off = 1; off < 256; off <<= 1) {
+    int v = (threadIdx.x >= off) ? sh[threadIdx.x - off] : 0;
+    __syncthreads();
+    sh[threadIdx.x] += v;
+    __syncthreads();
+  }
+  long long o = boff[blockIdx.x] + sh[threadIdx.x] - n;
+  for (int q = 0; q < n; ++q, ++o) {
+    if (o < cap) {
+      xyz[3 * o] = pts[3 * q];
+      xyz[3 * o + 1] = pts[3 * q + 1];
+      xyz[3 * o + 2] = pts[3 * q + 2];
+    }
+  }
+}
+
+/* ------------------------------ launchers ------------------------------- */
+template <typename T>
+hipError_t launch_diag(const DiagConsts &k, const Levels<T> &L, double *part, double *out3, hipStream_t st) {
+  const int V = k.nx * k.ny * k.nz;
+  const int nb = (V + 255) / 256;
+  hipLaunchKernelGGL((diag_kernel<T>), dim3(nb), dim3(256), 0, st, k, L, V, part);
+  hipLaunchKernelGGL(sum3_kernel, dim3(1), dim3(256), 0, st, part, nb, out3);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_winding(int nx, int ny, int nz, const T *fr, const T *fi, const T *tx, const T *ty,
+                          const T *tz, double q, int8_t *out, double *part, double *out1, hipStream_t st) {
+  const int V = nx * ny * nz;
+  const int nb = (V + 255) / 256;
+  hipLaunchKernelGGL((winding_kernel<T>), dim3(nb), dim3(256), 0, st, nx, ny, nz, fr, fi, tx, ty, tz, q, out,
+                     part);
+  hipLaunchKernelGGL(sum3_kernel, dim3(1), dim3(256), 0, st, part, nb, out1);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_crossings(int nx, int ny, int nz, double dx, double dy, double dz, const T *fr, const T *fi,
+                            int *bcount, long long *boff, long long *total, double *xyz, long long cap,
+                            hipStream_t st, bool write) {
+  const int V = nx * ny * nz;
+  const int nb = (V + 255) / 256;
+  hipLaunchKernelGGL((cross_count_kernel<T>), dim3(nb), dim3(256), 0, st, nx, ny, nz, dx, dy, dz, fr, fi,
+                     bcount);
+  hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(1024), 0, st, bcount, boff, nb, total);
+  if (write)
+    hipLaunchKernelGGL((cross_write_kernel<T>), dim3(nb), dim3(256), 0, st, nx, ny, nz, dx, dy, dz, fr, fi,
+                       boff, xyz, cap);
+  return hipGetLastError();
+}
+
+#define AX_INST(T)                                                                                          \
+  template hipError_t launch_diag<T>(const DiagConsts &, const Levels<T> &, double *, double *, hipStream_t); \
+  template hipError_t launch_winding<T>(int, int, int, const T *, const T *, const T *, const T *, const T *,  \
+                                        double, int8_t *, double *, double *, hipStream_t);                  \
+  template hipError_t launch_crossings<T>(int, int, int, double, double, double, const T *, const T *, int *, \
+                                          long long *, long long *, double *, long long, hipStream_t, bool);
+AX_INST(double)
+AX_INST(float)
+#undef AX_INST
+
+}  // namespace ax

@@ -1,0 +1,56 @@
+/*
+ * ax_internal.h - declarations shared by the library's translation units.
+ */
+#ifndef AXIONS_AX_INTERNAL_H
+#define AXIONS_AX_INTERNAL_H
+
+#include <cstdarg>
+
+#include "ax_device.h"
+
+struct ax_lattice;
+
+namespace ax {
+
+struct DiagConsts {
+  int nx, ny, nz, edge, model, energy_form, ncomp;
+  double dx, dy, dz, dt, g, lambda, eta;
+  double lambda1, lambda2, lambda12, eta1, eta2, q1, q2;
+  double inner; /* (nx-2)(ny-2)(nz-2) as a double */
+};
+
+int fail(int code, const char *fmt, ...);
+int hip_fail(hipError_t e, const char *what);
+int check_params(const ax_params *p);
+StepConsts step_consts(const ax_params *p, int n);
+DiagConsts diag_consts(const ax_params *p);
+
+template <typename T>
+Levels<T> levels_of(ax_lattice *L, int n);
+hipStream_t lattice_stream(ax_lattice *L);
+const ax_params &lattice_params(const ax_lattice *L);
+void *lattice_state(ax_lattice *L);
+size_t lattice_sites(const ax_lattice *L);
+
+int launch_step(ax_lattice *L, int n);
+int ensure_winding_buffer(ax_lattice *L);
+int launch_winding_field(ax_lattice *L, int field, int8_t *out, double *part, double *res3);
+
+template <typename R>
+hipError_t launch_step_site(const StepConsts &k, const Levels<R> &L, int model, hipStream_t st);
+bool use_tiled_ah(const ax_params &p);
+hipError_t launch_step_tiled_ah(const StepConsts &k, const Levels<double> &L, hipStream_t st);
+
+template <typename T>
+hipError_t launch_diag(const DiagConsts &k, const Levels<T> &L, double *part, double *out3, hipStream_t st);
+template <typename T>
+hipError_t launch_winding(int nx, int ny, int nz, const T *fr, const T *fi, const T *tx, const T *ty, const T *tz,
+                          double q, int8_t *out, double *part, double *out3, hipStream_t st);
+template <typename T>
+hipError_t launch_crossings(int nx, int ny, int nz, double dx, double dy, double dz, const T *fr, const T *fi,
+                            int *bcount, long long *boff, long long *total, double *xyz, long long cap,
+                            hipStream_t st, bool write);
+
+}  // namespace ax
+
+#endif

@@ -1,0 +1,167 @@
+"""Python mirror of the C ABI (include/axions/axions.h) over libaxions.so.
+
+The reference drives its lattice from one main() (mpi_evolution.cpp:64-998);
+`Lattice` exposes the same steps as methods: seeded IC, TimeStep loop,
+per-step diagnostics, string detection, field read-back.  Every call goes to
+the HIP library; there is no Python or CPU fallback -- a missing or
+unloadable libaxions.so raises at import.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from .abi import AxDiag, AxParams, AX_OK, ncomp, state_len
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libaxions.so")
+
+
+class AxError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"axions error {code}: {msg}")
+        self.code = code
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is not built; run __graft_entry__.build() or "
+                          "python axions_amd/build.py")
+    L = ctypes.CDLL(LIB_PATH)
+    vp, P, sz = ctypes.c_void_p, ctypes.POINTER(AxParams), ctypes.c_size_t
+    i32 = ctypes.c_int32
+    sig = {
+        "ax_default_params": (None, [P]),
+        "ax_strerror": (ctypes.c_char_p, [ctypes.c_int]),
+        "ax_last_error": (ctypes.c_char_p, []),
+        "ax_state_size": (ctypes.c_int, [P, ctypes.POINTER(i32), ctypes.POINTER(sz)]),
+        "ax_create": (ctypes.c_int, [P, ctypes.POINTER(vp)]),
+        "ax_destroy": (None, [vp]),
+        "ax_get_params": (ctypes.c_int, [vp, P]),
+        "ax_init_random_phase": (ctypes.c_int, [vp, ctypes.c_uint32, i32]),
+        "ax_set_state": (ctypes.c_int, [vp, vp, sz]),
+        "ax_get_state": (ctypes.c_int, [vp, vp, sz]),
+        "ax_set_timestep": (ctypes.c_int, [vp, i32]),
+        "ax_get_timestep": (ctypes.c_int, [vp, ctypes.POINTER(i32)]),
+        "ax_step": (ctypes.c_int, [vp, i32]),
+        "ax_sync": (ctypes.c_int, [vp]),
+        "ax_diagnostics": (ctypes.c_int, [vp, i32, ctypes.POINTER(AxDiag)]),
+        "ax_winding": (ctypes.c_int, [vp, vp, sz]),
+        "ax_winding_field": (ctypes.c_int, [vp, i32, vp, sz]),
+        "ax_crossings": (ctypes.c_int, [vp, vp, sz, ctypes.POINTER(sz)]),
+        "ax_random_phases": (ctypes.c_int, [ctypes.c_uint32, sz, vp]),
+        "ax_kernel_timer": (ctypes.c_int, [vp, i32]),
+        "ax_kernel_time": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int64)]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype, f.argtypes = res, args
+    return L
+
+
+lib = _load()
+
+
+def _check(rc):
+    if rc != AX_OK:
+        raise AxError(rc, lib.ax_last_error().decode() or lib.ax_strerror(rc).decode())
+
+
+def random_phases(seed, n):
+    """Phases of the seeded IC: std::mt19937(seed) through
+    std::uniform_real_distribution<double>(0, 2pi) (mpi_evolution.cpp:290-301)."""
+    out = np.empty(n, dtype=np.float64)
+    _check(lib.ax_random_phases(seed, n, out.ctypes.data))
+    return out
+
+
+class Lattice:
+    """One lattice resident in HBM.  Parameters use the reference's names."""
+
+    def __init__(self, params: AxParams):
+        self.params = params
+        h = ctypes.c_void_p()
+        _check(lib.ax_create(ctypes.byref(params), ctypes.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib.ax_destroy(self._h)
+            self._h = None
+
+    __del__ = close
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    @property
+    def handle(self):
+        return self._h
+
+    @property
+    def shape(self):
+        p = self.params
+        return (p.nx, p.ny, p.nz)
+
+    def init_random_phase(self, seed=42, moore_phase_mode=0):
+        _check(lib.ax_init_random_phase(self._h, seed, moore_phase_mode))
+
+    def set_state(self, st, timestep=0):
+        st = np.ascontiguousarray(st, dtype=np.float64)
+        _check(lib.ax_set_state(self._h, st.ctypes.data, st.size))
+        _check(lib.ax_set_timestep(self._h, timestep))
+
+    def get_state(self):
+        st = np.empty(state_len(self.params), dtype=np.float64)
+        _check(lib.ax_get_state(self._h, st.ctypes.data, st.size))
+        return st
+
+    @property
+    def timestep(self):
+        t = ctypes.c_int32()
+        _check(lib.ax_get_timestep(self._h, ctypes.byref(t)))
+        return t.value
+
+    def step(self, n=1):
+        _check(lib.ax_step(self._h, n))
+
+    def sync(self):
+        _check(lib.ax_sync(self._h))
+
+    def kernel_timer(self, enable=True):
+        _check(lib.ax_kernel_timer(self._h, 1 if enable else 0))
+
+    def kernel_time(self):
+        """(GPU milliseconds, steps) of the step kernels timed since the last call."""
+        ms, n = ctypes.c_double(), ctypes.c_int64()
+        _check(lib.ax_kernel_time(self._h, ctypes.byref(ms), ctypes.byref(n)))
+        return ms.value, n.value
+
+    def diagnostics(self, winding=True):
+        d = AxDiag()
+        _check(lib.ax_diagnostics(self._h, 1 if winding else 0, ctypes.byref(d)))
+        return {f: getattr(d, f) for f, _ in d._fields_}
+
+    def winding(self, field=0):
+        p = self.params
+        out = np.empty(3 * p.nx * p.ny * p.nz, dtype=np.int8)
+        _check(lib.ax_winding_field(self._h, field, out.ctypes.data, out.size))
+        return out
+
+    def crossings(self, cap=None):
+        p = self.params
+        cap = 6 * p.nx * p.ny * p.nz if cap is None else cap
+        xyz = np.empty((max(cap, 1), 3), dtype=np.float64)
+        cnt = ctypes.c_size_t()
+        _check(lib.ax_crossings(self._h, xyz.ctypes.data, cap, ctypes.byref(cnt)))
+        return xyz[:min(cap, cnt.value)].copy(), cnt.value
+
+    def levels(self, st=None):
+        """(current, previous) levels [comp][x][y][z] of a state array."""
+        st = self.get_state() if st is None else st
+        n = self.timestep
+        s = st.reshape(ncomp(self.params), 2, *self.shape)
+        return s[:, (n + 1) % 2], s[:, n % 2]

@@ -112,6 +112,8 @@ int ax_get_params(const ax_lattice *lat, ax_params *out);
  * independent phase for phi1 then phi2 at each site; 1 uses one phase for
  * both (moore_mpi_evolution.cpp:304-313). */
 int ax_init_random_phase(ax_lattice *lat, uint32_t seed, int32_t moore_phase_mode);
+/* The first n phases of that stream (host only, no GPU needed). */
+int ax_random_phases(uint32_t seed, size_t n, double *out);
 
 /* Copy the full two-level state in (out of) the reference layout; values
  * are converted to (from) the storage precision.  Replaces the ic.txt
@@ -129,6 +131,13 @@ int ax_get_timestep(const ax_lattice *lat, int32_t *timestep);
 int ax_step(ax_lattice *lat, int32_t nsteps);
 /* Wait for all work enqueued on the handle. */
 int ax_sync(ax_lattice *lat);
+/* Measurement aid (no reference counterpart; the reference times only the
+ * whole run, evolution.cpp:1462-1464): while enabled, each ax_step call
+ * brackets its back-to-back step kernels with HIP events on the handle's
+ * stream.  ax_kernel_time waits, returns the summed GPU milliseconds and
+ * the number of steps they cover, and resets both. */
+int ax_kernel_timer(ax_lattice *lat, int32_t enable);
+int ax_kernel_time(ax_lattice *lat, double *ms, int64_t *steps);
 
 /* Energy, Gauss-law deviation, finiteness and string counts of the current
  * state (evolution.cpp:386-427 + 1404-1408; Moore mooreEvol.cpp:352-389).
@@ -140,6 +149,9 @@ int ax_diagnostics(ax_lattice *lat, int32_t want_winding, ax_diag *out);
  * from site s along the two axes).  No counterpart in the reference; see
  * the library documentation for the definition. */
 int ax_winding(ax_lattice *lat, int8_t *out, size_t n);
+/* Same for a chosen scalar: field 0 = phi (AH) or phi1 (Moore, links
+ * scaled by q1), field 1 = phi2 (Moore, links scaled by q2). */
+int ax_winding_field(ax_lattice *lat, int32_t field, int8_t *out, size_t n);
 
 /* Face crossings of phi = 0 from the bilinear-interpolation test of
  * mpi_evolution.cpp:536-789 / evolution.cpp:723-1016, in the reference's

@@ -117,6 +117,9 @@ def run(name):
     fx["frames"] = frames
     with open(os.path.join(HERE, f"{name}.json"), "w") as f:
         json.dump(fx, f)
+    for d in ("Data", "GifData"):   # run outputs are not kept beside the binary
+        for f in os.listdir(os.path.join(dest, d)):
+            os.remove(os.path.join(dest, d, f))
     print(name, "energy[0..2]", energy[:3], "frames", len(frames), "final", fx["final_sha256"][:12])
 
 

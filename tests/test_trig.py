@@ -1,0 +1,74 @@
+"""The shared sin/cos/atan2 of include/axions/ax_trig.h (used by the HIP
+kernels and by the oracle): accuracy against mpmath, IEEE special values,
+and internal consistency.  The host build is exercised through the oracle
+library; the device build is compared bit for bit in test_gpu_parity."""
+import math
+
+import mpmath as mp
+import numpy as np
+import pytest
+
+from oracle import oracle_py as orc
+
+mp.mp.dps = 40
+
+
+def _ulp_err(vals, fn, args):
+    errs = []
+    for v, a in zip(vals, args):
+        ex = fn(*[mp.mpf(float(x)) for x in a])
+        if ex == 0:
+            errs.append(0.0 if v == 0 else np.inf)
+            continue
+        errs.append(float(abs(mp.mpf(float(v)) - ex) / math.ulp(float(ex))))
+    return np.array(errs)
+
+
+@pytest.mark.parametrize("lo,hi", [(0.0, 1.0), (-10.0, 10.0), (-1e3, 1e3), (-2.0 ** 20, 2.0 ** 20)])
+def test_sin_cos_accuracy(lo, hi):
+    x = np.random.default_rng(11).uniform(lo, hi, 3000)
+    es = _ulp_err(orc.trig(0, x), mp.sin, [(v,) for v in x])
+    ec = _ulp_err(orc.trig(1, x), mp.cos, [(v,) for v in x])
+    assert es.max() < 0.9 and ec.max() < 0.9, (es.max(), ec.max())
+
+
+def test_sin_cos_special_values():
+    x = np.array([0.0, -0.0, 1e-300, -1e-300, 5e-324, np.inf, -np.inf, np.nan])
+    s, c = orc.trig(0, x), orc.trig(1, x)
+    assert s[0] == 0 and not np.signbit(s[0]) and s[1] == 0 and np.signbit(s[1])
+    assert c[0] == 1.0 and c[1] == 1.0
+    assert s[2] == 1e-300 and s[3] == -1e-300 and s[4] == 5e-324
+    assert np.all(np.isnan(s[5:])) and np.all(np.isnan(c[5:]))
+    # sin(pi/2) rounds to 1, cos(pi) to -1
+    assert orc.trig(0, np.array([math.pi / 2]))[0] == 1.0
+    assert orc.trig(1, np.array([math.pi]))[0] == -1.0
+
+
+def test_sin_cos_symmetry_and_identity():
+    x = np.random.default_rng(3).uniform(-50, 50, 20000)
+    s, c = orc.trig(0, x), orc.trig(1, x)
+    assert np.array_equal(orc.trig(0, -x), -s)          # odd
+    assert np.array_equal(orc.trig(1, -x), c)           # even
+    assert np.max(np.abs(s * s + c * c - 1)) < 4e-16
+
+
+def test_atan2():
+    rng = np.random.default_rng(5)
+    y, x = rng.normal(size=3000), rng.normal(size=3000)
+    e = _ulp_err(orc.trig(2, y, x), mp.atan2, list(zip(y, x)))
+    assert e.max() < 2.5, e.max()
+    pairs = [(0.0, 1.0), (-0.0, 1.0), (0.0, -1.0), (-0.0, -1.0), (1.0, 0.0), (-1.0, 0.0),
+             (0.0, 0.0), (-0.0, -0.0), (1.0, 1.0), (-1.0, -1.0), (3.0, -4.0)]
+    for a, b in pairs:
+        got = orc.trig(2, np.array([a]), np.array([b]))[0]
+        ref = math.atan2(a, b)
+        assert got == ref and np.signbit(got) == np.signbit(ref), (a, b, got, ref)
+
+
+def test_sincosf_accuracy():
+    x = np.random.default_rng(9).uniform(-100, 100, 3000).astype(np.float32).astype(np.float64)
+    for fn, f in ((3, mp.sin), (4, mp.cos)):
+        got = orc.trig(fn, x)
+        ex = np.array([float(f(mp.mpf(float(v)))) for v in x])
+        ulp = np.spacing(np.abs(ex).astype(np.float32)).astype(np.float64)
+        assert np.max(np.abs(got - ex) / ulp) < 1.6
