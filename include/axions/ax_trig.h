@@ -138,6 +138,17 @@ AX_HD double ax_cos(double x) {
   return c;
 }
 
+/* True when the reduction of x picks k = 0, i.e. |RN(x*2/pi)| <= 0.5.  For
+ * such x, ax_sin(x) == ax_sin_small(x) bit for bit (k = 0 makes the
+ * reduction return r_hi = x, r_lo = +0 and quadrant 0), so a caller may use
+ * the short form when every argument it holds qualifies. */
+AX_HD bool ax_sin_is_small(double x) { return fabs(x * AX_TWO_OVER_PI) <= 0.5; }
+
+AX_HD double ax_sin_small(double x) {
+  double s = ax_sin_kernel(x, 0.0);
+  return (x == 0.0) ? x : s;
+}
+
 /* atan kernel tail A(z), z = u^2, |u| <= tan(pi/8) */
 #define AX_A0 -0.3333333333333333
 #define AX_A1 0.19999999999995513

@@ -49,6 +49,13 @@ CASES = {
                                       g=0.8, alpha=2.0, damped_nt=10, dampFac=0.3), 25),
     "ah_odd_sizes": (lambda: ax.ah_params(17, ny=19, nz=70), 12),
     "ah_plain_64": (lambda: ax.ah_params(64), 20),
+    # nz a multiple of 64 routes AH FP64 through the tiled z-row kernel
+    "tiled_rect_damped": (lambda: ax.ah_params(40, ny=24, nz=128, alpha=3.0, damped_nt=10, dampFac=0.5), 16),
+    "tiled_aniso_damped": (lambda: ax.ah_params(10, ny=12, nz=192, dx=0.7, dy=0.6, dz=0.55, dt=0.13, lam=1.5,
+                                                 eta=0.9, g=0.8, alpha=2.0, damped_nt=8, dampFac=0.3,
+                                                 theta_damp_form=1), 14),
+    "tiled_aniso_plain": (lambda: ax.ah_params(70, ny=5, nz=64, dx=0.7, dy=0.6, dz=0.55, dt=0.13), 12),
+    "tiled_512_row": (lambda: ax.ah_params(33, ny=4, nz=512), 6),
     "moore_periodic_12": (lambda: ax.moore_params(12), 25),
     "moore_fixed_12": (lambda: ax.moore_params(12, boundary=AX_BC_FIXED), 25),
 }
@@ -72,6 +79,21 @@ def test_step_bitexact_vs_oracle(name):
             if not bits_equal(g, st):
                 d = np.nonzero(g != st)[0]
                 pytest.fail(f"{name}: {d.size} values differ after {done} steps, first flat index {d[0]}")
+
+
+def test_tiled_general_sine_path():
+    """Large plaquette angles force the general ax_sin path of the tiled kernel."""
+    p = ax.ah_params(36, ny=6, nz=128)
+    say("[parity] tiled, large plaquette angles")
+    rng = np.random.default_rng(4)
+    st = orc.init_random_phase(p, 5, 0).reshape(5, 2, -1)
+    st[2:5] = rng.uniform(-3.0, 3.0, st[2:5].shape)
+    st = st.ravel().copy()
+    with ax.Lattice(p) as lat:
+        lat.set_state(st)
+        lat.step(5)
+        orc.step(p, st, 0, 5, 0, NT)
+        assert bits_equal(lat.get_state(), st)
 
 
 @pytest.mark.parametrize("name", ["ah_damp_form1_16", "ah_aniso", "moore_periodic_12", "moore_fixed_12"])
