@@ -60,7 +60,27 @@ def cpu_baseline(n, steps, threads, log):
                       f"AH {n}^3 FP64, {steps} steps after 1 untimed, {dt:.2f} s"}
 
 
+def copy_calibration(nbytes, reps):
+    """Device-to-device copies of a known size (HIP runtime copy kernel), run
+    under rocprofv3 --pmc to turn FETCH_SIZE/WRITE_SIZE into bytes."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(nbytes)) == 0
+    hip.hipMemset(a, 1, ctypes.c_size_t(nbytes))
+    for _ in range(reps):
+        assert hip.hipMemcpy(b, a, ctypes.c_size_t(nbytes), 3) == 0   # hipMemcpyDeviceToDevice
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(json.dumps({"copy_calibration_bytes": nbytes, "reps": reps}))
+
+
 def main():
+    if "--copy-calibration" in sys.argv:
+        copy_calibration(1 << 30, 3)
+        return
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000)

@@ -35,6 +35,9 @@
 #define AXIONS_AX_TRIG_H
 
 #include <math.h>
+#ifndef __cplusplus
+#include <stdbool.h>
+#endif
 
 #if defined(__HIPCC__) || defined(__HIP__)
 #define AX_HD __host__ __device__ __forceinline__
@@ -144,8 +147,20 @@ AX_HD double ax_cos(double x) {
  * the short form when every argument it holds qualifies. */
 AX_HD bool ax_sin_is_small(double x) { return fabs(x * AX_TWO_OVER_PI) <= 0.5; }
 
+/* ax_sin_kernel(x, +0) without the r_lo term: for finite z that term is
+ * fma(-z/2, +0, +0) = +0, and fma(v, p, +0) rounds like v*p except when
+ * v*p = -0, which only happens for x = +0 where both sums give +0 (and the
+ * zero test below returns x anyway). */
 AX_HD double ax_sin_small(double x) {
-  double s = ax_sin_kernel(x, 0.0);
+  double z = x * x;
+  double v = z * x;
+  double p = fma(z, AX_S6, AX_S5);
+  p = fma(z, p, AX_S4);
+  p = fma(z, p, AX_S3);
+  p = fma(z, p, AX_S2);
+  p = fma(z, p, AX_S1);
+  p = fma(z, p, AX_S0);
+  double s = x + v * p;
   return (x == 0.0) ? x : s;
 }
 

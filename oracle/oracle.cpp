@@ -680,13 +680,15 @@ int orc_crossings(const ax_params *p, const double *st, int n, double *xyz, size
   return 0;
 }
 
-/* fn: 0 sin, 1 cos, 2 atan2(x, y), 3 sinf, 4 cosf */
+/* fn: 0 sin, 1 cos, 2 atan2(x, y), 3 sinf, 4 cosf, 5 ax_sin_small (NaN where
+ * ax_sin_is_small(x) is false) */
 void orc_trig(int fn, const double *x, const double *y, double *out, size_t n) {
   for (size_t i = 0; i < n; ++i) {
     switch (fn) {
       case 0: out[i] = ax_sin(x[i]); break;
       case 1: out[i] = ax_cos(x[i]); break;
       case 2: out[i] = ax_atan2(x[i], y[i]); break;
+      case 5: out[i] = ax_sin_is_small(x[i]) ? ax_sin_small(x[i]) : NAN; break;
       case 3: { float s, c; ax_sincosf((float)x[i], &s, &c); out[i] = s; break; }
       default: { float s, c; ax_sincosf((float)x[i], &s, &c); out[i] = c; break; }
     }

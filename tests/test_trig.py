@@ -72,3 +72,29 @@ def test_sincosf_accuracy():
         ex = np.array([float(f(mp.mpf(float(v)))) for v in x])
         ulp = np.spacing(np.abs(ex).astype(np.float32)).astype(np.float64)
         assert np.max(np.abs(got - ex) / ulp) < 1.6
+
+
+def test_sin_small_form_is_bit_identical():
+    """The k = 0 short form used by the tiled kernel equals ax_sin bit for bit
+    on its whole domain |RN(x*2/pi)| <= 0.5, including the boundary and zeros."""
+    rng = np.random.default_rng(17)
+    lim = 0.5 * math.pi / 2
+    x = np.concatenate([rng.uniform(-lim, lim, 400000),
+                        rng.standard_normal(20000) * 1e-3,
+                        10.0 ** rng.uniform(-320, 0, 20000) * rng.choice([-1, 1], 20000),
+                        np.array([0.0, -0.0, 5e-324, -5e-324, 1e-200])])
+    edge = [lim]
+    lo = hi = lim
+    for _ in range(200):     # the doubles on either side of the k = 0 boundary
+        lo, hi = np.nextafter(lo, 0), np.nextafter(hi, 2)
+        edge += [lo, hi]
+    edge = np.array(edge)
+    x = np.concatenate([x, edge, -edge])
+    small = orc.trig(5, x)
+    ok = ~np.isnan(small)
+    assert ok.sum() > 400000
+    full = orc.trig(0, x[ok])
+    assert np.array_equal(small[ok].view(np.uint64), full.view(np.uint64))
+    # the predicate is exactly "reduction picks k = 0"
+    kpick = np.rint(x * (2 / math.pi)) + 0.0
+    assert np.array_equal(ok, kpick == 0)
