@@ -15,6 +15,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstdio>
 #include <new>
 #include <random>
@@ -98,6 +99,7 @@ struct ax_lattice {
   ax_params p;
   int ncomp = 0;
   size_t V = 0;
+  size_t pitch = 0;         /* elements from one (component, level) plane to the next */
   size_t esize = 0;
   int timestep = 0;
   int device = 0;
@@ -126,8 +128,8 @@ Levels<T> levels_of(ax_lattice *L, int n) {
   const int tN = (n + 1) % 2, tP = n % 2;
   for (int c = 0; c < 7; ++c) {
     int cc = c < L->ncomp ? c : 0;
-    lv.cur[c] = base + ((size_t)cc * 2 + tN) * L->V;
-    lv.prv[c] = base + ((size_t)cc * 2 + tP) * L->V;
+    lv.cur[c] = base + ((size_t)cc * 2 + tN) * L->pitch;
+    lv.prv[c] = base + ((size_t)cc * 2 + tP) * L->pitch;
   }
   return lv;
 }
@@ -196,13 +198,21 @@ int ax_create(const ax_params *p, ax_lattice **out) {
   L->ncomp = p->model == AX_MODEL_AH ? 5 : 7;
   L->V = (size_t)p->nx * p->ny * p->nz;
   L->esize = p->precision == 64 ? 8 : 4;
+  /* Field planes sit (V*esize + 4352) bytes apart instead of exactly V*esize:
+   * with power-of-two lattices the plain layout puts the same site of all
+   * ten planes on the same L2 set and channel. */
+  {
+    const char *pe = getenv("AX_PLANE_PAD_BYTES");
+    const size_t pad = pe ? (size_t)atol(pe) : 4352;
+    L->pitch = L->V + (pad + L->esize - 1) / L->esize;
+  }
   hipGetDevice(&L->device);
   hipError_t e;
   if ((e = hipStreamCreateWithFlags(&L->stream, hipStreamNonBlocking)) != hipSuccess) {
     delete L;
     return hip_fail(e, "hipStreamCreate");
   }
-  const size_t bytes = (size_t)L->ncomp * 2 * L->V * L->esize;
+  const size_t bytes = (size_t)L->ncomp * 2 * L->pitch * L->esize;
   const size_t nb = (L->V + 255) / 256;
   if (hipMalloc(&L->state, bytes) != hipSuccess || hipMalloc((void **)&L->part, 3 * nb * sizeof(double)) != hipSuccess ||
       hipMalloc((void **)&L->dres, 16 * sizeof(double)) != hipSuccess ||
@@ -266,7 +276,7 @@ int ax_sync(ax_lattice *L) {
 
 /* Upload one field plane (V values) from doubles into level lev of comp. */
 static int upload_plane(ax_lattice *L, int comp, int lev, const double *src, std::vector<float> &tmp) {
-  char *dst = (char *)L->state + ((size_t)comp * 2 + lev) * L->V * L->esize;
+  char *dst = (char *)L->state + ((size_t)comp * 2 + lev) * L->pitch * L->esize;
   hipError_t e;
   if (L->esize == 8) {
     e = hipMemcpyAsync(dst, src, L->V * 8, hipMemcpyHostToDevice, L->stream);
@@ -295,16 +305,20 @@ int ax_set_state(ax_lattice *L, const double *host, size_t n) {
 int ax_get_state(ax_lattice *L, double *host, size_t n) {
   if (!L || !host) return fail(AX_EINVAL, "NULL argument");
   if (n != (size_t)L->ncomp * 2 * L->V) return fail(AX_EINVAL, "state length %zu != %zu", n, (size_t)L->ncomp * 2 * L->V);
-  hipError_t e;
-  if (L->esize == 8) {
-    e = hipMemcpyAsync(host, L->state, n * 8, hipMemcpyDeviceToHost, L->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(L->stream);
-  } else {
-    std::vector<float> tmp(n);
-    e = hipMemcpyAsync(tmp.data(), L->state, n * 4, hipMemcpyDeviceToHost, L->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(L->stream);
-    for (size_t i = 0; i < n; ++i) host[i] = tmp[i];
+  hipError_t e = hipSuccess;
+  std::vector<float> tmp(L->esize == 4 ? L->V : 0);
+  for (int m = 0; m < 2 * L->ncomp && e == hipSuccess; ++m) {
+    const char *src = (const char *)L->state + (size_t)m * L->pitch * L->esize;
+    double *dst = host + (size_t)m * L->V;
+    if (L->esize == 8) {
+      e = hipMemcpyAsync(dst, src, L->V * 8, hipMemcpyDeviceToHost, L->stream);
+    } else {
+      e = hipMemcpyAsync(tmp.data(), src, L->V * 4, hipMemcpyDeviceToHost, L->stream);
+      if (e == hipSuccess) e = hipStreamSynchronize(L->stream);
+      for (size_t i = 0; i < L->V; ++i) dst[i] = tmp[i];
+    }
   }
+  if (e == hipSuccess) e = hipStreamSynchronize(L->stream);
   return e == hipSuccess ? AX_OK : hip_fail(e, "get_state");
 }
 
@@ -350,7 +364,7 @@ int ax_init_random_phase(ax_lattice *L, uint32_t seed, int32_t mode) {
   for (size_t i = 0; i < ph.size(); ++i) ph[i] = dist(gen);
   std::vector<double> re(L->V), im(L->V);
   std::vector<float> tmp;
-  hipError_t e = hipMemsetAsync(L->state, 0, (size_t)L->ncomp * 2 * L->V * L->esize, L->stream);
+  hipError_t e = hipMemsetAsync(L->state, 0, (size_t)L->ncomp * 2 * L->pitch * L->esize, L->stream);
   if (e != hipSuccess) return hip_fail(e, "hipMemset");
   const int nfield = moore ? 2 : 1;
   for (int f = 0; f < nfield; ++f) {
@@ -492,11 +506,11 @@ int ax_crossings(ax_lattice *L, double *xyz, size_t cap, size_t *count) {
   const int tN = (L->timestep + 1) % 2;
   if (L->esize == 8) {
     const double *b = (const double *)L->state;
-    e = launch_crossings<double>(p.nx, p.ny, p.nz, p.dx, p.dy, p.dz, b + (size_t)tN * L->V, b + (size_t)(2 + tN) * L->V,
+    e = launch_crossings<double>(p.nx, p.ny, p.nz, p.dx, p.dy, p.dz, b + (size_t)tN * L->pitch, b + (size_t)(2 + tN) * L->pitch,
                                  L->bcount, L->boff, L->ctotal, L->cxyz, (long long)cap, L->stream, cap > 0);
   } else {
     const float *b = (const float *)L->state;
-    e = launch_crossings<float>(p.nx, p.ny, p.nz, p.dx, p.dy, p.dz, b + (size_t)tN * L->V, b + (size_t)(2 + tN) * L->V,
+    e = launch_crossings<float>(p.nx, p.ny, p.nz, p.dx, p.dy, p.dz, b + (size_t)tN * L->pitch, b + (size_t)(2 + tN) * L->pitch,
                                 L->bcount, L->boff, L->ctotal, L->cxyz, (long long)cap, L->stream, cap > 0);
   }
   if (e != hipSuccess) return hip_fail(e, "crossing launch");
@@ -534,7 +548,7 @@ int launch_winding_field(ax_lattice *L, int field, int8_t *out, double *part, do
   const int t0 = p.model == AX_MODEL_AH ? 2 : 4;
   const double q = p.model == AX_MODEL_AH ? 1.0 : (double)(field == 0 ? p.q1 : p.q2);
   hipError_t e;
-  auto P = [&](int c) { return (size_t)(c * 2 + tN) * L->V; };
+  auto P = [&](int c) { return (size_t)(c * 2 + tN) * L->pitch; };
   if (L->esize == 8) {
     const double *b = (const double *)L->state;
     e = launch_winding<double>(p.nx, p.ny, p.nz, b + P(c0), b + P(c0 + 1), b + P(t0), b + P(t0 + 1), b + P(t0 + 2), q,

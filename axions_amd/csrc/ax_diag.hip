@@ -166,20 +166,29 @@ __device__ __forceinline__ void site_energy_moore(const DiagConsts &k, const Lev
   absG = k.edge ? fabs(G) / k.inner : fabs(G);
 }
 
+/* Sites handled by one 256-thread diagnostic block (each thread takes
+ * kSitesPerThread sites 256 apart, so loads stay coalesced). */
+constexpr int kSitesPerThread = 2;
+constexpr int kSitesPerBlock = 256 * kSitesPerThread;
+
 /* out: per block [energy, sum|G|, nonfinite count] */
 template <typename T>
 __global__ void __launch_bounds__(256) diag_kernel(DiagConsts k, Levels<T> L, int V, double *part) {
   __shared__ double sh[4];
-  const int s = blockIdx.x * 256 + threadIdx.x;
   double rho = 0, aG = 0, bad = 0;
-  if (s < V) {
+  for (int q = 0; q < kSitesPerThread; ++q) {
+    const int s = blockIdx.x * kSitesPerBlock + q * 256 + threadIdx.x;
+    if (s >= V) break;
     const int kk = s % k.nz, r = s / k.nz, j = r % k.ny, i = r / k.ny;
     const bool inner = !k.edge || (i > 0 && i < k.nx - 1 && j > 0 && j < k.ny - 1 && kk > 0 && kk < k.nz - 1);
     if (inner) {
+      double r1 = 0, g1 = 0;
       if (k.model == AX_MODEL_AH)
-        site_energy_ah<T>(k, L, i, j, kk, rho, aG);
+        site_energy_ah<T>(k, L, i, j, kk, r1, g1);
       else
-        site_energy_moore<T>(k, L, i, j, kk, rho, aG);
+        site_energy_moore<T>(k, L, i, j, kk, r1, g1);
+      rho += r1;
+      aG += g1;
     }
     for (int c = 0; c < k.ncomp; ++c) {
       if (!isfinite(ld(L.cur[c], s))) bad += 1;
@@ -196,23 +205,23 @@ __global__ void __launch_bounds__(256) diag_kernel(DiagConsts k, Levels<T> L, in
   }
 }
 
-/* One block sums nb triples: thread t adds blocks t, t+256, ... in order,
+/* One block sums nb triples: thread t adds blocks t, t+1024, ... in order,
  * then a fixed tree; deterministic for a given nb. */
-__global__ void __launch_bounds__(256) sum3_kernel(const double *part, int nb, double *out) {
-  __shared__ double sh[4];
-  double a = 0, b = 0, c = 0;
-  for (int i = threadIdx.x; i < nb; i += 256) {
-    a += part[3 * i];
-    b += part[3 * i + 1];
-    c += part[3 * i + 2];
+__global__ void __launch_bounds__(1024) sum3_kernel(const double *part, int nb, double *out) {
+  __shared__ double sh[3][16];
+  double v[3] = {0, 0, 0};
+  for (int i = threadIdx.x; i < nb; i += 1024)
+    for (int c = 0; c < 3; ++c) v[c] += part[3 * i + c];
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  for (int c = 0; c < 3; ++c) {
+    double x = wave_sum(v[c]);
+    if (l == 0) sh[c][w] = x;
   }
-  a = block_sum256(a, sh);
-  b = block_sum256(b, sh);
-  c = block_sum256(c, sh);
-  if (threadIdx.x == 0) {
-    out[0] = a;
-    out[1] = b;
-    out[2] = c;
+  __syncthreads();
+  if (threadIdx.x < 3) {
+    double acc = 0;
+    for (int q = 0; q < 16; ++q) acc += sh[threadIdx.x][q];
+    out[threadIdx.x] = acc;
   }
 }
 
@@ -235,9 +244,10 @@ __global__ void __launch_bounds__(256) winding_kernel(int nx, int ny, int nz, co
                                                       int8_t *out, double *part) {
   __shared__ double sh[4];
   const int V = nx * ny * nz;
-  const int s = blockIdx.x * 256 + threadIdx.x;
   double tot[3] = {0, 0, 0};
-  if (s < V) {
+  for (int qq = 0; qq < kSitesPerThread; ++qq) {
+    const int s = blockIdx.x * kSitesPerBlock + qq * 256 + threadIdx.x;
+    if (s >= V) break;
     const int kk = s % nz, r = s / nz, j = r % ny, i = r / ny;
     const int ip = wrapp(i, nx), jp = wrapp(j, ny), kp = wrapp(kk, nz);
     auto id = [&](int a, int b, int c) { return (a * ny + b) * nz + c; };
@@ -246,26 +256,37 @@ __global__ void __launch_bounds__(256) winding_kernel(int nx, int ny, int nz, co
     const int sa[3] = {sx, sy, sz};
     const int pa[3] = {0, 0, 1}, pb[3] = {1, 2, 2};
     const double c0r = ld(fr, s), c0i = ld(fi, s);
+    /* the 9 distinct edges of the 3 plaquettes at s: d0[a] = edge (s, a),
+     * d1[a][b] = edge (s+a, b) */
+    double pr_a[3], pi_a[3], d0[3], d1[3][3], th0[3], th1[3][3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      pr_a[a] = ld(fr, sa[a]);
+      pi_a[a] = ld(fi, sa[a]);
+      th0[a] = q * ld(th[a], s);
+      d0[a] = link_phase(c0r, c0i, pr_a[a], pi_a[a], th0[a]);
+    }
 #pragma unroll
     for (int pl = 0; pl < 3; ++pl) {
       const int a = pa[pl], b = pb[pl];
-      const int s_a = sa[a], s_b = sa[b];
       const int ii = (a == 0 || b == 0) ? ip : i;
       const int jj = (a == 1 || b == 1) ? jp : j;
       const int kq = (a == 2 || b == 2) ? kp : kk;
       const int s_ab = id(ii, jj, kq);
-      const double ar = ld(fr, s_a), ai = ld(fi, s_a), br = ld(fr, s_b), bi = ld(fi, s_b);
       const double abr = ld(fr, s_ab), abi = ld(fi, s_ab);
-      const double A0 = q * ld(th[a], s), B1 = q * ld(th[b], s_a), A2 = q * ld(th[a], s_b), B3 = q * ld(th[b], s);
-      double da0 = link_phase(c0r, c0i, ar, ai, A0);
-      double db1 = link_phase(ar, ai, abr, abi, B1);
-      double da2 = link_phase(br, bi, abr, abi, A2);
-      double db3 = link_phase(c0r, c0i, br, bi, B3);
-      double P = A0 + B1 - A2 - B3;
-      double sum = da0 + db1 - da2 - db3 + P;
+      th1[a][b] = q * ld(th[b], sa[a]);
+      th1[b][a] = q * ld(th[a], sa[b]);
+      d1[a][b] = link_phase(pr_a[a], pi_a[a], abr, abi, th1[a][b]);
+      d1[b][a] = link_phase(pr_a[b], pi_a[b], abr, abi, th1[b][a]);
+    }
+#pragma unroll
+    for (int pl = 0; pl < 3; ++pl) {
+      const int a = pa[pl], b = pb[pl];
+      double P = th0[a] + th1[a][b] - th1[b][a] - th0[b];
+      double sum = d0[a] + d1[a][b] - d1[b][a] - d0[b] + P;
       int w = (int)rint(sum * 0.15915494309189535);
       out[(size_t)pl * V + s] = (int8_t)w;
-      tot[pl] = (double)(w < 0 ? -w : w);
+      tot[pl] += (double)(w < 0 ? -w : w);
     }
   }
   double a = block_sum256(tot[0], sh);
@@ -423,9 +444,9 @@ __global__ void __launch_bounds__(256) cross_write_kernel(int nx, int ny, int nz
 template <typename T>
 hipError_t launch_diag(const DiagConsts &k, const Levels<T> &L, double *part, double *out3, hipStream_t st) {
   const int V = k.nx * k.ny * k.nz;
-  const int nb = (V + 255) / 256;
+  const int nb = (V + kSitesPerBlock - 1) / kSitesPerBlock;
   hipLaunchKernelGGL((diag_kernel<T>), dim3(nb), dim3(256), 0, st, k, L, V, part);
-  hipLaunchKernelGGL(sum3_kernel, dim3(1), dim3(256), 0, st, part, nb, out3);
+  hipLaunchKernelGGL(sum3_kernel, dim3(1), dim3(1024), 0, st, part, nb, out3);
   return hipGetLastError();
 }
 
@@ -433,10 +454,10 @@ template <typename T>
 hipError_t launch_winding(int nx, int ny, int nz, const T *fr, const T *fi, const T *tx, const T *ty,
                           const T *tz, double q, int8_t *out, double *part, double *out1, hipStream_t st) {
   const int V = nx * ny * nz;
-  const int nb = (V + 255) / 256;
+  const int nb = (V + kSitesPerBlock - 1) / kSitesPerBlock;
   hipLaunchKernelGGL((winding_kernel<T>), dim3(nb), dim3(256), 0, st, nx, ny, nz, fr, fi, tx, ty, tz, q, out,
                      part);
-  hipLaunchKernelGGL(sum3_kernel, dim3(1), dim3(256), 0, st, part, nb, out1);
+  hipLaunchKernelGGL(sum3_kernel, dim3(1), dim3(1024), 0, st, part, nb, out1);
   return hipGetLastError();
 }
 

@@ -32,6 +32,7 @@
 namespace ax {
 
 struct TiledArgs {
+  int xcd_map;     /* 1: XCD-aware block order */
   int xseg;        /* x planes marched by one workgroup */
   int nseg;        /* x segments */
   int work;        /* ny * nseg work items */
@@ -91,7 +92,7 @@ __global__ void __launch_bounds__(ZREC ? 64 : 512, MINW) ah_tiled_kernel(StepCon
   const int nw = NZ >> 6;
   /* XCD-aware mapping: dispatch puts block b on XCD b%8; give each XCD a
    * contiguous range of work items so neighbouring rows share L2. */
-  const int w = (blockIdx.x & 7) * t.per_xcd + (blockIdx.x >> 3);
+  const int w = t.xcd_map ? (blockIdx.x & 7) * t.per_xcd + (blockIdx.x >> 3) : (int)blockIdx.x;
   if (w >= t.work) return;
   /* ZREC: one wave per (64-site z chunk, row, x segment); otherwise one
    * workgroup per (row, x segment) holding the whole z row */
@@ -352,13 +353,17 @@ hipError_t launch_step_tiled_ah(const StepConsts &k, const Levels<double> &L, hi
   const bool zrec = var != 0 && var != 4;
   if (zrec) t.work *= k.nz / 64;
   t.per_xcd = (t.work + 7) / 8;
-  dim3 grid(8 * t.per_xcd), block(zrec ? 64 : k.nz);
+  const char *xm = getenv("AX_TILED_XCD");
+  t.xcd_map = xm ? atoi(xm) : 0;
+  const char *le = getenv("AX_TILED_LDS_BYTES");   /* caps workgroups per CU */
+  const size_t lds = le ? (size_t)atol(le) : 0;
+  dim3 grid(t.xcd_map ? 8 * t.per_xcd : t.work), block(zrec ? 64 : k.nz);
 #define AX_L4(ZR, PF, MW)                                                                          \
   do {                                                                                             \
-    if (pow2 && plain) hipLaunchKernelGGL((ah_tiled_kernel<true, true, ZR, PF, MW>), grid, block, 0, st, k, L, t); \
-    else if (pow2) hipLaunchKernelGGL((ah_tiled_kernel<true, false, ZR, PF, MW>), grid, block, 0, st, k, L, t);   \
-    else if (plain) hipLaunchKernelGGL((ah_tiled_kernel<false, true, ZR, PF, MW>), grid, block, 0, st, k, L, t);  \
-    else hipLaunchKernelGGL((ah_tiled_kernel<false, false, ZR, PF, MW>), grid, block, 0, st, k, L, t);            \
+    if (pow2 && plain) hipLaunchKernelGGL((ah_tiled_kernel<true, true, ZR, PF, MW>), grid, block, lds, st, k, L, t); \
+    else if (pow2) hipLaunchKernelGGL((ah_tiled_kernel<true, false, ZR, PF, MW>), grid, block, lds, st, k, L, t);   \
+    else if (plain) hipLaunchKernelGGL((ah_tiled_kernel<false, true, ZR, PF, MW>), grid, block, lds, st, k, L, t);  \
+    else hipLaunchKernelGGL((ah_tiled_kernel<false, false, ZR, PF, MW>), grid, block, lds, st, k, L, t);            \
   } while (0)
   switch (var) {
     case 0: AX_L4(false, 1, 1); break;
