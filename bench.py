@@ -32,6 +32,20 @@ HBM_PEAK_GBS = 8000.0
 METRIC = "site-updates/s at 512³ FP64/FP32, % HBM roofline; ensemble p50/p99 latency"
 
 
+def measured_traffic(n, precision):
+    """HBM bytes per step launch from the committed rocprofv3 PMC summary of
+    this exact workload (FETCH_SIZE/WRITE_SIZE scaled by a copy of known
+    size), or None when no summary covers it."""
+    path = os.path.join(REPO, "profiles", f"ah_step_tiled_{n}_fp{precision}.md")
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
 def physical_cores():
     try:
         import psutil
@@ -163,7 +177,7 @@ def main():
                                f"lambda=2 g=eta=1, energy+Gauss+winding every {a.diag_every} steps",
                    "n": a.n, "precision": a.precision, "diag_every": a.diag_every},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(a.n, a.precision),
                      "kernel_ms": avg_kernel_s * 1e3, "bytes_per_site": bytes_per_site},
         "cpu_baseline": cpu,
         "final_diag": {"timestep": dlast["timestep"], "energy": dlast["energy"],
