@@ -102,6 +102,7 @@ struct ax_lattice {
   size_t pitch = 0;         /* elements from one (component, level) plane to the next */
   size_t esize = 0;
   int timestep = 0;
+  int xseg = 32;            /* x planes per wave in the tiled step (ax_set_tile) */
   int device = 0;
   hipStream_t stream = nullptr;
   void *state = nullptr;
@@ -198,6 +199,7 @@ int ax_create(const ax_params *p, ax_lattice **out) {
   L->ncomp = p->model == AX_MODEL_AH ? 5 : 7;
   L->V = (size_t)p->nx * p->ny * p->nz;
   L->esize = p->precision == 64 ? 8 : 4;
+  L->xseg = default_xseg(*p);
   /* Field planes sit (V*esize + 4352) bytes apart instead of exactly V*esize:
    * with power-of-two lattices the plain layout puts the same site of all
    * ten planes on the same L2 set and channel. */
@@ -402,6 +404,19 @@ int ax_step(ax_lattice *L, int32_t nsteps) {
   return AX_OK;
 }
 
+int ax_set_tile(ax_lattice *L, int32_t xseg) {
+  if (!L || xseg < 0) return fail(AX_EINVAL, "bad tile request");
+  L->xseg = xseg;
+  return AX_OK;
+}
+
+int ax_get_tile(const ax_lattice *L, int32_t *xseg, int32_t *tiled) {
+  if (!L || !xseg) return fail(AX_EINVAL, "NULL argument");
+  *xseg = L->xseg;
+  if (tiled) *tiled = use_tiled_ah(L->p) ? 1 : 0;
+  return AX_OK;
+}
+
 int ax_kernel_timer(ax_lattice *L, int32_t enable) {
   if (!L) return fail(AX_EINVAL, "NULL lattice");
   L->timing = enable != 0;
@@ -564,14 +579,15 @@ int launch_winding_field(ax_lattice *L, int field, int8_t *out, double *part, do
 int launch_step(ax_lattice *L, int n) {
   StepConsts k = step_consts(&L->p, n);
   hipError_t e;
+  const bool tiled = use_tiled_ah(L->p);
   if (L->esize == 8) {
     Levels<double> lv = levels_of<double>(L, n);
-    if (use_tiled_ah(L->p))
-      e = launch_step_tiled_ah(k, lv, L->stream);
-    else
-      e = launch_step_site<double>(k, lv, L->p.model, L->stream);
+    e = tiled ? launch_step_tiled_ah<double>(k, lv, L->xseg, L->stream)
+              : launch_step_site<double>(k, lv, L->p.model, L->stream);
   } else {
-    e = launch_step_site<float>(k, levels_of<float>(L, n), L->p.model, L->stream);
+    Levels<float> lv = levels_of<float>(L, n);
+    e = tiled ? launch_step_tiled_ah<float>(k, lv, L->xseg, L->stream)
+              : launch_step_site<float>(k, lv, L->p.model, L->stream);
   }
   return e == hipSuccess ? AX_OK : hip_fail(e, "step launch");
 }

@@ -39,7 +39,9 @@ int launch_winding_field(ax_lattice *L, int field, int8_t *out, double *part, do
 template <typename R>
 hipError_t launch_step_site(const StepConsts &k, const Levels<R> &L, int model, hipStream_t st);
 bool use_tiled_ah(const ax_params &p);
-hipError_t launch_step_tiled_ah(const StepConsts &k, const Levels<double> &L, hipStream_t st);
+int default_xseg(const ax_params &p);
+template <typename R>
+hipError_t launch_step_tiled_ah(const StepConsts &k, const Levels<R> &L, int xseg, hipStream_t st);
 
 template <typename T>
 hipError_t launch_diag(const DiagConsts &k, const Levels<T> &L, double *part, double *out3, hipStream_t st);

@@ -1,0 +1,309 @@
+/*
+ * ax_march.h - the Abelian-Higgs site update as one 64-lane wave marching
+ * along x over a 64-site z chunk of one (y) row.  Shared by the
+ * single-lattice tiled kernel (ax_step_tiled.hip) and the batched ensemble
+ * kernel (ax_pool.hip).
+ *
+ * Same arithmetic, operand order and results as the site kernel
+ * (mpi_evolution.cpp:418-501) for FP64, and as its FP32 instantiation for
+ * FP32; what changes is how often each sine and cosine is evaluated:
+ *   - x-backward values (phi(s-x), cos/sin of theta_x(s-x), and the
+ *     plaquette sines whose copy at s-x is needed) come from the previous
+ *     iteration of the march, kept in registers;
+ *   - y- and z-backward values are recomputed from their operands with the
+ *     same expressions the neighbour uses for its own forward values;
+ *   - the six plaquette sines at s are computed once (the second sine of
+ *     each F term in mpi_evolution.cpp:471-487 is the first one of the
+ *     neighbour, written with the same operands in the same order), and take
+ *     the short k = 0 form of the sine when every lane's argument qualifies,
+ *     which holds almost always because plaquette angles stay below pi/4
+ *     while link angles drift.
+ * Each field value is read from HBM once per step: plane x+1 (current
+ * level) is loaded as the next iteration's centre and carried; the y/z
+ * neighbour rows are cache hits.
+ *
+ * A chunk may be partial (nz not a multiple of 64): lanes past the row end
+ * load the last site of the row and store nothing.  No lane exchanges data
+ * with another, so partial chunks need no special handling beyond that.
+ */
+#ifndef AXIONS_AX_MARCH_H
+#define AXIONS_AX_MARCH_H
+
+#include "ax_device.h"
+
+namespace ax {
+
+template <typename R>
+struct TrigR;
+template <>
+struct TrigR<double> {
+  __device__ static void sc(double x, double *s, double *c) { ax_sincos(x, s, c); }
+  __device__ static double sn(double x) { return ax_sin(x); }
+  __device__ static bool small(double x) { return ax_sin_is_small(x); }
+  __device__ static double sn_small(double x) { return ax_sin_small(x); }
+};
+template <>
+struct TrigR<float> {
+  __device__ static void sc(float x, float *s, float *c) { ax_sincosf(x, s, c); }
+  __device__ static float sn(float x) { return ax_sinf(x); }
+  __device__ static bool small(float x) { return ax_sinf_is_small(x); }
+  __device__ static float sn_small(float x) { return ax_sinf_small(x); }
+};
+
+/* Per-step scalars in the arithmetic type R.  FP64: the values of
+ * StepConsts.  FP32: those doubles rounded once to float, exactly as the
+ * site kernel's (R)k.x casts do. */
+template <typename R>
+struct MarchConsts {
+  R dx2, dy2, dz2, dt, dt2, hl, eta2, m2gg, damp, fric;
+  R inv_dx2, inv_dy2, inv_dz2;  /* exact reciprocals, used when all d^2 are powers of two */
+  R lim;                        /* largest |d| with d/dt finite (plain-mode guard) */
+  int theta_damp_form;
+  int plain;                    /* fric and damp both +0 in R */
+};
+
+template <typename R>
+__host__ __device__ inline void march_consts_from(const StepConsts &k, R lim, MarchConsts<R> &m) {
+  m.dx2 = (R)k.dx2; m.dy2 = (R)k.dy2; m.dz2 = (R)k.dz2;
+  m.dt = (R)k.dt; m.dt2 = (R)k.dt2;
+  m.hl = (R)k.hl; m.eta2 = (R)k.eta2; m.m2gg = (R)k.m2gg;
+  m.damp = (R)k.damp; m.fric = (R)k.fric;
+  m.inv_dx2 = (R)1 / m.dx2; m.inv_dy2 = (R)1 / m.dy2; m.inv_dz2 = (R)1 / m.dz2;
+  m.lim = lim;
+  m.theta_damp_form = k.theta_damp_form;
+  m.plain = m.fric == 0 && !signbit(m.fric) && m.damp == 0 && !signbit(m.damp);
+}
+
+template <typename R, bool POW2>
+__device__ __forceinline__ R mdiv(R a, R d, R inv) {
+  if (POW2) return a * inv;   /* d is a power of two: a/d == a*(1/d) exactly */
+  return a / d;
+}
+
+/* fric*((a-b)/dt) when fric is +0 (plain run): 0*(a-b) has the sign and
+ * NaN-ness of the quotient unless (a-b)/dt overflows, which *ovf reports. */
+template <typename R>
+__device__ __forceinline__ R zero_rate(R diff, R lim, bool *ovf) {
+  *ovf = *ovf || !(fabs(diff) <= lim);
+  return (R)0 * diff;
+}
+
+/* base + wave-uniform row byte offset + per-lane byte offset; both offsets
+ * are 32-bit (every field plane is below 4 GiB), which lets the compiler use
+ * the scalar-base + 32-bit vector-offset load form. */
+typedef __attribute__((address_space(1))) char gchar;
+__device__ __forceinline__ gchar *uniform_row(const void *base, uint32_t row) {
+  /* readfirstlane marks the row address wave-uniform and keeps the compiler
+   * from folding the lane offset into a 64-bit per-lane address; the global
+   * address space keeps the loads global (not flat) */
+  const uint64_t v = (uint64_t)((const char *)base + row);
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+  return (gchar *)(((uint64_t)hi << 32) | lo);
+}
+template <typename R>
+__device__ __forceinline__ R ldo(const R *base, uint32_t row, uint32_t lane_off) {
+  return *(const __attribute__((address_space(1))) R *)(uniform_row(base, row) + lane_off);
+}
+template <typename R>
+__device__ __forceinline__ void sto(R *base, uint32_t row, uint32_t lane_off, R v) {
+  *(__attribute__((address_space(1))) R *)(uniform_row(base, row) + lane_off) = v;
+}
+
+/* Everything one site update reads for one x plane besides the carried
+ * values: plane x+1 at (y,z) and the previous level at s (HBM), and the
+ * neighbour rows/columns of plane x and x+1 (cache hits). */
+template <typename R>
+struct PlaneOps {
+  R PRn, PIn, TXn, TYn, TZn;
+  R oPR, oPI, oX, oY, oZ;
+  R PRyp, PIyp, PRym, PIym, PRzp, PIzp, PRzm, PIzm;
+  R TXyp, TZyp, TXzp, TYzp, TXym, TYym, TZym, TYxpym, TYymzp;
+  R TXzm, TYzm, TZzm, TZxpzm, TZypzm;
+};
+
+/* PLAIN: 0 friction products always evaluated, 1 fric = damp = +0 known at
+ * compile time, 2 decided at run time from k.plain (wave-uniform). */
+template <typename R, bool POW2, int PLAIN>
+__device__ __forceinline__ void ah_march(const MarchConsts<R> &k, const Levels<R> &L, int NX, int NY, int NZ,
+                                         int y, int zc, int x0, int x1) {
+  typedef TrigR<R> T;
+  const int lane = (int)(threadIdx.x & 63);
+  const int zr = zc * 64 + lane;
+  const bool act = zr < NZ;
+  const int z = act ? zr : NZ - 1;
+  const int ym = y == 0 ? NY - 1 : y - 1, yp = y == NY - 1 ? 0 : y + 1;
+  const uint32_t E = (uint32_t)sizeof(R);
+  const uint32_t oz = (uint32_t)z * E;
+  const uint32_t ozm = (uint32_t)(z == 0 ? NZ - 1 : z - 1) * E;
+  const uint32_t ozp = (uint32_t)(z == NZ - 1 ? 0 : z + 1) * E;
+  const uint32_t PLB = (uint32_t)NY * (uint32_t)NZ * E;          /* plane bytes */
+  const uint32_t R0 = (uint32_t)y * NZ * E, RM = (uint32_t)ym * NZ * E, RP = (uint32_t)yp * NZ * E;
+  const R *PR = L.cur[0], *PI = L.cur[1], *TX = L.cur[2], *TY = L.cur[3], *TZ = L.cur[4];
+  R *OPR = L.prv[0], *OPI = L.prv[1], *OTX = L.prv[2], *OTY = L.prv[3], *OTZ = L.prv[4];
+  const R dx2 = k.dx2, dy2 = k.dy2, dz2 = k.dz2, dt = k.dt, dt2 = k.dt2;
+  const R hl = k.hl, eta2 = k.eta2, m2gg = k.m2gg, fric = k.fric, damp = k.damp;
+  const bool plain = PLAIN == 1 || (PLAIN == 2 && k.plain);
+
+  auto load_plane = [&](int x, PlaneOps<R> &d) {
+    const int xp = x == NX - 1 ? 0 : x + 1;
+    const uint32_t b = (uint32_t)x * PLB, bn = (uint32_t)xp * PLB;
+    d.PRn = ldo(PR, bn + R0, oz); d.PIn = ldo(PI, bn + R0, oz);
+    d.TXn = ldo(TX, bn + R0, oz); d.TYn = ldo(TY, bn + R0, oz); d.TZn = ldo(TZ, bn + R0, oz);
+    d.oPR = ldo(OPR, b + R0, oz); d.oPI = ldo(OPI, b + R0, oz);
+    d.oX = ldo(OTX, b + R0, oz); d.oY = ldo(OTY, b + R0, oz); d.oZ = ldo(OTZ, b + R0, oz);
+    d.PRyp = ldo(PR, b + RP, oz); d.PIyp = ldo(PI, b + RP, oz);
+    d.PRym = ldo(PR, b + RM, oz); d.PIym = ldo(PI, b + RM, oz);
+    d.PRzp = ldo(PR, b + R0, ozp); d.PIzp = ldo(PI, b + R0, ozp);
+    d.PRzm = ldo(PR, b + R0, ozm); d.PIzm = ldo(PI, b + R0, ozm);
+    d.TXyp = ldo(TX, b + RP, oz); d.TZyp = ldo(TZ, b + RP, oz);
+    d.TXzp = ldo(TX, b + R0, ozp); d.TYzp = ldo(TY, b + R0, ozp);
+    d.TXym = ldo(TX, b + RM, oz); d.TYym = ldo(TY, b + RM, oz); d.TZym = ldo(TZ, b + RM, oz);
+    d.TYxpym = ldo(TY, bn + RM, oz); d.TYymzp = ldo(TY, b + RM, ozp);
+    d.TXzm = ldo(TX, b + R0, ozm); d.TYzm = ldo(TY, b + R0, ozm); d.TZzm = ldo(TZ, b + R0, ozm);
+    d.TZxpzm = ldo(TZ, bn + R0, ozm); d.TZypzm = ldo(TZ, b + RP, ozm);
+  };
+
+  /* ---- prologue: plane x0-1 quantities needed by plane x0 ---- */
+  const int xm0 = x0 == 0 ? NX - 1 : x0 - 1;
+  R PRm, PIm, cxm, sxm, Qyx_m, Qzx_m;
+  {
+    const uint32_t b = (uint32_t)xm0 * PLB, bn = (uint32_t)x0 * PLB;
+    PRm = ldo(PR, b + R0, oz);
+    PIm = ldo(PI, b + R0, oz);
+    const R X = ldo(TX, b + R0, oz), Y = ldo(TY, b + R0, oz), Z = ldo(TZ, b + R0, oz);
+    T::sc(X, &sxm, &cxm);
+    const R ayx = Y + ldo(TX, b + RP, oz) - ldo(TY, bn + R0, oz) - X;
+    const R azx = Z + ldo(TX, b + R0, ozp) - ldo(TZ, bn + R0, oz) - X;
+    Qyx_m = T::sn(ayx);
+    Qzx_m = T::sn(azx);
+  }
+  const uint32_t b0 = (uint32_t)x0 * PLB;
+  R PRc = ldo(PR, b0 + R0, oz), PIc = ldo(PI, b0 + R0, oz);
+  R TXc = ldo(TX, b0 + R0, oz), TYc = ldo(TY, b0 + R0, oz), TZc = ldo(TZ, b0 + R0, oz);
+
+  for (int x = x0; x < x1; ++x) {
+    PlaneOps<R> D;
+    load_plane(x, D);
+    const uint32_t so = (uint32_t)x * PLB + R0;
+
+    /* links at s, s-y and s-z */
+    R cx, sx, cy, sy, cz, sz, cym, sym_, czm, szm;
+    T::sc(TXc, &sx, &cx);
+    T::sc(TYc, &sy, &cy);
+    T::sc(TZc, &sz, &cz);
+    T::sc(D.TYym, &sym_, &cym);
+    T::sc(D.TZzm, &szm, &czm);
+    /* plaquette angles, reference operand order */
+    const R a_xy = TXc + D.TYn - D.TXyp - TYc;
+    const R a_xz = TXc + D.TZn - D.TXzp - TZc;
+    const R a_yx = TYc + D.TXyp - D.TYn - TXc;
+    const R a_yz = TYc + D.TZyp - D.TYzp - TZc;
+    const R a_zx = TZc + D.TXzp - D.TZn - TXc;
+    const R a_zy = TZc + D.TYzp - D.TZyp - TYc;
+    const R b_xy = D.TXym + D.TYxpym - TXc - D.TYym;
+    const R b_zy = D.TZym + D.TYymzp - TZc - D.TYym;
+    const R b_xz = D.TXzm + D.TZxpzm - TXc - D.TZzm;
+    const R b_yz = D.TYzm + D.TZypzm - TYc - D.TZzm;
+    R Qxy, Qxz, Qyx, Qyz, Qzx, Qzy, Qxy_m, Qzy_m, Qxz_m, Qyz_m;
+    const bool small = T::small(a_xy) && T::small(a_xz) && T::small(a_yx) && T::small(a_yz) &&
+                       T::small(a_zx) && T::small(a_zy) && T::small(b_xy) && T::small(b_zy) &&
+                       T::small(b_xz) && T::small(b_yz);
+    if (__all(small)) {
+      Qxy = T::sn_small(a_xy); Qxz = T::sn_small(a_xz); Qyx = T::sn_small(a_yx);
+      Qyz = T::sn_small(a_yz); Qzx = T::sn_small(a_zx); Qzy = T::sn_small(a_zy);
+      Qxy_m = T::sn_small(b_xy); Qzy_m = T::sn_small(b_zy);
+      Qxz_m = T::sn_small(b_xz); Qyz_m = T::sn_small(b_yz);
+    } else {
+      Qxy = T::sn(a_xy); Qxz = T::sn(a_xz); Qyx = T::sn(a_yx);
+      Qyz = T::sn(a_yz); Qzx = T::sn(a_zx); Qzy = T::sn(a_zy);
+      Qxy_m = T::sn(b_xy); Qzy_m = T::sn(b_zy);
+      Qxz_m = T::sn(b_xz); Qyz_m = T::sn(b_yz);
+    }
+
+    /* gauge field first (frees the plaquette sines early) */
+    const R fxy = mdiv<R, POW2>(Qxy - Qxy_m, dy2, k.inv_dy2);
+    const R fxz = mdiv<R, POW2>(Qxz - Qxz_m, dz2, k.inv_dz2);
+    const R fyx = mdiv<R, POW2>(Qyx - Qyx_m, dx2, k.inv_dx2);
+    const R fyz = mdiv<R, POW2>(Qyz - Qyz_m, dz2, k.inv_dz2);
+    const R fzx = mdiv<R, POW2>(Qzx - Qzx_m, dx2, k.inv_dx2);
+    const R fzy = mdiv<R, POW2>(Qzy - Qzy_m, dy2, k.inv_dy2);
+    const R jx = cx * (PIc * D.PRn - PRc * D.PIn) + sx * (PRc * D.PRn + PIc * D.PIn);
+    const R jy = cy * (PIc * D.PRyp - PRc * D.PIyp) + sy * (PRc * D.PRyp + PIc * D.PIyp);
+    const R jz = cz * (PIc * D.PRzp - PRc * D.PIzp) + sz * (PRc * D.PRzp + PIc * D.PIzp);
+    R gX, gY, gZ, f0, f1;
+    bool literal = !plain;
+    if (plain) {
+      bool ovf = false;
+      gX = zero_rate(TXc - D.oX, k.lim, &ovf);
+      gY = zero_rate(TYc - D.oY, k.lim, &ovf);
+      gZ = zero_rate(TZc - D.oZ, k.lim, &ovf);
+      f0 = zero_rate(PRc - D.oPR, k.lim, &ovf);
+      f1 = zero_rate(PIc - D.oPI, k.lim, &ovf);
+      /* some |a-b|/dt overflows: take the literal products (never in a sane state) */
+      literal = __any(ovf);
+    }
+    if (literal) {
+      f0 = fric * ((PRc - D.oPR) / dt);
+      f1 = fric * ((PIc - D.oPI) / dt);
+      if (k.theta_damp_form == 0) {
+        gX = damp * ((TXc - D.oX) / dt); gY = damp * ((TYc - D.oY) / dt); gZ = damp * ((TZc - D.oZ) / dt);
+      } else {
+        gX = damp * (TXc - D.oX) / dt; gY = damp * (TYc - D.oY) / dt; gZ = damp * (TZc - D.oZ) / dt;
+      }
+    }
+    const R ax_ = m2gg * jx - fxy - fxz - gX;
+    const R ay_ = m2gg * jy - fyx - fyz - gY;
+    const R az_ = m2gg * jz - fzx - fzy - gZ;
+    if (act) {
+      sto(OTX, so, oz, 2 * TXc - D.oX + dt2 * ax_);
+      sto(OTY, so, oz, 2 * TYc - D.oY + dt2 * ay_);
+      sto(OTZ, so, oz, 2 * TZc - D.oZ + dt2 * az_);
+    }
+
+    /* scalar field */
+    const R mag2 = PRc * PRc + PIc * PIc;
+    const R pot = hl * (mag2 - eta2);
+    const R lxx0 = mdiv<R, POW2>(cx * D.PRn + sx * D.PIn - 2 * PRc + cxm * PRm - sxm * PIm, dx2, k.inv_dx2);
+    const R lyy0 = mdiv<R, POW2>(cy * D.PRyp + sy * D.PIyp - 2 * PRc + cym * D.PRym - sym_ * D.PIym, dy2, k.inv_dy2);
+    const R lzz0 = mdiv<R, POW2>(cz * D.PRzp + sz * D.PIzp - 2 * PRc + czm * D.PRzm - szm * D.PIzm, dz2, k.inv_dz2);
+    const R lxx1 = mdiv<R, POW2>(cx * D.PIn + (-sx) * D.PRn - 2 * PIc + cxm * PIm - (-sxm) * PRm, dx2, k.inv_dx2);
+    const R lyy1 = mdiv<R, POW2>(cy * D.PIyp + (-sy) * D.PRyp - 2 * PIc + cym * D.PIym - (-sym_) * D.PRym, dy2, k.inv_dy2);
+    const R lzz1 = mdiv<R, POW2>(cz * D.PIzp + (-sz) * D.PRzp - 2 * PIc + czm * D.PIzm - (-szm) * D.PRzm, dz2, k.inv_dz2);
+    const R acc0 = lxx0 + lyy0 + lzz0 - pot * PRc - f0;
+    const R acc1 = lxx1 + lyy1 + lzz1 - pot * PIc - f1;
+    if (act) {
+      sto(OPR, so, oz, 2 * PRc - D.oPR + dt2 * acc0);
+      sto(OPI, so, oz, 2 * PIc - D.oPI + dt2 * acc1);
+    }
+
+    /* carry to x+1 */
+    PRm = PRc; PIm = PIc;
+    cxm = cx; sxm = sx; Qyx_m = Qyx; Qzx_m = Qzx;
+    PRc = D.PRn; PIc = D.PIn; TXc = D.TXn; TYc = D.TYn; TZc = D.TZn;
+  }
+}
+
+/* largest d (of type R) with d/dt finite in R */
+template <typename R>
+inline R rate_limit(R dt) {
+  const R big = std::numeric_limits<R>::max();
+  R d = big * (dt < 1 ? dt : (R)1);
+  while (!(d / dt <= big)) d = std::nextafter(d, (R)0);
+  for (;;) {
+    R n = std::nextafter(d, std::numeric_limits<R>::infinity());
+    if (!(n / dt <= big) || n > big) break;
+    d = n;
+  }
+  return d;
+}
+
+inline bool is_pow2(double v) {
+  int e;
+  double m = frexp(v, &e);
+  return v > 0 && m == 0.5;
+}
+
+}  // namespace ax
+
+#endif

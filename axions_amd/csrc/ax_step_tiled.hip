@@ -1,382 +1,70 @@
 /*
- * ax_step_tiled.hip - Abelian-Higgs FP64 step, z-row workgroups marching
- * along x.
+ * ax_step_tiled.hip - Abelian-Higgs step of one periodic lattice, one 64-lane
+ * wave per (64-site z chunk, y row, x segment) marching along x (the update
+ * itself is ah_march in ax_march.h).  FP64 and FP32.
  *
- * Same arithmetic, operand order and results as the site kernel
- * (mpi_evolution.cpp:418-501); what changes is how often each sine and
- * cosine is evaluated.  Per site the update needs cos/sin of the three
- * links at s and at s-a, and the sines of the six plaquette angles at s and
- * of their copies at s-b (the second sine of each F term in
- * mpi_evolution.cpp:471-487 is the first one of the neighbour, written with
- * the same operands in the same order, hence the same bits):
- *   - x-backward values come from the previous iteration of the x march
- *     (kept in registers);
- *   - z-backward values come from the lane to the left (one workgroup holds a
- *     whole periodic z row, wave edges go through LDS);
- *   - y-backward values are recomputed.
- * This cuts the trig work from 18 to 12 evaluations per site, and the
- * plaquette sines take the short form of ax_sin when every lane's argument
- * reduces with k = 0 (ax_sin_is_small), which holds almost always because
- * plaquette angles stay below pi/4 while link angles drift.
- *
- * Each field value is read from HBM once per step: plane x+1 is loaded one
- * iteration ahead and carried; the y/z neighbour rows are cache hits.
- *
- * Layout assumptions (checked by use_tiled_ah): nz a multiple of 64,
- * nz <= 512 (one z row per workgroup), periodic boundaries, FP64.
+ * Work item order: z chunk fastest, then y, then x segment, so the waves
+ * resident at one time cover whole neighbouring rows of a few x planes and
+ * their y-neighbour reads hit in L2.
  */
 #include <cstdlib>
+#include <limits>
 
 #include "ax_internal.h"
+#include "ax_march.h"
 
 namespace ax {
 
-struct TiledArgs {
-  int xcd_map;     /* 1: XCD-aware block order */
-  int xseg;        /* x planes marched by one workgroup */
-  int nseg;        /* x segments */
-  int work;        /* ny * nseg work items */
-  int per_xcd;     /* ceil(work / 8) */
-  double inv_dx2, inv_dy2, inv_dz2;  /* exact reciprocals when pow2 */
-  double lim;      /* largest |d| with d/dt finite (plain-mode guard) */
+struct TileGrid {
+  int nw;      /* z chunks per row */
+  int xseg;    /* x planes marched by one wave */
+  int work;    /* nw * ny * nseg */
 };
 
-template <bool POW2>
-__device__ __forceinline__ double dvd(double a, double d, double inv) {
-  if (POW2) return a * inv;   /* d is a power of two: a/d == a*(1/d) exactly */
-  return a / d;
-}
-
-/* fric*((a-b)/dt) when fric is +0 (plain run): 0*(a-b) has the sign and
- * NaN-ness of the quotient unless (a-b)/dt overflows, which *ovf reports. */
-__device__ __forceinline__ double zero_times_rate(double diff, double lim, bool *ovf) {
-  *ovf = *ovf || !(fabs(diff) <= lim);
-  return 0.0 * diff;
-}
-
-/* Everything one site update reads, for one x plane: plane x+1 at the same
- * (y,z) (HBM), the previous level at s (HBM), and neighbour rows/columns of
- * plane x (cache hits).  Loaded one iteration ahead of use. */
-struct PlaneLoads {
-  double PRn, PIn, TXn, TYn, TZn;
-  double oPR, oPI, oX, oY, oZ;
-  double PRyp, PIyp, PRym, PIym, PRzp, PIzp, PRzm, PIzm;
-  double TXyp, TZyp, TXzp, TYzp, TXym, TYym, TZym, TYxpym, TYymzp;
-  double TXzm, TYzm, TZzm, TZxpzm, TZypzm;   /* z-backward operands (ZREC) */
-};
-
-/* base + uniform row byte offset + per-lane byte offset; both offsets are
- * 32-bit (use_tiled_ah keeps every field plane below 4 GiB), which lets the
- * compiler use the scalar-base + 32-bit vector-offset load form. */
-typedef __attribute__((address_space(1))) char gchar;
-__device__ __forceinline__ gchar *uniform_row(const void *base, uint32_t row) {
-  /* readfirstlane marks the row address wave-uniform and keeps the compiler
-   * from folding the lane offset into a 64-bit per-lane address; the global
-   * address space keeps the loads global (not flat) */
-  const uint64_t v = (uint64_t)((const char *)base + row);
-  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
-  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
-  return (gchar *)(((uint64_t)hi << 32) | lo);
-}
-__device__ __forceinline__ double ldo(const double *base, uint32_t row, uint32_t lane_off) {
-  return *(const __attribute__((address_space(1))) double *)(uniform_row(base, row) + lane_off);
-}
-__device__ __forceinline__ void sto(double *base, uint32_t row, uint32_t lane_off, double v) {
-  *(__attribute__((address_space(1))) double *)(uniform_row(base, row) + lane_off) = v;
-}
-
-template <bool POW2, bool PLAIN, bool ZREC, int PF, int MINW>
-__global__ void __launch_bounds__(ZREC ? 64 : 512, MINW) ah_tiled_kernel(StepConsts k, Levels<double> L, TiledArgs t) {
-  __shared__ double zsh[2][8][4];
-  const int NX = k.nx, NY = k.ny, NZ = k.nz;
-  const int nw = NZ >> 6;
-  /* XCD-aware mapping: dispatch puts block b on XCD b%8; give each XCD a
-   * contiguous range of work items so neighbouring rows share L2. */
-  const int w = t.xcd_map ? (blockIdx.x & 7) * t.per_xcd + (blockIdx.x >> 3) : (int)blockIdx.x;
-  if (w >= t.work) return;
-  /* ZREC: one wave per (64-site z chunk, row, x segment); otherwise one
-   * workgroup per (row, x segment) holding the whole z row */
-  const int zc = ZREC ? w % nw : 0;
-  const int wr = ZREC ? w / nw : w;
-  const int z = ZREC ? zc * 64 + (int)threadIdx.x : (int)threadIdx.x;
-  const int lane = z & 63, wave = z >> 6;
-  const int y = wr % NY, seg = wr / NY;
-  const int x0 = seg * t.xseg;
-  const int x1 = min(NX, x0 + t.xseg);
-  const int ym = y == 0 ? NY - 1 : y - 1, yp = y == NY - 1 ? 0 : y + 1;
-  const uint32_t oz = (uint32_t)z * 8u;
-  const uint32_t ozm = (uint32_t)(z == 0 ? NZ - 1 : z - 1) * 8u;
-  const uint32_t ozp = (uint32_t)(z == NZ - 1 ? 0 : z + 1) * 8u;
-  const uint32_t PLB = (uint32_t)NY * (uint32_t)NZ * 8u;          /* plane bytes */
-  const uint32_t R0 = (uint32_t)y * NZ * 8u, RM = (uint32_t)ym * NZ * 8u, RP = (uint32_t)yp * NZ * 8u;
-  const double *PR = L.cur[0], *PI = L.cur[1], *TX = L.cur[2], *TY = L.cur[3], *TZ = L.cur[4];
-  double *OPR = L.prv[0], *OPI = L.prv[1], *OTX = L.prv[2], *OTY = L.prv[3], *OTZ = L.prv[4];
-  const double dx2 = k.dx2, dy2 = k.dy2, dz2 = k.dz2, dt = k.dt, dt2 = k.dt2;
-  const double hl = k.hl, eta2 = k.eta2, m2gg = k.m2gg, fric = k.fric, damp = k.damp;
-
-  auto load_plane = [&](int x, PlaneLoads &d) {
-    const int xp = x == NX - 1 ? 0 : x + 1;
-    const uint32_t b = (uint32_t)x * PLB, bn = (uint32_t)xp * PLB;
-    if (PF != 2) {
-      d.PRn = ldo(PR, bn + R0, oz); d.PIn = ldo(PI, bn + R0, oz);
-      d.TXn = ldo(TX, bn + R0, oz); d.TYn = ldo(TY, bn + R0, oz); d.TZn = ldo(TZ, bn + R0, oz);
-      d.oPR = ldo(OPR, b + R0, oz); d.oPI = ldo(OPI, b + R0, oz);
-      d.oX = ldo(OTX, b + R0, oz); d.oY = ldo(OTY, b + R0, oz); d.oZ = ldo(OTZ, b + R0, oz);
-    }
-    d.PRyp = ldo(PR, b + RP, oz); d.PIyp = ldo(PI, b + RP, oz);
-    d.PRym = ldo(PR, b + RM, oz); d.PIym = ldo(PI, b + RM, oz);
-    d.PRzp = ldo(PR, b + R0, ozp); d.PIzp = ldo(PI, b + R0, ozp);
-    d.PRzm = ldo(PR, b + R0, ozm); d.PIzm = ldo(PI, b + R0, ozm);
-    d.TXyp = ldo(TX, b + RP, oz); d.TZyp = ldo(TZ, b + RP, oz);
-    d.TXzp = ldo(TX, b + R0, ozp); d.TYzp = ldo(TY, b + R0, ozp);
-    d.TXym = ldo(TX, b + RM, oz); d.TYym = ldo(TY, b + RM, oz); d.TZym = ldo(TZ, b + RM, oz);
-    d.TYxpym = ldo(TY, bn + RM, oz); d.TYymzp = ldo(TY, b + RM, ozp);
-    if (ZREC) {
-      d.TXzm = ldo(TX, b + R0, ozm); d.TYzm = ldo(TY, b + R0, ozm); d.TZzm = ldo(TZ, b + R0, ozm);
-      d.TZxpzm = ldo(TZ, bn + R0, ozm); d.TZypzm = ldo(TZ, b + RP, ozm);
-    }
-  };
-
-  /* ---- prologue: plane x0-1 quantities needed by plane x0 ---- */
-  const int xm0 = x0 == 0 ? NX - 1 : x0 - 1;
-  double PRm, PIm, cxm, sxm, Qyx_m, Qzx_m;
-  {
-    const uint32_t b = (uint32_t)xm0 * PLB, bn = (uint32_t)x0 * PLB;
-    PRm = ldo(PR, b + R0, oz);
-    PIm = ldo(PI, b + R0, oz);
-    const double X = ldo(TX, b + R0, oz), Y = ldo(TY, b + R0, oz), Z = ldo(TZ, b + R0, oz);
-    ax_sincos(X, &sxm, &cxm);
-    const double ayx = Y + ldo(TX, b + RP, oz) - ldo(TY, bn + R0, oz) - X;
-    const double azx = Z + ldo(TX, b + R0, ozp) - ldo(TZ, bn + R0, oz) - X;
-    Qyx_m = ax_sin(ayx);
-    Qzx_m = ax_sin(azx);
-  }
-  const uint32_t b0 = (uint32_t)x0 * PLB;
-  double PRc = ldo(PR, b0 + R0, oz), PIc = ldo(PI, b0 + R0, oz);
-  double TXc = ldo(TX, b0 + R0, oz), TYc = ldo(TY, b0 + R0, oz), TZc = ldo(TZ, b0 + R0, oz);
-  PlaneLoads D;
-  if (PF == 1) load_plane(x0, D);
-  double hPRn, hPIn, hTXn, hTYn, hTZn, hoPR, hoPI, hoX, hoY, hoZ;   /* PF == 2 carry */
-  auto load_hbm = [&](int x) {
-    const int xp = x == NX - 1 ? 0 : x + 1;
-    const uint32_t b = (uint32_t)x * PLB, bn = (uint32_t)xp * PLB;
-    hPRn = ldo(PR, bn + R0, oz); hPIn = ldo(PI, bn + R0, oz);
-    hTXn = ldo(TX, bn + R0, oz); hTYn = ldo(TY, bn + R0, oz); hTZn = ldo(TZ, bn + R0, oz);
-    hoPR = ldo(OPR, b + R0, oz); hoPI = ldo(OPI, b + R0, oz);
-    hoX = ldo(OTX, b + R0, oz); hoY = ldo(OTY, b + R0, oz); hoZ = ldo(OTZ, b + R0, oz);
-  };
-  if (PF == 2) load_hbm(x0);
-
-  for (int x = x0; x < x1; ++x) {
-    PlaneLoads Dn;
-    if (PF == 1) {
-      load_plane(x + 1 == NX ? 0 : x + 1, Dn);   /* next iteration's operands, in flight meanwhile */
-    } else {
-      load_plane(x, D);
-      if (PF == 2) {
-        /* HBM operands of this plane were loaded one iteration ago */
-        D.PRn = hPRn; D.PIn = hPIn; D.TXn = hTXn; D.TYn = hTYn; D.TZn = hTZn;
-        D.oPR = hoPR; D.oPI = hoPI; D.oX = hoX; D.oY = hoY; D.oZ = hoZ;
-        load_hbm(x + 1 == NX ? 0 : x + 1);
-      }
-    }
-    const uint32_t so = (uint32_t)x * PLB + R0;
-
-    /* links at s and s-y */
-    double cx, sx, cy, sy, cz, sz, cym, sym_;
-    ax_sincos(TXc, &sx, &cx);
-    ax_sincos(TYc, &sy, &cy);
-    ax_sincos(TZc, &sz, &cz);
-    ax_sincos(D.TYym, &sym_, &cym);
-    /* plaquette angles, reference operand order */
-    const double a_xy = TXc + D.TYn - D.TXyp - TYc;
-    const double a_xz = TXc + D.TZn - D.TXzp - TZc;
-    const double a_yx = TYc + D.TXyp - D.TYn - TXc;
-    const double a_yz = TYc + D.TZyp - D.TYzp - TZc;
-    const double a_zx = TZc + D.TXzp - D.TZn - TXc;
-    const double a_zy = TZc + D.TYzp - D.TZyp - TYc;
-    const double b_xy = D.TXym + D.TYxpym - TXc - D.TYym;
-    const double b_zy = D.TZym + D.TYymzp - TZc - D.TYym;
-    double Qxy, Qxz, Qyx, Qyz, Qzx, Qzy, Qxy_m, Qzy_m;
-    const bool small = ax_sin_is_small(a_xy) && ax_sin_is_small(a_xz) && ax_sin_is_small(a_yx) &&
-                       ax_sin_is_small(a_yz) && ax_sin_is_small(a_zx) && ax_sin_is_small(a_zy) &&
-                       ax_sin_is_small(b_xy) && ax_sin_is_small(b_zy);
-    if (__all(small)) {
-      Qxy = ax_sin_small(a_xy); Qxz = ax_sin_small(a_xz); Qyx = ax_sin_small(a_yx);
-      Qyz = ax_sin_small(a_yz); Qzx = ax_sin_small(a_zx); Qzy = ax_sin_small(a_zy);
-      Qxy_m = ax_sin_small(b_xy); Qzy_m = ax_sin_small(b_zy);
-    } else {
-      Qxy = ax_sin(a_xy); Qxz = ax_sin(a_xz); Qyx = ax_sin(a_yx);
-      Qyz = ax_sin(a_yz); Qzx = ax_sin(a_zx); Qzy = ax_sin(a_zy);
-      Qxy_m = ax_sin(b_xy); Qzy_m = ax_sin(b_zy);
-    }
-    double czm, szm, Qxz_m, Qyz_m;
-    if (ZREC) {
-      /* z-backward values recomputed from the neighbour's operands (same
-       * expressions as its own forward values) */
-      ax_sincos(D.TZzm, &szm, &czm);
-      const double b_xz = D.TXzm + D.TZxpzm - TXc - D.TZzm;
-      const double b_yz = D.TYzm + D.TZypzm - TYc - D.TZzm;
-      if (__all(ax_sin_is_small(b_xz) && ax_sin_is_small(b_yz))) {
-        Qxz_m = ax_sin_small(b_xz); Qyz_m = ax_sin_small(b_yz);
-      } else {
-        Qxz_m = ax_sin(b_xz); Qyz_m = ax_sin(b_yz);
-      }
-    } else {
-      /* z-backward values: lane to the left; wave edge through LDS */
-      czm = __shfl_up(cz, 1, 64); szm = __shfl_up(sz, 1, 64);
-      Qxz_m = __shfl_up(Qxz, 1, 64); Qyz_m = __shfl_up(Qyz, 1, 64);
-      const int buf = x & 1;
-      if (lane == 63) {
-        zsh[buf][wave][0] = cz; zsh[buf][wave][1] = sz;
-        zsh[buf][wave][2] = Qxz; zsh[buf][wave][3] = Qyz;
-      }
-      __syncthreads();
-      if (lane == 0) {
-        const int src = wave == 0 ? nw - 1 : wave - 1;
-        czm = zsh[buf][src][0]; szm = zsh[buf][src][1];
-        Qxz_m = zsh[buf][src][2]; Qyz_m = zsh[buf][src][3];
-      }
-    }
-
-    /* gauge field first (frees the plaquette sines early) */
-    const double fxy = dvd<POW2>(Qxy - Qxy_m, dy2, t.inv_dy2);
-    const double fxz = dvd<POW2>(Qxz - Qxz_m, dz2, t.inv_dz2);
-    const double fyx = dvd<POW2>(Qyx - Qyx_m, dx2, t.inv_dx2);
-    const double fyz = dvd<POW2>(Qyz - Qyz_m, dz2, t.inv_dz2);
-    const double fzx = dvd<POW2>(Qzx - Qzx_m, dx2, t.inv_dx2);
-    const double fzy = dvd<POW2>(Qzy - Qzy_m, dy2, t.inv_dy2);
-    const double jx = cx * (PIc * D.PRn - PRc * D.PIn) + sx * (PRc * D.PRn + PIc * D.PIn);
-    const double jy = cy * (PIc * D.PRyp - PRc * D.PIyp) + sy * (PRc * D.PRyp + PIc * D.PIyp);
-    const double jz = cz * (PIc * D.PRzp - PRc * D.PIzp) + sz * (PRc * D.PRzp + PIc * D.PIzp);
-    bool ovf = false;
-    double gX, gY, gZ, f0, f1;
-    if (PLAIN) {
-      gX = zero_times_rate(TXc - D.oX, t.lim, &ovf);
-      gY = zero_times_rate(TYc - D.oY, t.lim, &ovf);
-      gZ = zero_times_rate(TZc - D.oZ, t.lim, &ovf);
-      f0 = zero_times_rate(PRc - D.oPR, t.lim, &ovf);
-      f1 = zero_times_rate(PIc - D.oPI, t.lim, &ovf);
-      if (__any(ovf)) {
-        /* some |a-b|/dt overflows: take the literal products (never in a sane state) */
-        f0 = fric * ((PRc - D.oPR) / dt);
-        f1 = fric * ((PIc - D.oPI) / dt);
-        if (k.theta_damp_form == 0) {
-          gX = damp * ((TXc - D.oX) / dt); gY = damp * ((TYc - D.oY) / dt); gZ = damp * ((TZc - D.oZ) / dt);
-        } else {
-          gX = damp * (TXc - D.oX) / dt; gY = damp * (TYc - D.oY) / dt; gZ = damp * (TZc - D.oZ) / dt;
-        }
-      }
-    } else {
-      f0 = fric * ((PRc - D.oPR) / dt);
-      f1 = fric * ((PIc - D.oPI) / dt);
-      if (k.theta_damp_form == 0) {
-        gX = damp * ((TXc - D.oX) / dt); gY = damp * ((TYc - D.oY) / dt); gZ = damp * ((TZc - D.oZ) / dt);
-      } else {
-        gX = damp * (TXc - D.oX) / dt; gY = damp * (TYc - D.oY) / dt; gZ = damp * (TZc - D.oZ) / dt;
-      }
-    }
-    const double ax_ = m2gg * jx - fxy - fxz - gX;
-    const double ay_ = m2gg * jy - fyx - fyz - gY;
-    const double az_ = m2gg * jz - fzx - fzy - gZ;
-    sto(OTX, so, oz, 2 * TXc - D.oX + dt2 * ax_);
-    sto(OTY, so, oz, 2 * TYc - D.oY + dt2 * ay_);
-    sto(OTZ, so, oz, 2 * TZc - D.oZ + dt2 * az_);
-
-    /* scalar field */
-    const double mag2 = PRc * PRc + PIc * PIc;
-    const double pot = hl * (mag2 - eta2);
-    const double lxx0 = dvd<POW2>(cx * D.PRn + sx * D.PIn - 2 * PRc + cxm * PRm - sxm * PIm, dx2, t.inv_dx2);
-    const double lyy0 = dvd<POW2>(cy * D.PRyp + sy * D.PIyp - 2 * PRc + cym * D.PRym - sym_ * D.PIym, dy2, t.inv_dy2);
-    const double lzz0 = dvd<POW2>(cz * D.PRzp + sz * D.PIzp - 2 * PRc + czm * D.PRzm - szm * D.PIzm, dz2, t.inv_dz2);
-    const double lxx1 = dvd<POW2>(cx * D.PIn + (-sx) * D.PRn - 2 * PIc + cxm * PIm - (-sxm) * PRm, dx2, t.inv_dx2);
-    const double lyy1 = dvd<POW2>(cy * D.PIyp + (-sy) * D.PRyp - 2 * PIc + cym * D.PIym - (-sym_) * D.PRym, dy2, t.inv_dy2);
-    const double lzz1 = dvd<POW2>(cz * D.PIzp + (-sz) * D.PRzp - 2 * PIc + czm * D.PIzm - (-szm) * D.PRzm, dz2, t.inv_dz2);
-    const double acc0 = lxx0 + lyy0 + lzz0 - pot * PRc - f0;
-    const double acc1 = lxx1 + lyy1 + lzz1 - pot * PIc - f1;
-    sto(OPR, so, oz, 2 * PRc - D.oPR + dt2 * acc0);
-    sto(OPI, so, oz, 2 * PIc - D.oPI + dt2 * acc1);
-
-    /* carry to x+1 */
-    PRm = PRc; PIm = PIc;
-    cxm = cx; sxm = sx; Qyx_m = Qyx; Qzx_m = Qzx;
-    PRc = D.PRn; PIc = D.PIn; TXc = D.TXn; TYc = D.TYn; TZc = D.TZn;
-    if (PF == 1) D = Dn;
-  }
-}
-
-static bool is_pow2(double v) {
-  int e;
-  double m = frexp(v, &e);
-  return v > 0 && m == 0.5;
+template <typename R, bool POW2, bool PLAIN>
+__global__ void __launch_bounds__(64) ah_tiled_kernel(MarchConsts<R> k, Levels<R> L, int nx, int ny, int nz,
+                                                      TileGrid g) {
+  const int w = (int)blockIdx.x;
+  if (w >= g.work) return;
+  const int zc = w % g.nw, r = w / g.nw;
+  const int y = r % ny, seg = r / ny;
+  const int x0 = seg * g.xseg;
+  const int x1 = min(nx, x0 + g.xseg);
+  ah_march<R, POW2, PLAIN ? 1 : 0>(k, L, nx, ny, nz, y, zc, x0, x1);
 }
 
 bool use_tiled_ah(const ax_params &p) {
-  return p.model == AX_MODEL_AH && p.precision == 64 && p.boundary == AX_BC_PERIODIC && p.nz % 64 == 0 &&
-         p.nz <= 512 && p.nx >= 3 && p.ny >= 3 &&
-         (double)p.nx * p.ny * p.nz * 8.0 < 4294967296.0 && getenv("AX_FORCE_SITE_KERNEL") == nullptr;
+  const int tail = p.nz % 64;
+  return p.model == AX_MODEL_AH && p.boundary == AX_BC_PERIODIC && (tail == 0 || tail >= 32) &&
+         p.nx >= 3 && p.ny >= 3 &&
+         (double)p.nx * p.ny * p.nz * (p.precision == 64 ? 8.0 : 4.0) < 4294967296.0 &&
+         getenv("AX_FORCE_SITE_KERNEL") == nullptr;
 }
 
-/* largest d with d/dt finite */
-static double rate_limit(double dt) {
-  double d = 1.7976931348623157e308 * (dt < 1 ? dt : 1.0);
-  while (!(d / dt <= 1.7976931348623157e308)) d = nextafter(d, 0.0);
-  for (;;) {
-    double n = nextafter(d, 2e308);
-    if (!(n / dt <= 1.7976931348623157e308) || n > 1.7976931348623157e308) break;
-    d = n;
-  }
-  return d;
-}
-
-hipError_t launch_step_tiled_ah(const StepConsts &k, const Levels<double> &L, hipStream_t st) {
-  TiledArgs t;
+int default_xseg(const ax_params &p) {
   const char *xe = getenv("AX_TILED_XSEG");
-  t.xseg = xe ? atoi(xe) : 32;
-  if (t.xseg <= 0 || t.xseg > k.nx) t.xseg = k.nx;
-  t.nseg = (k.nx + t.xseg - 1) / t.xseg;
-  t.work = k.ny * t.nseg;
-  t.per_xcd = (t.work + 7) / 8;
+  int x = xe ? atoi(xe) : 32;
+  return x;
+}
+
+template <typename R>
+hipError_t launch_step_tiled_ah(const StepConsts &k, const Levels<R> &L, int xseg, hipStream_t st) {
+  TileGrid g;
+  g.xseg = (xseg <= 0 || xseg > k.nx) ? k.nx : xseg;
+  g.nw = (k.nz + 63) / 64;
+  g.work = g.nw * k.ny * ((k.nx + g.xseg - 1) / g.xseg);
+  MarchConsts<R> m;
+  march_consts_from<R>(k, rate_limit<R>((R)k.dt), m);
   const bool pow2 = is_pow2(k.dx2) && is_pow2(k.dy2) && is_pow2(k.dz2);
-  t.inv_dx2 = 1.0 / k.dx2;
-  t.inv_dy2 = 1.0 / k.dy2;
-  t.inv_dz2 = 1.0 / k.dz2;
-  t.lim = rate_limit(k.dt);
-  /* plain: both friction coefficients are +0 for this step */
-  const bool plain = k.fric == 0.0 && !signbit(k.fric) && k.damp == 0.0 && !signbit(k.damp);
-  /* variant: 0 z-exchange + prefetch, 1 z-recompute + prefetch,
-   * 2 z-recompute, 3 z-recompute at >= 3 waves/SIMD, 4 z-exchange,
-   * 5 z-recompute + HBM-operand prefetch, 6 = 5 at >= 3 waves/SIMD,
-   * 7 = 5 at >= 4 waves/SIMD */
-  const char *ve = getenv("AX_TILED_VARIANT");
-  const int var = ve ? atoi(ve) : 2;
-  const bool zrec = var != 0 && var != 4;
-  if (zrec) t.work *= k.nz / 64;
-  t.per_xcd = (t.work + 7) / 8;
-  const char *xm = getenv("AX_TILED_XCD");
-  t.xcd_map = xm ? atoi(xm) : 0;
-  const char *le = getenv("AX_TILED_LDS_BYTES");   /* caps workgroups per CU */
-  const size_t lds = le ? (size_t)atol(le) : 0;
-  dim3 grid(t.xcd_map ? 8 * t.per_xcd : t.work), block(zrec ? 64 : k.nz);
-#define AX_L4(ZR, PF, MW)                                                                          \
-  do {                                                                                             \
-    if (pow2 && plain) hipLaunchKernelGGL((ah_tiled_kernel<true, true, ZR, PF, MW>), grid, block, lds, st, k, L, t); \
-    else if (pow2) hipLaunchKernelGGL((ah_tiled_kernel<true, false, ZR, PF, MW>), grid, block, lds, st, k, L, t);   \
-    else if (plain) hipLaunchKernelGGL((ah_tiled_kernel<false, true, ZR, PF, MW>), grid, block, lds, st, k, L, t);  \
-    else hipLaunchKernelGGL((ah_tiled_kernel<false, false, ZR, PF, MW>), grid, block, lds, st, k, L, t);            \
-  } while (0)
-  switch (var) {
-    case 0: AX_L4(false, 1, 1); break;
-    case 2: AX_L4(true, 0, 1); break;
-    case 3: AX_L4(true, 0, 3); break;
-    case 4: AX_L4(false, 0, 1); break;
-    case 5: AX_L4(true, 2, 1); break;
-    case 6: AX_L4(true, 2, 3); break;
-    case 7: AX_L4(true, 2, 4); break;
-    default: AX_L4(true, 1, 1); break;
-  }
-#undef AX_L4
+  dim3 grid(g.work), block(64);
+  if (pow2 && m.plain) hipLaunchKernelGGL((ah_tiled_kernel<R, true, true>), grid, block, 0, st, m, L, k.nx, k.ny, k.nz, g);
+  else if (pow2) hipLaunchKernelGGL((ah_tiled_kernel<R, true, false>), grid, block, 0, st, m, L, k.nx, k.ny, k.nz, g);
+  else if (m.plain) hipLaunchKernelGGL((ah_tiled_kernel<R, false, true>), grid, block, 0, st, m, L, k.nx, k.ny, k.nz, g);
+  else hipLaunchKernelGGL((ah_tiled_kernel<R, false, false>), grid, block, 0, st, m, L, k.nx, k.ny, k.nz, g);
   return hipGetLastError();
 }
+
+template hipError_t launch_step_tiled_ah<double>(const StepConsts &, const Levels<double> &, int, hipStream_t);
+template hipError_t launch_step_tiled_ah<float>(const StepConsts &, const Levels<float> &, int, hipStream_t);
 
 }  // namespace ax

@@ -51,6 +51,8 @@ def _load():
         "ax_crossings": (ctypes.c_int, [vp, vp, sz, ctypes.POINTER(sz)]),
         "ax_random_phases": (ctypes.c_int, [ctypes.c_uint32, sz, vp]),
         "ax_kernel_timer": (ctypes.c_int, [vp, i32]),
+        "ax_set_tile": (ctypes.c_int, [vp, i32]),
+        "ax_get_tile": (ctypes.c_int, [vp, ctypes.POINTER(i32), ctypes.POINTER(i32)]),
         "ax_kernel_time": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int64)]),
     }
     for name, (res, args) in sig.items():
@@ -130,6 +132,16 @@ class Lattice:
 
     def sync(self):
         _check(lib.ax_sync(self._h))
+
+    def set_tile(self, xseg):
+        """x planes one wave of the marching step advances (0 = whole x extent)."""
+        _check(lib.ax_set_tile(self._h, xseg))
+
+    def tile(self):
+        """(xseg, uses_marching_kernel)."""
+        x, t = ctypes.c_int32(), ctypes.c_int32()
+        _check(lib.ax_get_tile(self._h, ctypes.byref(x), ctypes.byref(t)))
+        return x.value, bool(t.value)
 
     def kernel_timer(self, enable=True):
         _check(lib.ax_kernel_timer(self._h, 1 if enable else 0))

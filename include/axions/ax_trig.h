@@ -249,4 +249,24 @@ AX_HD void ax_sincosf(float x, float *s, float *c) {
   *c = ((q + 1) & 2) ? -c0 : c0;
 }
 
+AX_HD float ax_sinf(float x) {
+  float s, c;
+  ax_sincosf(x, &s, &c);
+  return s;
+}
+
+/* True when the reduction in ax_sincosf picks k = 0; then the three fmaf
+ * steps return r = x exactly and the quadrant is 0, so ax_sinf(x) ==
+ * ax_sinf_small(x) bit for bit. */
+AX_HD bool ax_sinf_is_small(float x) { return fabsf(x * AX_F_TWO_OVER_PI) <= 0.5f; }
+
+AX_HD float ax_sinf_small(float x) {
+  float z = x * x;
+  float ps = fmaf(z, 2.7557319e-06f, -1.9841270e-04f);
+  ps = fmaf(z, ps, 8.3333333e-03f);
+  ps = fmaf(z, ps, -1.6666667e-01f);
+  float sk = fmaf(z * x, ps, x);
+  return (x == 0.0f) ? x : sk;
+}
+
 #endif /* AXIONS_AX_TRIG_H */

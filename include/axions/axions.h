@@ -131,6 +131,14 @@ int ax_get_timestep(const ax_lattice *lat, int32_t *timestep);
 int ax_step(ax_lattice *lat, int32_t nsteps);
 /* Wait for all work enqueued on the handle. */
 int ax_sync(ax_lattice *lat);
+/* Tile shape of the marching step kernel (no reference counterpart: the
+ * reference's site loop has no tiles, mpi_evolution.cpp:381-384): xseg =
+ * x planes one wave advances through (0 = the whole x extent).  Results do
+ * not depend on it.  ax_get_tile also reports whether the lattice uses the
+ * marching kernel at all (periodic Abelian-Higgs; otherwise the site
+ * kernel). */
+int ax_set_tile(ax_lattice *lat, int32_t xseg);
+int ax_get_tile(const ax_lattice *lat, int32_t *xseg, int32_t *tiled);
 /* Measurement aid (no reference counterpart; the reference times only the
  * whole run, evolution.cpp:1462-1464): while enabled, each ax_step call
  * brackets its back-to-back step kernels with HIP events on the handle's

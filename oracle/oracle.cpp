@@ -586,6 +586,124 @@ void crossings_impl(const ax_params *p, const double *st, int n, Pts &P) {
       }
 }
 
+/* ------------------- storage-type-R step (FP32 variant) -------------------
+ * The FP32 variant has no reference counterpart; it is defined as the FP64
+ * AH step evaluated in the storage type R: the per-step scalars are derived
+ * in double exactly as above (dx*dx, dt*dt, 0.5*lambda, eta*eta, (-2*g)*g,
+ * damp + alpha*scaling/time), rounded once to R, and every site expression
+ * of ah_site is evaluated in R with the same operand order, using the
+ * binary32 ax_sincosf for R = float.  ah_site_r<double> is the FP64 step
+ * itself (tests check it bit for bit against run_steps<TrigAx>), which pins
+ * the operand order of the float instantiation. */
+template <class R>
+struct RConsts {
+  R dx2, dy2, dz2, dt, dt2, hl, eta2, m2gg, damp, fric;
+  int form;
+};
+struct TrigF {
+  static float sn(float x) { return ax_sinf(x); }
+  static float cs(float x) {
+    float s, c;
+    ax_sincosf(x, &s, &c);
+    return c;
+  }
+};
+template <class R>
+struct TrigOf;
+template <>
+struct TrigOf<double> : TrigAx {};
+template <>
+struct TrigOf<float> : TrigF {};
+
+template <class R>
+void ah_site_r(const Lat &L, R *st, int tN, int tP, const Nb &n, const RConsts<R> &k) {
+  typedef TrigOf<R> T;
+  auto F = [&](int c, int lev) { return st + ((size_t)c * 2 + lev) * L.V; };
+  const R *pr = F(0, tN), *pi = F(1, tN), *tx = F(2, tN), *ty = F(3, tN), *tz = F(4, tN);
+  R *P0 = F(0, tP), *P1 = F(1, tP), *PX = F(2, tP), *PY = F(3, tP), *PZ = F(4, tP);
+  const size_t s = L.id(n.i, n.j, n.k);
+  const size_t xp = L.id(n.ip, n.j, n.k), xm = L.id(n.im, n.j, n.k);
+  const size_t yp = L.id(n.i, n.jp, n.k), ym = L.id(n.i, n.jm, n.k);
+  const size_t zp = L.id(n.i, n.j, n.kp), zm = L.id(n.i, n.j, n.km);
+  const size_t xp_ym = L.id(n.ip, n.jm, n.k), xp_zm = L.id(n.ip, n.j, n.km);
+  const size_t xm_yp = L.id(n.im, n.jp, n.k), yp_zm = L.id(n.i, n.jp, n.km);
+  const size_t xm_zp = L.id(n.im, n.j, n.kp), ym_zp = L.id(n.i, n.jm, n.kp);
+  const R sx = T::sn(tx[s]), cx = T::cs(tx[s]), sy = T::sn(ty[s]), cy = T::cs(ty[s]);
+  const R sz = T::sn(tz[s]), cz = T::cs(tz[s]);
+  const R sxm = T::sn(tx[xm]), cxm = T::cs(tx[xm]), sym = T::sn(ty[ym]), cym = T::cs(ty[ym]);
+  const R szm = T::sn(tz[zm]), czm = T::cs(tz[zm]);
+  const R ar = pr[s], ai = pi[s];
+  const R mag2 = ar * ar + ai * ai;
+  const R pot = k.hl * (mag2 - k.eta2);
+  {
+    R lxx = (cx * pr[xp] + sx * pi[xp] - 2 * ar + cxm * pr[xm] - sxm * pi[xm]) / k.dx2;
+    R lyy = (cy * pr[yp] + sy * pi[yp] - 2 * ar + cym * pr[ym] - sym * pi[ym]) / k.dy2;
+    R lzz = (cz * pr[zp] + sz * pi[zp] - 2 * ar + czm * pr[zm] - szm * pi[zm]) / k.dz2;
+    R old = P0[s];
+    R vel = (ar - old) / k.dt;
+    R acc = lxx + lyy + lzz - pot * ar - k.fric * vel;
+    P0[s] = 2 * ar - old + k.dt2 * acc;
+  }
+  {
+    R lxx = (cx * pi[xp] + (-sx) * pr[xp] - 2 * ai + cxm * pi[xm] - (-sxm) * pr[xm]) / k.dx2;
+    R lyy = (cy * pi[yp] + (-sy) * pr[yp] - 2 * ai + cym * pi[ym] - (-sym) * pr[ym]) / k.dy2;
+    R lzz = (cz * pi[zp] + (-sz) * pr[zp] - 2 * ai + czm * pi[zm] - (-szm) * pr[zm]) / k.dz2;
+    R old = P1[s];
+    R vel = (ai - old) / k.dt;
+    R acc = lxx + lyy + lzz - pot * ai - k.fric * vel;
+    P1[s] = 2 * ai - old + k.dt2 * acc;
+  }
+  const R jx = cx * (ai * pr[xp] - ar * pi[xp]) + sx * (ar * pr[xp] + ai * pi[xp]);
+  const R jy = cy * (ai * pr[yp] - ar * pi[yp]) + sy * (ar * pr[yp] + ai * pi[yp]);
+  const R jz = cz * (ai * pr[zp] - ar * pi[zp]) + sz * (ar * pr[zp] + ai * pi[zp]);
+  const R X = tx[s], Y = ty[s], Z = tz[s];
+  const R fxy = (T::sn(X + ty[xp] - tx[yp] - Y) - T::sn(tx[ym] + ty[xp_ym] - X - ty[ym])) / k.dy2;
+  const R fxz = (T::sn(X + tz[xp] - tx[zp] - Z) - T::sn(tx[zm] + tz[xp_zm] - X - tz[zm])) / k.dz2;
+  const R fyx = (T::sn(Y + tx[yp] - ty[xp] - X) - T::sn(ty[xm] + tx[xm_yp] - Y - tx[xm])) / k.dx2;
+  const R fyz = (T::sn(Y + tz[yp] - ty[zp] - Z) - T::sn(ty[zm] + tz[yp_zm] - Y - tz[zm])) / k.dz2;
+  const R fzx = (T::sn(Z + tx[zp] - tz[xp] - X) - T::sn(tz[xm] + tx[xm_zp] - Z - tx[xm])) / k.dx2;
+  const R fzy = (T::sn(Z + ty[zp] - tz[yp] - Y) - T::sn(tz[ym] + ty[ym_zp] - Z - ty[ym])) / k.dy2;
+  const R oX = PX[s], oY = PY[s], oZ = PZ[s];
+  R ax_, ay_, az_;
+  if (k.form == 0) {
+    ax_ = k.m2gg * jx - fxy - fxz - k.damp * ((X - oX) / k.dt);
+    ay_ = k.m2gg * jy - fyx - fyz - k.damp * ((Y - oY) / k.dt);
+    az_ = k.m2gg * jz - fzx - fzy - k.damp * ((Z - oZ) / k.dt);
+  } else {
+    ax_ = k.m2gg * jx - fxy - fxz - k.damp * (X - oX) / k.dt;
+    ay_ = k.m2gg * jy - fyx - fyz - k.damp * (Y - oY) / k.dt;
+    az_ = k.m2gg * jz - fzx - fzy - k.damp * (Z - oZ) / k.dt;
+  }
+  PX[s] = 2 * X - oX + k.dt2 * ax_;
+  PY[s] = 2 * Y - oY + k.dt2 * ay_;
+  PZ[s] = 2 * Z - oZ + k.dt2 * az_;
+}
+
+template <class R>
+void run_steps_r(const ax_params *p, R *st, int ts0, int nsteps, int nthreads) {
+  Lat L(p);
+  const int e = (p->boundary == AX_BC_FIXED) ? 1 : 0;
+  for (int n = ts0; n < ts0 + nsteps; ++n) {
+    const int tN = (n + 1) % 2, tP = n % 2;
+    const double time = 1 + n * p->dt;
+    const double damp = (n < p->damped_nt) ? p->dampFac : 0;
+    const double fric = damp + p->alpha * p->scaling / time;
+    RConsts<R> k;
+    k.dx2 = (R)(p->dx * p->dx); k.dy2 = (R)(p->dy * p->dy); k.dz2 = (R)(p->dz * p->dz);
+    k.dt = (R)p->dt; k.dt2 = (R)(p->dt * p->dt);
+    k.hl = (R)(0.5 * p->lambda); k.eta2 = (R)(p->eta * p->eta);
+    k.m2gg = (R)(-2 * p->g * p->g);
+    k.damp = (R)damp; k.fric = (R)fric;
+    k.form = p->theta_damp_form;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static) num_threads(nthreads > 0 ? nthreads : 1)
+#endif
+    for (int i = e; i < L.nx - e; ++i)
+      for (int j = e; j < L.ny - e; ++j)
+        for (int kk = e; kk < L.nz - e; ++kk) ah_site_r<R>(L, st, tN, tP, Nb(L, i, j, kk), k);
+  }
+}
+
 }  // namespace
 
 /* ================================ C ABI =================================== */
@@ -650,6 +768,17 @@ int orc_step(const ax_params *p, double *st, int ts0, int nsteps, int trig_mode,
   return 0;
 }
 
+/* Abelian-Higgs step in the storage type of p->precision (st is float* for
+ * 32, double* for 64), shared trig only. */
+int orc_step_r(const ax_params *p, void *st, int ts0, int nsteps, int nthreads) {
+  if (p->model != AX_MODEL_AH) return -1;
+  if (p->precision == 32)
+    run_steps_r<float>(p, (float *)st, ts0, nsteps, nthreads);
+  else
+    run_steps_r<double>(p, (double *)st, ts0, nsteps, nthreads);
+  return 0;
+}
+
 int orc_diag(const ax_params *p, const double *st, int n, int trig_mode, double *out) {
   if (p->model == AX_MODEL_AH) {
     if (trig_mode == 1) diag_ah<TrigLibm>(p, st, n, out);
@@ -681,7 +810,8 @@ int orc_crossings(const ax_params *p, const double *st, int n, double *xyz, size
 }
 
 /* fn: 0 sin, 1 cos, 2 atan2(x, y), 3 sinf, 4 cosf, 5 ax_sin_small (NaN where
- * ax_sin_is_small(x) is false) */
+ * ax_sin_is_small(x) is false), 6 ax_sinf_small (NaN where ax_sinf_is_small
+ * is false) */
 void orc_trig(int fn, const double *x, const double *y, double *out, size_t n) {
   for (size_t i = 0; i < n; ++i) {
     switch (fn) {
@@ -689,6 +819,7 @@ void orc_trig(int fn, const double *x, const double *y, double *out, size_t n) {
       case 1: out[i] = ax_cos(x[i]); break;
       case 2: out[i] = ax_atan2(x[i], y[i]); break;
       case 5: out[i] = ax_sin_is_small(x[i]) ? ax_sin_small(x[i]) : NAN; break;
+      case 6: out[i] = ax_sinf_is_small((float)x[i]) ? (double)ax_sinf_small((float)x[i]) : NAN; break;
       case 3: { float s, c; ax_sincosf((float)x[i], &s, &c); out[i] = s; break; }
       default: { float s, c; ax_sincosf((float)x[i], &s, &c); out[i] = c; break; }
     }

@@ -30,6 +30,7 @@ def lib():
         L.orc_phases.argtypes = [ctypes.c_uint32, ctypes.c_size_t, vp]
         L.orc_init_random_phase.argtypes = [P, ctypes.c_uint32, ctypes.c_int, vp]
         L.orc_step.argtypes = [P, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        L.orc_step_r.argtypes = [P, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
         L.orc_diag.argtypes = [P, vp, ctypes.c_int, ctypes.c_int, vp]
         L.orc_winding.argtypes = [P, vp, ctypes.c_int, ctypes.c_int, vp]
         L.orc_crossings.argtypes = [P, vp, ctypes.c_int, vp, ctypes.c_size_t,
@@ -60,6 +61,14 @@ def init_random_phase(p, seed, mode=0):
 def step(p, st, ts0, nsteps, trig_mode=0, nthreads=1):
     assert st.dtype == np.float64 and st.flags.c_contiguous and st.size == state_len(p)
     lib().orc_step(ctypes.byref(p), st.ctypes.data, ts0, nsteps, trig_mode, nthreads)
+
+
+def step_r(p, st, ts0, nsteps, nthreads=1):
+    """AH step in the storage type of p.precision (float32 state for 32)."""
+    dt = np.float32 if p.precision == 32 else np.float64
+    assert st.dtype == dt and st.flags.c_contiguous and st.size == state_len(p)
+    rc = lib().orc_step_r(ctypes.byref(p), st.ctypes.data, ts0, nsteps, nthreads)
+    assert rc == 0
 
 
 def diag(p, st, n, trig_mode=0):

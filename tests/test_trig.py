@@ -98,3 +98,28 @@ def test_sin_small_form_is_bit_identical():
     # the predicate is exactly "reduction picks k = 0"
     kpick = np.rint(x * (2 / math.pi)) + 0.0
     assert np.array_equal(ok, kpick == 0)
+
+
+def test_sinf_small_form_is_bit_identical():
+    """Float counterpart: ax_sinf_small equals the sine of ax_sincosf bit for
+    bit wherever the float reduction picks k = 0."""
+    rng = np.random.default_rng(23)
+    lim = np.float32(0.5 * math.pi / 2)
+    x = np.concatenate([rng.uniform(-lim, lim, 400000).astype(np.float32),
+                        (rng.standard_normal(20000) * 1e-3).astype(np.float32),
+                        (10.0 ** rng.uniform(-44, 0, 20000) * rng.choice([-1, 1], 20000)).astype(np.float32),
+                        np.array([0.0, -0.0, 1e-45, -1e-45], dtype=np.float32)])
+    edge = [lim]
+    lo = hi = lim
+    for _ in range(200):
+        lo, hi = np.nextafter(lo, np.float32(0)), np.nextafter(hi, np.float32(2))
+        edge += [lo, hi]
+    edge = np.array(edge, dtype=np.float32)
+    x = np.concatenate([x, edge, -edge]).astype(np.float64)
+    small = orc.trig(6, x)
+    ok = ~np.isnan(small)
+    assert ok.sum() > 400000
+    full = orc.trig(3, x[ok])
+    assert np.array_equal(small[ok].view(np.uint64), full.view(np.uint64))
+    kpick = np.rint(x.astype(np.float32) * np.float32(2 / math.pi)) + np.float32(0.0)
+    assert np.array_equal(ok, kpick == 0)
