@@ -7,3 +7,4 @@ C ABI in include/axions/axions.h); this package is its Python mirror.
 from .abi import (AX_BC_FIXED, AX_BC_PERIODIC, AX_MODEL_AH, AX_MODEL_MOORE, AxDiag, AxParams,  # noqa: F401
                   ah_params, moore_params, ncomp, state_len)
 from .lattice import AxError, Lattice, lib, random_phases  # noqa: F401
+from .pool import Pool  # noqa: F401

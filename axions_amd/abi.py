@@ -104,3 +104,52 @@ def ncomp(p):
 
 def state_len(p):
     return ncomp(p) * 2 * p.nx * p.ny * p.nz
+
+
+AX_POOL_KEEP_STATE = 1
+AX_REQ_QUEUED, AX_REQ_RUNNING, AX_REQ_DONE, AX_REQ_FAILED = 0, 1, 2, 3
+
+
+class AxPoolConfig(ctypes.Structure):
+    _fields_ = [
+        ("hbm_budget", ctypes.c_size_t),
+        ("max_lattices", ctypes.c_int32),
+        ("chunk_steps", ctypes.c_int32),
+        ("ic_threads", ctypes.c_int32),
+        ("use_graph", ctypes.c_int32),
+        ("autotune", ctypes.c_int32),
+        ("xseg_default", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+    ]
+
+
+class AxRequestInfo(ctypes.Structure):
+    _fields_ = [
+        ("id", ctypes.c_int64),
+        ("seed", ctypes.c_uint32),
+        ("n", ctypes.c_int32), ("steps", ctypes.c_int32), ("status", ctypes.c_int32),
+        ("nsamples", ctypes.c_int32), ("xseg", ctypes.c_int32),
+        ("t_submit", ctypes.c_double), ("t_admit", ctypes.c_double), ("t_done", ctypes.c_double),
+    ]
+
+
+class AxPoolStats(ctypes.Structure):
+    _fields_ = [
+        ("submitted", ctypes.c_int64), ("admitted", ctypes.c_int64),
+        ("completed", ctypes.c_int64), ("failed", ctypes.c_int64),
+        ("chunks", ctypes.c_int64),
+        ("resident", ctypes.c_int32), ("queued", ctypes.c_int32),
+        ("max_resident", ctypes.c_int32), ("reserved", ctypes.c_int32),
+        ("site_updates", ctypes.c_double),
+        ("elapsed", ctypes.c_double),
+        ("budget", ctypes.c_size_t), ("in_use", ctypes.c_size_t), ("peak_in_use", ctypes.c_size_t),
+    ]
+
+
+def lattice_bytes(n, precision, pad_bytes=4352):
+    """Device bytes the pool holds for one n^3 request (5 components x 2
+    levels of planes, each padded by pad_bytes, rounded to 256 B)."""
+    es = 8 if precision == 64 else 4
+    V = n ** 3
+    b = 10 * (V + (pad_bytes + es - 1) // es) * es
+    return (b + 255) // 256 * 256

@@ -36,6 +36,27 @@ struct StepConsts {
   double q1, q2;
 };
 
+/* The per-step scalars of TimeStep n.  Evaluated on the host for
+ * single-lattice launches and on the device by the batched kernel; both are
+ * the same IEEE operations (no contraction), hence the same bits. */
+__host__ __device__ inline StepConsts make_step_consts(const ax_params &p, int n) {
+  StepConsts k;
+  k.nx = p.nx; k.ny = p.ny; k.nz = p.nz;
+  k.edge = p.boundary == AX_BC_FIXED;
+  k.dx2 = p.dx * p.dx; k.dy2 = p.dy * p.dy; k.dz2 = p.dz * p.dz;
+  k.dt = p.dt; k.dt2 = p.dt * p.dt;
+  k.hl = 0.5 * p.lambda; k.eta2 = p.eta * p.eta;
+  k.m2gg = -2 * p.g * p.g;
+  const double time = 1 + n * p.dt;                      /* mpi_evolution.cpp:356 */
+  k.damp = (n < p.damped_nt) ? p.dampFac : 0;           /* :367-368 */
+  k.fric = k.damp + p.alpha * p.scaling / time;
+  k.theta_damp_form = p.theta_damp_form;
+  k.hl1 = 0.5 * p.lambda1; k.hl2 = 0.5 * p.lambda2; k.hl12 = 0.5 * p.lambda12;
+  k.eta1sq = p.eta1 * p.eta1; k.eta2sq = p.eta2 * p.eta2;
+  k.q1 = p.q1; k.q2 = p.q2;
+  return k;
+}
+
 /* Pointers to the two time levels of every component. */
 template <typename T>
 struct Levels {

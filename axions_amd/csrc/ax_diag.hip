@@ -18,6 +18,7 @@
  * update of the same TimeStep.
  */
 #include "ax_internal.h"
+#include "ax_wind.h"
 
 namespace ax {
 
@@ -226,18 +227,6 @@ __global__ void __launch_bounds__(1024) sum3_kernel(const double *part, int nb, 
 }
 
 /* ------------------------------- winding -------------------------------- */
-/* arg( conj(phi(s)) exp(-i theta) phi(s+a) ): the covariant transport of the
- * reference's Laplacian (evolution.cpp:304-311) made gauge invariant. */
-__device__ __forceinline__ double link_phase(double cr, double ci, double nr, double ni, double th) {
-  double sn, c;
-  ax_sincos(th, &sn, &c);
-  double wr = c * nr + sn * ni;
-  double wi = c * ni - sn * nr;
-  double re = cr * wr + ci * wi;
-  double im = cr * wi - ci * wr;
-  return ax_atan2(im, re);
-}
-
 template <typename T>
 __global__ void __launch_bounds__(256) winding_kernel(int nx, int ny, int nz, const T *fr, const T *fi,
                                                       const T *tx, const T *ty, const T *tz, double q,
@@ -249,44 +238,12 @@ __global__ void __launch_bounds__(256) winding_kernel(int nx, int ny, int nz, co
     const int s = blockIdx.x * kSitesPerBlock + qq * 256 + threadIdx.x;
     if (s >= V) break;
     const int kk = s % nz, r = s / nz, j = r % ny, i = r / ny;
-    const int ip = wrapp(i, nx), jp = wrapp(j, ny), kp = wrapp(kk, nz);
-    auto id = [&](int a, int b, int c) { return (a * ny + b) * nz + c; };
-    const int sx = id(ip, j, kk), sy = id(i, jp, kk), sz = id(i, j, kp);
-    const T *th[3] = {tx, ty, tz};
-    const int sa[3] = {sx, sy, sz};
-    const int pa[3] = {0, 0, 1}, pb[3] = {1, 2, 2};
-    const double c0r = ld(fr, s), c0i = ld(fi, s);
-    /* the 9 distinct edges of the 3 plaquettes at s: d0[a] = edge (s, a),
-     * d1[a][b] = edge (s+a, b) */
-    double pr_a[3], pi_a[3], d0[3], d1[3][3], th0[3], th1[3][3];
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-      pr_a[a] = ld(fr, sa[a]);
-      pi_a[a] = ld(fi, sa[a]);
-      th0[a] = q * ld(th[a], s);
-      d0[a] = link_phase(c0r, c0i, pr_a[a], pi_a[a], th0[a]);
-    }
+    int w[3];
+    site_winding<T>(nx, ny, nz, fr, fi, tx, ty, tz, q, i, j, kk, w);
 #pragma unroll
     for (int pl = 0; pl < 3; ++pl) {
-      const int a = pa[pl], b = pb[pl];
-      const int ii = (a == 0 || b == 0) ? ip : i;
-      const int jj = (a == 1 || b == 1) ? jp : j;
-      const int kq = (a == 2 || b == 2) ? kp : kk;
-      const int s_ab = id(ii, jj, kq);
-      const double abr = ld(fr, s_ab), abi = ld(fi, s_ab);
-      th1[a][b] = q * ld(th[b], sa[a]);
-      th1[b][a] = q * ld(th[a], sa[b]);
-      d1[a][b] = link_phase(pr_a[a], pi_a[a], abr, abi, th1[a][b]);
-      d1[b][a] = link_phase(pr_a[b], pi_a[b], abr, abi, th1[b][a]);
-    }
-#pragma unroll
-    for (int pl = 0; pl < 3; ++pl) {
-      const int a = pa[pl], b = pb[pl];
-      double P = th0[a] + th1[a][b] - th1[b][a] - th0[b];
-      double sum = d0[a] + d1[a][b] - d1[b][a] - d0[b] + P;
-      int w = (int)rint(sum * 0.15915494309189535);
-      out[(size_t)pl * V + s] = (int8_t)w;
-      tot[pl] += (double)(w < 0 ? -w : w);
+      out[(size_t)pl * V + s] = (int8_t)w[pl];
+      tot[pl] += (double)(w[pl] < 0 ? -w[pl] : w[pl]);
     }
   }
   double a = block_sum256(tot[0], sh);

@@ -1,0 +1,1009 @@
+/*
+ * ax_pool.hip - ensemble pool: continuous batching of many independent
+ * periodic Abelian-Higgs realizations of uneven N on one GPU.
+ *
+ * Device side
+ *   pool_step_kernel   persistent kernel; one launch advances every resident
+ *                      lattice by one TimeStep.  Its work items are the
+ *                      (z chunk, y row, x segment) marches of ah_march for all
+ *                      lattices back to back; a fixed grid of one-wave
+ *                      workgroups strides through them.  Which lattices exist,
+ *                      where their state lives, their timestep and tile shape
+ *                      come from a table in device memory rewritten before
+ *                      each chunk, so the launch itself never changes and a
+ *                      chunk of K steps is one captured hipGraph replayed
+ *                      chunk after chunk.
+ *   pool_wind_kernel   wound-plaquette counts of every resident lattice
+ *                      (ax_wind.h), summed per lattice with integer atomics
+ *                      (order-independent, so deterministic), plus a
+ *                      non-finite flag.
+ *   pool_zero_kernel   clears those counters (a kernel, not a memset, so the
+ *                      graph holds kernel nodes only).
+ *
+ * Host side
+ *   A scheduler thread owns the compute stream: it harvests finished chunks
+ *   (xi samples, completions, freed memory), admits queued requests in
+ *   arrival order while the HBM arena and slot count allow, and keeps up to
+ *   two chunks in flight.  IC worker threads build each request's
+ *   random-phase scalar field on the host (the libstdc++ mt19937 stream of
+ *   ax_init_random_phase) into pinned staging buffers ahead of admission; the
+ *   upload runs on a second stream that the next chunk waits for.
+ */
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <limits>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <random>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "ax_internal.h"
+#include "ax_march.h"
+#include "ax_sched.h"
+#include "ax_wind.h"
+
+namespace ax {
+
+struct BatchEntry {
+  uint64_t base;        /* device address of plane (component 0, level 0) */
+  uint64_t pitch;       /* elements from one plane to the next */
+  int32_t n;            /* lattice edge (n^3 sites) */
+  int32_t nw, xseg, nseg;
+  int32_t work_begin;   /* first step work item of this lattice */
+  int32_t wblk_begin;   /* first 256-site winding block of this lattice */
+  int32_t t0;           /* TimeStep at the start of the chunk */
+  int32_t steps;        /* TimeSteps to take in this chunk (<= chunk length) */
+  int32_t pre;          /* 1: count windings before the first step (t0 == 0) */
+  int32_t pad[3];
+};
+static_assert(sizeof(BatchEntry) == 64, "entry layout");
+
+struct BatchHeader {
+  int32_t count;        /* lattices in the table */
+  int32_t work;         /* total step work items */
+  int32_t wblocks;      /* total winding blocks */
+  int32_t pad;
+};
+
+/* counters per table entry: [phase 0: xy, xz, yz, bad][phase 1: xy, xz, yz, bad] */
+constexpr int kCnt = 8;
+
+__device__ __forceinline__ int find_entry(const BatchEntry *ent, int count, int w, bool by_work) {
+  int lo = 0, hi = count - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    const int b = by_work ? ent[mid].work_begin : ent[mid].wblk_begin;
+    if (b <= w) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+template <typename R>
+__device__ __forceinline__ Levels<R> entry_levels(const BatchEntry &E, int t) {
+  Levels<R> L;
+  R *base = (R *)E.base;
+  const int tN = (t + 1) % 2, tP = t % 2;
+  for (int c = 0; c < 7; ++c) {
+    const int cc = c < 5 ? c : 0;
+    L.cur[c] = base + ((size_t)cc * 2 + tN) * E.pitch;
+    L.prv[c] = base + ((size_t)cc * 2 + tP) * E.pitch;
+  }
+  return L;
+}
+
+template <typename R, bool POW2>
+__global__ void __launch_bounds__(64) pool_step_kernel(ax_params p, R lim, const BatchHeader *__restrict__ hdr,
+                                                       const BatchEntry *__restrict__ ent, int j) {
+  const int count = hdr->count, work = hdr->work;
+  for (int w = (int)blockIdx.x; w < work; w += (int)gridDim.x) {
+    const int e = find_entry(ent, count, w, true);
+    const BatchEntry E = ent[e];
+    if (j >= E.steps) continue;
+    const int local = w - E.work_begin;
+    const int zc = local % E.nw, r = local / E.nw;
+    const int y = r % E.n, seg = r / E.n;
+    const int x0 = seg * E.xseg, x1 = min(E.n, x0 + E.xseg);
+    const int t = E.t0 + j;
+    MarchConsts<R> m;
+    march_consts_from<R>(make_step_consts(p, t), lim, m);
+    ah_march<R, POW2, 2>(m, entry_levels<R>(E, t), E.n, E.n, E.n, y, zc, x0, x1);
+  }
+}
+
+__device__ __forceinline__ int wave_isum(int v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) pool_wind_kernel(const BatchHeader *__restrict__ hdr,
+                                                        const BatchEntry *__restrict__ ent,
+                                                        unsigned long long *counts, int phase) {
+  __shared__ int sh[4][4];
+  const int count = hdr->count, nb = hdr->wblocks;
+  for (int b = (int)blockIdx.x; b < nb; b += (int)gridDim.x) {
+    const int e = find_entry(ent, count, b, false);
+    const BatchEntry E = ent[e];
+    if (phase == 0 && !E.pre) continue;
+    const int n = E.n;
+    const int V = n * n * n;
+    const int s = (b - E.wblk_begin) * 256 + (int)threadIdx.x;
+    const int t = E.t0 + (phase == 0 ? 0 : E.steps);   /* state after t TimeSteps */
+    int w[3] = {0, 0, 0}, bad = 0;
+    if (s < V) {
+      const Levels<T> L = entry_levels<T>(E, t);
+      const int kk = s % n, rr = s / n, jj = rr % n, ii = rr / n;
+      site_winding<T>(n, n, n, L.cur[0], L.cur[1], L.cur[2], L.cur[3], L.cur[4], 1.0, ii, jj, kk, w);
+      for (int c = 0; c < 5; ++c) bad |= !isfinite((double)L.cur[c][s]);
+    }
+    int v[4] = {abs(w[0]), abs(w[1]), abs(w[2]), bad};
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) v[c] = wave_isum(v[c]);
+    __syncthreads();                    /* sh free from the previous block */
+    if (lane == 0)
+      for (int c = 0; c < 4; ++c) sh[wave][c] = v[c];
+    __syncthreads();
+    if (threadIdx.x < 4) {
+      const int c = threadIdx.x;
+      const int tot = sh[0][c] + sh[1][c] + sh[2][c] + sh[3][c];
+      if (tot) atomicAdd(&counts[(size_t)e * kCnt + phase * 4 + c], (unsigned long long)tot);
+    }
+  }
+}
+
+__global__ void __launch_bounds__(256) pool_zero_kernel(const BatchHeader *__restrict__ hdr,
+                                                        unsigned long long *counts) {
+  const int n = hdr->count * kCnt;
+  for (int i = (int)threadIdx.x; i < n; i += 256) counts[i] = 0;
+}
+
+}  // namespace ax
+
+using namespace ax;
+
+namespace {
+
+double now_s(std::chrono::steady_clock::time_point t0) {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+struct Staging {
+  void *host = nullptr;      /* pinned: re plane then im plane */
+  size_t cap = 0;
+  int busy = 0;              /* 0 free, 1 being filled or waiting, 2 upload in flight */
+  hipEvent_t done = nullptr; /* upload finished (busy == 2) */
+};
+
+struct Req {
+  int64_t id = 0;
+  uint32_t seed = 0;
+  int n = 0, steps = 0, flags = 0;
+  int status = AX_REQ_QUEUED;
+  double t_submit = 0, t_admit = 0, t_done = 0;
+  int ic = 0;                /* 0 waiting, 1 building, 2 ready */
+  int staging = -1;
+  int64_t off = -1;          /* arena offset while resident */
+  size_t bytes = 0;
+  int t = 0;                 /* TimeSteps enqueued */
+  bool pre_done = false;
+  int xseg = 32;
+  hipEvent_t upload = nullptr;
+  bool wait_upload = false;
+  std::vector<int32_t> ts;
+  std::vector<int64_t> wound;
+  std::vector<double> xi;
+};
+
+struct ChunkRec {
+  int64_t seq = 0;
+  int buf = 0;
+  hipEvent_t done = nullptr;
+  struct Item {
+    int64_t id;
+    int t0, steps, pre;
+  };
+  std::vector<Item> items;
+  double site_updates = 0;
+};
+
+}  // namespace
+
+struct ax_pool {
+  ax_params p{};
+  ax_pool_config cfg{};
+  size_t esize = 8;
+  size_t pad_bytes = 4352;
+  bool pow2 = false;
+  double lim64 = 0;
+  float lim32 = 0;
+  std::chrono::steady_clock::time_point t0;
+
+  int device = 0;
+  hipStream_t cs = nullptr, us = nullptr;   /* compute, upload */
+  char *arena_base = nullptr;
+  Arena arena;
+  BatchHeader *d_hdr = nullptr;
+  BatchEntry *d_ent = nullptr;
+  unsigned long long *d_cnt = nullptr;
+  static constexpr int kBufs = 3;
+  char *h_tab[kBufs] = {};          /* pinned: header + entries */
+  unsigned long long *h_cnt[kBufs] = {};
+  int grid_step = 0, grid_wind = 0;
+  hipGraphExec_t gexec = nullptr;
+  hipGraph_t graph = nullptr;
+
+  std::mutex mu;
+  std::condition_variable cv_sched, cv_ic, cv_done;
+  bool stop = false;
+  std::string error;
+  std::deque<int64_t> queue;        /* not yet admitted, arrival order */
+  std::vector<int64_t> resident;    /* admission order */
+  std::unordered_map<int64_t, std::unique_ptr<Req>> reqs;
+  std::vector<Staging> staging;
+  std::deque<ChunkRec> inflight;
+  std::vector<std::pair<int64_t, int64_t>> deferred;   /* (request, last chunk that may touch it) */
+  int64_t next_seq = 0;
+  int next_buf = 0;
+  int64_t next_id = 1;
+  std::map<int, int> tuned;         /* n -> xseg */
+  ax_pool_stats st{};
+  std::thread sched;
+  std::vector<std::thread> icw;
+
+  int launch_chunk_kernels(hipStream_t s);
+  int capture_graph();
+  void scheduler();
+  void ic_worker();
+  void harvest(ChunkRec &c);
+  bool admit_head();
+  int enqueue_chunk();
+  int tune(int n, int *xseg, double *ms);
+  void fail_all(const std::string &why);
+  void free_slot(Req &r);
+};
+
+namespace {
+
+template <typename R>
+void build_ic(uint32_t seed, size_t V, double eta, R *re, R *im) {
+  /* the stream and arithmetic of ax_init_random_phase (mpi_evolution.cpp:287-305) */
+  const double pi = 4.0 * atan(1.0);
+  std::mt19937 gen(seed);
+  std::uniform_real_distribution<double> dist(0.0, 2 * pi);
+  for (size_t i = 0; i < V; ++i) {
+    double s, c;
+    sincos(dist(gen), &s, &c);
+    re[i] = (R)(eta * c);
+    im[i] = (R)(eta * s);
+  }
+}
+
+}  // namespace
+
+int ax_pool::launch_chunk_kernels(hipStream_t s) {
+  const int K = cfg.chunk_steps;
+  hipLaunchKernelGGL(pool_zero_kernel, dim3(1), dim3(256), 0, s, d_hdr, d_cnt);
+  if (esize == 8) hipLaunchKernelGGL((pool_wind_kernel<double>), dim3(grid_wind), dim3(256), 0, s, d_hdr, d_ent, d_cnt, 0);
+  else hipLaunchKernelGGL((pool_wind_kernel<float>), dim3(grid_wind), dim3(256), 0, s, d_hdr, d_ent, d_cnt, 0);
+  for (int j = 0; j < K; ++j) {
+    if (esize == 8) {
+      if (pow2) hipLaunchKernelGGL((pool_step_kernel<double, true>), dim3(grid_step), dim3(64), 0, s, p, lim64, d_hdr, d_ent, j);
+      else hipLaunchKernelGGL((pool_step_kernel<double, false>), dim3(grid_step), dim3(64), 0, s, p, lim64, d_hdr, d_ent, j);
+    } else {
+      if (pow2) hipLaunchKernelGGL((pool_step_kernel<float, true>), dim3(grid_step), dim3(64), 0, s, p, lim32, d_hdr, d_ent, j);
+      else hipLaunchKernelGGL((pool_step_kernel<float, false>), dim3(grid_step), dim3(64), 0, s, p, lim32, d_hdr, d_ent, j);
+    }
+  }
+  if (esize == 8) hipLaunchKernelGGL((pool_wind_kernel<double>), dim3(grid_wind), dim3(256), 0, s, d_hdr, d_ent, d_cnt, 1);
+  else hipLaunchKernelGGL((pool_wind_kernel<float>), dim3(grid_wind), dim3(256), 0, s, d_hdr, d_ent, d_cnt, 1);
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? AX_OK : hip_fail(e, "pool chunk launch");
+}
+
+int ax_pool::capture_graph() {
+  hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+  if (e != hipSuccess) return hip_fail(e, "begin capture");
+  int rc = launch_chunk_kernels(cs);
+  e = hipStreamEndCapture(cs, &graph);
+  if (rc) return rc;
+  if (e != hipSuccess) return hip_fail(e, "end capture");
+  e = hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0);
+  if (e != hipSuccess) return hip_fail(e, "graph instantiate");
+  return AX_OK;
+}
+
+void ax_pool::free_slot(Req &r) {
+  if (r.off >= 0) arena.release((size_t)r.off);
+  r.off = -1;
+  if (r.upload) {
+    hipEventDestroy(r.upload);
+    r.upload = nullptr;
+  }
+  resident.erase(std::remove(resident.begin(), resident.end(), r.id), resident.end());
+}
+
+void ax_pool::fail_all(const std::string &why) {
+  error = why;
+  for (auto &kv : reqs) {
+    Req &r = *kv.second;
+    if (r.status == AX_REQ_QUEUED || r.status == AX_REQ_RUNNING) {
+      r.status = AX_REQ_FAILED;
+      r.t_done = now_s(t0);
+      st.failed++;
+    }
+  }
+  queue.clear();
+  stop = true;
+  cv_done.notify_all();
+  cv_ic.notify_all();
+}
+
+void ax_pool::harvest(ChunkRec &c) {
+  const unsigned long long *cnt = h_cnt[c.buf];
+  const double dx = p.dx, dy = p.dy, dz = p.dz;
+  for (size_t e = 0; e < c.items.size(); ++e) {
+    const ChunkRec::Item &it = c.items[e];
+    auto f = reqs.find(it.id);
+    if (f == reqs.end()) continue;
+    Req &r = *f->second;
+    if (r.status != AX_REQ_RUNNING) continue;
+    const double V = (double)r.n * r.n * r.n;
+    for (int ph = 0; ph < 2; ++ph) {
+      if (ph == 0 && !it.pre) continue;
+      const unsigned long long *v = cnt + e * kCnt + ph * 4;
+      if (v[3]) {                          /* non-finite value in the state */
+        r.status = AX_REQ_FAILED;
+        r.t_done = now_s(t0);
+        st.failed++;
+        break;
+      }
+      r.ts.push_back(ph == 0 ? it.t0 : it.t0 + it.steps);
+      r.wound.push_back((int64_t)(v[0] + v[1] + v[2]));
+      r.xi.push_back(xi_of((double)v[0], (double)v[1], (double)v[2], V, dx, dy, dz));
+    }
+    if (r.status == AX_REQ_RUNNING && it.t0 + it.steps >= r.steps) {
+      r.status = AX_REQ_DONE;
+      r.t_done = now_s(t0);
+      st.completed++;
+    }
+    if (r.status == AX_REQ_FAILED) {
+      /* chunks enqueued after this one may still include the lattice:
+       * free its memory only once the last of them has been harvested */
+      r.t = r.steps;
+      resident.erase(std::remove(resident.begin(), resident.end(), r.id), resident.end());
+      deferred.emplace_back(r.id, next_seq - 1);
+    } else if (r.status == AX_REQ_DONE) {
+      if (r.flags & AX_POOL_KEEP_STATE)
+        resident.erase(std::remove(resident.begin(), resident.end(), r.id), resident.end());
+      else
+        free_slot(r);
+    }
+  }
+  for (size_t i = 0; i < deferred.size();) {
+    if (deferred[i].second <= c.seq) {
+      auto f = reqs.find(deferred[i].first);
+      if (f != reqs.end()) free_slot(*f->second);
+      deferred[i] = deferred.back();
+      deferred.pop_back();
+    } else {
+      ++i;
+    }
+  }
+  st.site_updates += c.site_updates;
+  hipEventDestroy(c.done);
+  cv_done.notify_all();
+}
+
+/* Admit the head of the queue if its IC is ready and its state fits. */
+bool ax_pool::admit_head() {
+  if (queue.empty() || (int)resident.size() >= cfg.max_lattices) return false;
+  Req &r = *reqs[queue.front()];
+  if (r.ic != 2) return false;
+  int64_t off = arena.alloc(r.bytes);
+  if (off < 0) return false;
+  queue.pop_front();
+  r.off = off;
+  Staging &sg = staging[r.staging];
+  const size_t V = (size_t)r.n * r.n * r.n;
+  const size_t pitch = lattice_pitch(V, esize, pad_bytes);
+  char *base = arena_base + off;
+  const size_t plane = V * esize, pb = pitch * esize;
+  /* phi at both levels from the staging buffer, links and their previous
+   * level zero: planes (c*2 + level) */
+  hipError_t e = hipMemcpyAsync(base + 0 * pb, sg.host, plane, hipMemcpyHostToDevice, us);
+  if (e == hipSuccess) e = hipMemcpyAsync(base + 2 * pb, (char *)sg.host + plane, plane, hipMemcpyHostToDevice, us);
+  if (e == hipSuccess) e = hipMemcpyAsync(base + 1 * pb, base + 0 * pb, plane, hipMemcpyDeviceToDevice, us);
+  if (e == hipSuccess) e = hipMemcpyAsync(base + 3 * pb, base + 2 * pb, plane, hipMemcpyDeviceToDevice, us);
+  if (e == hipSuccess) e = hipMemsetAsync(base + 4 * pb, 0, 6 * pb, us);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&r.upload, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventRecord(r.upload, us);
+  if (e == hipSuccess && !sg.done) e = hipEventCreateWithFlags(&sg.done, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventRecord(sg.done, us);
+  if (e != hipSuccess) {
+    fail_all(std::string("pool upload: ") + hipGetErrorString(e));
+    return false;
+  }
+  sg.busy = 2;
+  r.staging = -1;
+  r.wait_upload = true;
+  r.status = AX_REQ_RUNNING;
+  r.t_admit = now_s(t0);
+  auto tf = tuned.find(r.n);
+  r.xseg = tf != tuned.end() ? tf->second : cfg.xseg_default;
+  resident.push_back(r.id);
+  st.admitted++;
+  st.max_resident = std::max(st.max_resident, (int32_t)resident.size());
+  return true;
+}
+
+int ax_pool::enqueue_chunk() {
+  const int K = cfg.chunk_steps;
+  const int b = next_buf;
+  next_buf = (next_buf + 1) % kBufs;
+  BatchHeader *h = (BatchHeader *)h_tab[b];
+  BatchEntry *ent = (BatchEntry *)(h_tab[b] + 64);
+  ChunkRec c;
+  c.buf = b;
+  c.seq = next_seq;
+  int count = 0, work = 0, wblk = 0;
+  for (int64_t id : resident) {
+    Req &r = *reqs[id];
+    if (r.status != AX_REQ_RUNNING || r.t >= r.steps) continue;
+    BatchEntry &E = ent[count];
+    const size_t V = (size_t)r.n * r.n * r.n;
+    E.base = (uint64_t)(uintptr_t)(arena_base + r.off);
+    E.pitch = lattice_pitch(V, esize, pad_bytes);
+    E.n = r.n;
+    E.nw = (r.n + 63) / 64;
+    E.xseg = (r.xseg <= 0 || r.xseg > r.n) ? r.n : r.xseg;
+    E.nseg = (r.n + E.xseg - 1) / E.xseg;
+    E.work_begin = work;
+    E.wblk_begin = wblk;
+    E.t0 = r.t;
+    E.steps = std::min(K, r.steps - r.t);
+    E.pre = r.pre_done ? 0 : 1;
+    work += E.nw * r.n * E.nseg;
+    wblk += (int)((V + 255) / 256);
+    c.items.push_back({id, E.t0, E.steps, E.pre});
+    c.site_updates += (double)V * E.steps;
+    if (r.wait_upload) {
+      hipError_t e = hipStreamWaitEvent(cs, r.upload, 0);
+      if (e != hipSuccess) return hip_fail(e, "wait upload");
+      r.wait_upload = false;
+    }
+    r.t += E.steps;
+    r.pre_done = true;
+    ++count;
+  }
+  if (count == 0) return AX_OK;
+  h->count = count;
+  h->work = work;
+  h->wblocks = wblk;
+  h->pad = 0;
+  hipError_t e = hipMemcpyAsync(d_hdr, h_tab[b], 64 + (size_t)count * sizeof(BatchEntry), hipMemcpyHostToDevice, cs);
+  if (e != hipSuccess) return hip_fail(e, "table upload");
+  int rc = cfg.use_graph ? (hipGraphLaunch(gexec, cs) == hipSuccess ? AX_OK : hip_fail(hipGetLastError(), "graph launch"))
+                         : launch_chunk_kernels(cs);
+  if (rc) return rc;
+  e = hipMemcpyAsync(h_cnt[b], d_cnt, (size_t)count * kCnt * sizeof(unsigned long long), hipMemcpyDeviceToHost, cs);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&c.done, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventRecord(c.done, cs);
+  if (e != hipSuccess) return hip_fail(e, "chunk record");
+  st.chunks++;
+  next_seq++;
+  inflight.push_back(std::move(c));
+  return AX_OK;
+}
+
+void ax_pool::scheduler() {
+  hipSetDevice(device);
+  std::unique_lock<std::mutex> lk(mu);
+  while (!stop) {
+    /* staging buffers whose upload finished go back to the IC workers */
+    for (auto &sg : staging)
+      if (sg.busy == 2 && hipEventQuery(sg.done) == hipSuccess) {
+        sg.busy = 0;
+        cv_ic.notify_all();
+      }
+    /* finished chunks */
+    while (!inflight.empty()) {
+      hipError_t q = hipEventQuery(inflight.front().done);
+      if (q == hipErrorNotReady) break;
+      if (q != hipSuccess) {
+        fail_all(std::string("chunk: ") + hipGetErrorString(q));
+        break;
+      }
+      harvest(inflight.front());
+      inflight.pop_front();
+    }
+    if (stop) break;
+    /* a new N is tuned before its first admission, with the GPU drained */
+    const bool head_tune = cfg.autotune && !queue.empty() && !tuned.count(reqs[queue.front()]->n);
+    if (head_tune && inflight.empty()) {
+      const int n = reqs[queue.front()]->n;
+      int xs;
+      double ms;
+      lk.unlock();
+      int rc = tune(n, &xs, &ms);
+      lk.lock();
+      if (rc) tuned[n] = cfg.xseg_default;
+      continue;
+    }
+    bool admitted = false;
+    while (!head_tune && admit_head()) admitted = true;
+    if (stop) break;
+    if ((int)inflight.size() < 2 && !head_tune) {
+      bool any = false;
+      for (int64_t id : resident) {
+        Req &r = *reqs[id];
+        if (r.status == AX_REQ_RUNNING && r.t < r.steps) { any = true; break; }
+      }
+      if (any) {
+        int rc = enqueue_chunk();
+        if (rc) {
+          fail_all(std::string("enqueue: ") + ax_last_error());
+          break;
+        }
+        continue;
+      }
+    }
+    if (admitted) continue;
+    if (!inflight.empty()) {
+      hipEvent_t ev = inflight.front().done;
+      lk.unlock();
+      hipEventSynchronize(ev);
+      lk.lock();
+    } else {
+      cv_sched.wait_for(lk, std::chrono::milliseconds(2));
+    }
+  }
+  /* drain what is in flight before the streams go away */
+  lk.unlock();
+  hipStreamSynchronize(cs);
+  hipStreamSynchronize(us);
+}
+
+void ax_pool::ic_worker() {
+  std::unique_lock<std::mutex> lk(mu);
+  for (;;) {
+    Req *job = nullptr;
+    int sgi = -1;
+    cv_ic.wait(lk, [&] {
+      if (stop) return true;
+      job = nullptr;
+      for (int64_t id : queue) {          /* earliest queued request without an IC */
+        Req &r = *reqs[id];
+        if (r.ic == 0) { job = &r; break; }
+      }
+      if (!job) return false;
+      sgi = -1;
+      for (size_t i = 0; i < staging.size(); ++i)
+        if (staging[i].busy == 0) { sgi = (int)i; break; }
+      return sgi >= 0;
+    });
+    if (stop) return;
+    job->ic = 1;
+    job->staging = sgi;
+    Staging &sg = staging[sgi];
+    sg.busy = 1;
+    const size_t V = (size_t)job->n * job->n * job->n;
+    const size_t need = 2 * V * esize;
+    const uint32_t seed = job->seed;
+    const double eta = p.eta;
+    void *buf = sg.host;
+    const bool grow = sg.cap < need;
+    lk.unlock();
+    bool ok = true;
+    if (grow) {
+      if (buf) hipHostFree(buf);
+      buf = nullptr;
+      ok = hipHostMalloc(&buf, need, hipHostMallocDefault) == hipSuccess;
+    }
+    if (ok) {
+      if (esize == 8) build_ic<double>(seed, V, eta, (double *)buf, (double *)buf + V);
+      else build_ic<float>(seed, V, eta, (float *)buf, (float *)buf + V);
+    }
+    lk.lock();
+    if (grow) {
+      sg.host = buf;
+      sg.cap = ok ? need : 0;
+    }
+    if (!ok) {
+      sg.busy = 0;
+      fail_all("pinned staging allocation failed");
+      return;
+    }
+    job->ic = 2;
+    cv_sched.notify_all();
+  }
+}
+
+int ax_pool::tune(int n, int *xseg, double *ms) {
+  /* scratch lattice inside the arena, random-phase IC, each candidate timed
+   * with the batched kernel over a one-lattice table on its own stream */
+  int64_t off;
+  {
+    std::lock_guard<std::mutex> g(mu);
+    off = arena.alloc(lattice_bytes(n, esize, pad_bytes));
+  }
+  if (off < 0) return fail(AX_EBUDGET, "no room to tune n=%d", n);
+  hipStream_t s = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  BatchHeader *dh = nullptr;
+  int rc = AX_OK;
+  const size_t V = (size_t)n * n * n;
+  const size_t pitch = lattice_pitch(V, esize, pad_bytes);
+  char *base = arena_base + off;
+  std::vector<char> ic(2 * V * esize);
+  if (esize == 8) build_ic<double>(12345u, V, p.eta, (double *)ic.data(), (double *)ic.data() + V);
+  else build_ic<float>(12345u, V, p.eta, (float *)ic.data(), (float *)ic.data() + V);
+  std::vector<int> cand;
+  for (int c : {4, 8, 16, 32, 64, 128, n})
+    if (c <= n && std::find(cand.begin(), cand.end(), c) == cand.end()) cand.push_back(c);
+  double best = 1e300;
+  int bestx = cfg.xseg_default;
+  hipError_t e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreate(&e0);
+  if (e == hipSuccess) e = hipEventCreate(&e1);
+  if (e == hipSuccess) e = hipMalloc((void **)&dh, 64 + sizeof(BatchEntry));
+  const size_t plane = V * esize, pb = pitch * esize;
+  if (e == hipSuccess) e = hipMemcpyAsync(base, ic.data(), plane, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(base + 2 * pb, ic.data() + plane, plane, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(base + pb, base, plane, hipMemcpyDeviceToDevice, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(base + 3 * pb, base + 2 * pb, plane, hipMemcpyDeviceToDevice, s);
+  if (e == hipSuccess) e = hipMemsetAsync(base + 4 * pb, 0, 6 * pb, s);
+  const int reps = 2, warm = 2, timed = 6;
+  for (size_t ci = 0; ci < cand.size() && e == hipSuccess; ++ci) {
+    char tab[64 + sizeof(BatchEntry)];
+    memset(tab, 0, sizeof tab);
+    BatchHeader *h = (BatchHeader *)tab;
+    BatchEntry *E = (BatchEntry *)(tab + 64);
+    E->base = (uint64_t)(uintptr_t)base;
+    E->pitch = pitch;
+    E->n = n;
+    E->nw = (n + 63) / 64;
+    E->xseg = cand[ci];
+    E->nseg = (n + E->xseg - 1) / E->xseg;
+    E->steps = 1 << 30;
+    h->count = 1;
+    h->work = E->nw * n * E->nseg;
+    e = hipMemcpyAsync(dh, tab, sizeof tab, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    for (int rep = 0; rep < reps && e == hipSuccess; ++rep) {
+      for (int j = 0; j < warm + timed; ++j) {
+        if (j == warm) hipEventRecord(e0, s);
+        if (esize == 8) {
+          if (pow2) hipLaunchKernelGGL((pool_step_kernel<double, true>), dim3(grid_step), dim3(64), 0, s, p, lim64, dh, (const BatchEntry *)((char *)dh + 64), j);
+          else hipLaunchKernelGGL((pool_step_kernel<double, false>), dim3(grid_step), dim3(64), 0, s, p, lim64, dh, (const BatchEntry *)((char *)dh + 64), j);
+        } else {
+          if (pow2) hipLaunchKernelGGL((pool_step_kernel<float, true>), dim3(grid_step), dim3(64), 0, s, p, lim32, dh, (const BatchEntry *)((char *)dh + 64), j);
+          else hipLaunchKernelGGL((pool_step_kernel<float, false>), dim3(grid_step), dim3(64), 0, s, p, lim32, dh, (const BatchEntry *)((char *)dh + 64), j);
+        }
+      }
+      hipEventRecord(e1, s);
+      e = hipEventSynchronize(e1);
+      float t = 0;
+      if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+      if (e == hipSuccess && t / timed < best) {
+        best = t / timed;
+        bestx = cand[ci];
+      }
+    }
+  }
+  if (e != hipSuccess) rc = hip_fail(e, "tune");
+  if (s) hipStreamSynchronize(s);
+  if (dh) hipFree(dh);
+  if (e0) hipEventDestroy(e0);
+  if (e1) hipEventDestroy(e1);
+  if (s) hipStreamDestroy(s);
+  std::lock_guard<std::mutex> g(mu);
+  arena.release((size_t)off);
+  if (!rc) {
+    tuned[n] = bestx;
+    *xseg = bestx;
+    *ms = best;
+  }
+  return rc;
+}
+
+extern "C" {
+
+void ax_pool_default_config(ax_pool_config *c) {
+  if (!c) return;
+  memset(c, 0, sizeof *c);
+  c->hbm_budget = 0;
+  c->max_lattices = 1024;
+  c->chunk_steps = 10;
+  c->ic_threads = 8;
+  c->use_graph = 1;
+  c->autotune = 1;
+  c->xseg_default = 32;
+}
+
+int ax_pool_create(const ax_params *tmpl, const ax_pool_config *cfg_in, ax_pool **out) {
+  if (!tmpl || !out) return fail(AX_EINVAL, "NULL argument");
+  *out = nullptr;
+  if (tmpl->model != AX_MODEL_AH || tmpl->boundary != AX_BC_PERIODIC)
+    return fail(AX_EINVAL, "the pool runs periodic Abelian-Higgs lattices");
+  if (tmpl->precision != 64 && tmpl->precision != 32) return fail(AX_EINVAL, "precision %d", tmpl->precision);
+  if (!(tmpl->dx > 0 && tmpl->dy > 0 && tmpl->dz > 0 && tmpl->dt > 0)) return fail(AX_EINVAL, "spacings must be > 0");
+  ax_pool_config cfg;
+  ax_pool_default_config(&cfg);
+  if (cfg_in) cfg = *cfg_in;
+  if (cfg.max_lattices <= 0) cfg.max_lattices = 1024;
+  if (cfg.max_lattices > 4096) return fail(AX_EINVAL, "max_lattices %d > 4096", cfg.max_lattices);
+  if (cfg.chunk_steps <= 0) cfg.chunk_steps = 10;
+  if (cfg.chunk_steps > 1000) return fail(AX_EINVAL, "chunk_steps %d > 1000", cfg.chunk_steps);
+  if (cfg.ic_threads <= 0) cfg.ic_threads = 8;
+  if (cfg.ic_threads > 64) cfg.ic_threads = 64;
+  if (cfg.xseg_default <= 0) cfg.xseg_default = 32;
+
+  std::unique_ptr<ax_pool> P(new (std::nothrow) ax_pool);
+  if (!P) return fail(AX_ENOMEM, "host allocation");
+  P->p = *tmpl;
+  P->cfg = cfg;
+  P->esize = tmpl->precision == 64 ? 8 : 4;
+  const char *pe = getenv("AX_PLANE_PAD_BYTES");
+  P->pad_bytes = pe ? (size_t)atol(pe) : 4352;
+  P->pow2 = is_pow2(tmpl->dx * tmpl->dx) && is_pow2(tmpl->dy * tmpl->dy) && is_pow2(tmpl->dz * tmpl->dz);
+  P->lim64 = rate_limit<double>(tmpl->dt);
+  P->lim32 = rate_limit<float>((float)tmpl->dt);
+  P->t0 = std::chrono::steady_clock::now();
+  hipGetDevice(&P->device);
+
+  auto cleanup = [&](int rc) {
+    ax_pool_destroy(P.release());
+    return rc;
+  };
+  hipError_t e;
+  size_t fr = 0, tot = 0;
+  if ((e = hipMemGetInfo(&fr, &tot)) != hipSuccess) return cleanup(hip_fail(e, "hipMemGetInfo"));
+  size_t budget = cfg.hbm_budget ? cfg.hbm_budget : (size_t)(0.8 * (double)fr);
+  budget = budget / Arena::kAlign * Arena::kAlign;
+  if (budget == 0) return cleanup(fail(AX_EINVAL, "zero budget"));
+  if ((e = hipStreamCreateWithFlags(&P->cs, hipStreamNonBlocking)) != hipSuccess) return cleanup(hip_fail(e, "stream"));
+  if ((e = hipStreamCreateWithFlags(&P->us, hipStreamNonBlocking)) != hipSuccess) return cleanup(hip_fail(e, "stream"));
+  if (hipMalloc((void **)&P->arena_base, budget) != hipSuccess) {
+    hipGetLastError();
+    return cleanup(fail(AX_ENOMEM, "pool arena of %zu bytes", budget));
+  }
+  P->arena.reset(budget);
+  P->cfg.hbm_budget = budget;
+  const size_t tab = 64 + (size_t)cfg.max_lattices * sizeof(BatchEntry);
+  if (hipMalloc((void **)&P->d_hdr, tab) != hipSuccess ||
+      hipMalloc((void **)&P->d_cnt, (size_t)cfg.max_lattices * kCnt * sizeof(unsigned long long)) != hipSuccess) {
+    hipGetLastError();
+    return cleanup(fail(AX_ENOMEM, "pool tables"));
+  }
+  P->d_ent = (BatchEntry *)((char *)P->d_hdr + 64);
+  for (int b = 0; b < ax_pool::kBufs; ++b)
+    if (hipHostMalloc((void **)&P->h_tab[b], tab, hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void **)&P->h_cnt[b], (size_t)cfg.max_lattices * kCnt * sizeof(unsigned long long),
+                      hipHostMallocDefault) != hipSuccess) {
+      hipGetLastError();
+      return cleanup(fail(AX_ENOMEM, "pinned pool tables"));
+    }
+  /* persistent grids: every workgroup slot the device can hold at once */
+  int ncu = 0, bs = 0, bw = 0;
+  hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, P->device);
+  if (P->esize == 8)
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&bs, P->pow2 ? (const void *)pool_step_kernel<double, true>
+                                                                  : (const void *)pool_step_kernel<double, false>, 64, 0);
+  else
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&bs, P->pow2 ? (const void *)pool_step_kernel<float, true>
+                                                                  : (const void *)pool_step_kernel<float, false>, 64, 0);
+  if (e == hipSuccess)
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&bw, P->esize == 8 ? (const void *)pool_wind_kernel<double>
+                                                                         : (const void *)pool_wind_kernel<float>, 256, 0);
+  if (e != hipSuccess) return cleanup(hip_fail(e, "occupancy"));
+  P->grid_step = std::max(1, ncu) * std::max(1, bs);
+  P->grid_wind = std::max(1, ncu) * std::max(1, bw);
+  if (cfg.use_graph) {
+    int rc = P->capture_graph();
+    if (rc) return cleanup(rc);
+  }
+  P->staging.resize((size_t)cfg.ic_threads + 2);
+  P->st.budget = budget;
+  ax_pool *raw = P.get();
+  P->sched = std::thread([raw] { raw->scheduler(); });
+  for (int i = 0; i < cfg.ic_threads; ++i) P->icw.emplace_back([raw] { raw->ic_worker(); });
+  *out = P.release();
+  return AX_OK;
+}
+
+void ax_pool_destroy(ax_pool *P) {
+  if (!P) return;
+  {
+    std::lock_guard<std::mutex> g(P->mu);
+    P->stop = true;
+  }
+  P->cv_sched.notify_all();
+  P->cv_ic.notify_all();
+  P->cv_done.notify_all();
+  if (P->sched.joinable()) P->sched.join();
+  for (auto &t : P->icw)
+    if (t.joinable()) t.join();
+  if (P->cs) hipStreamSynchronize(P->cs);
+  if (P->us) hipStreamSynchronize(P->us);
+  for (auto &c : P->inflight)
+    if (c.done) hipEventDestroy(c.done);
+  for (auto &kv : P->reqs)
+    if (kv.second->upload) hipEventDestroy(kv.second->upload);
+  for (auto &sg : P->staging) {
+    if (sg.done) hipEventDestroy(sg.done);
+    if (sg.host) hipHostFree(sg.host);
+  }
+  if (P->gexec) hipGraphExecDestroy(P->gexec);
+  if (P->graph) hipGraphDestroy(P->graph);
+  for (int b = 0; b < ax_pool::kBufs; ++b) {
+    if (P->h_tab[b]) hipHostFree(P->h_tab[b]);
+    if (P->h_cnt[b]) hipHostFree(P->h_cnt[b]);
+  }
+  if (P->d_hdr) hipFree(P->d_hdr);
+  if (P->d_cnt) hipFree(P->d_cnt);
+  if (P->arena_base) hipFree(P->arena_base);
+  if (P->cs) hipStreamDestroy(P->cs);
+  if (P->us) hipStreamDestroy(P->us);
+  delete P;
+}
+
+int ax_pool_submit(ax_pool *P, uint32_t seed, int32_t n, int32_t steps, int32_t flags, int64_t *id) {
+  if (!P || !id) return fail(AX_EINVAL, "NULL argument");
+  if (n < 3 || steps < 1 || (flags & ~AX_POOL_KEEP_STATE)) return fail(AX_EINVAL, "bad request n=%d steps=%d", n, steps);
+  if ((double)n * n * n * (double)P->esize >= 4294967296.0) return fail(AX_EINVAL, "n=%d too large for the pool", n);
+  const size_t bytes = lattice_bytes(n, P->esize, P->pad_bytes);
+  std::lock_guard<std::mutex> g(P->mu);
+  if (P->stop) return fail(AX_EHIP, "pool stopped: %s", P->error.c_str());
+  if (bytes > P->arena.capacity()) return fail(AX_EBUDGET, "n=%d needs %zu bytes, budget %zu", n, bytes, P->arena.capacity());
+  std::unique_ptr<Req> r(new Req);
+  r->id = P->next_id++;
+  r->seed = seed;
+  r->n = n;
+  r->steps = steps;
+  r->flags = flags;
+  r->bytes = bytes;
+  r->t_submit = now_s(P->t0);
+  *id = r->id;
+  P->queue.push_back(r->id);
+  P->reqs[r->id] = std::move(r);
+  P->st.submitted++;
+  P->cv_ic.notify_all();
+  P->cv_sched.notify_all();
+  return AX_OK;
+}
+
+int ax_pool_wait(ax_pool *P, int64_t id, double timeout_s) {
+  if (!P) return fail(AX_EINVAL, "NULL pool");
+  std::unique_lock<std::mutex> lk(P->mu);
+  auto finished = [&] {
+    if (id >= 0) {
+      auto f = P->reqs.find(id);
+      return f == P->reqs.end() || f->second->status >= AX_REQ_DONE;
+    }
+    for (auto &kv : P->reqs)
+      if (kv.second->status < AX_REQ_DONE) return false;
+    return true;
+  };
+  if (id >= 0 && !P->reqs.count(id)) return fail(AX_ENOTFOUND, "request %lld", (long long)id);
+  const auto until = std::chrono::steady_clock::now() + std::chrono::duration<double>(timeout_s < 0 ? 1e9 : timeout_s);
+  while (!finished()) {
+    if (P->cv_done.wait_until(lk, until) == std::cv_status::timeout && !finished()) return 1;
+  }
+  return AX_OK;
+}
+
+int ax_pool_info(ax_pool *P, int64_t id, ax_request_info *out) {
+  if (!P || !out) return fail(AX_EINVAL, "NULL argument");
+  std::lock_guard<std::mutex> g(P->mu);
+  auto f = P->reqs.find(id);
+  if (f == P->reqs.end()) return fail(AX_ENOTFOUND, "request %lld", (long long)id);
+  const Req &r = *f->second;
+  out->id = r.id;
+  out->seed = r.seed;
+  out->n = r.n;
+  out->steps = r.steps;
+  out->status = r.status;
+  out->nsamples = (int32_t)r.ts.size();
+  out->xseg = r.xseg;
+  out->t_submit = r.t_submit;
+  out->t_admit = r.t_admit;
+  out->t_done = r.t_done;
+  return AX_OK;
+}
+
+int ax_pool_xi(ax_pool *P, int64_t id, int32_t *ts, int64_t *wound, double *xi, size_t cap, size_t *count) {
+  if (!P || !count || (cap && (!ts || !wound || !xi))) return fail(AX_EINVAL, "NULL argument");
+  std::lock_guard<std::mutex> g(P->mu);
+  auto f = P->reqs.find(id);
+  if (f == P->reqs.end()) return fail(AX_ENOTFOUND, "request %lld", (long long)id);
+  const Req &r = *f->second;
+  *count = r.ts.size();
+  const size_t m = std::min(cap, r.ts.size());
+  for (size_t i = 0; i < m; ++i) {
+    ts[i] = r.ts[i];
+    wound[i] = r.wound[i];
+    xi[i] = r.xi[i];
+  }
+  return AX_OK;
+}
+
+int ax_pool_get_state(ax_pool *P, int64_t id, double *host, size_t n) {
+  if (!P || !host) return fail(AX_EINVAL, "NULL argument");
+  std::unique_lock<std::mutex> lk(P->mu);
+  auto f = P->reqs.find(id);
+  if (f == P->reqs.end()) return fail(AX_ENOTFOUND, "request %lld", (long long)id);
+  Req &r = *f->second;
+  if (r.status != AX_REQ_DONE || r.off < 0) return fail(AX_EINVAL, "request %lld has no kept final state", (long long)id);
+  const size_t V = (size_t)r.n * r.n * r.n;
+  if (n != 10 * V) return fail(AX_EINVAL, "state length %zu != %zu", n, 10 * V);
+  const size_t pitch = lattice_pitch(V, P->esize, P->pad_bytes);
+  const char *base = P->arena_base + r.off;
+  lk.unlock();   /* the slot stays allocated until ax_pool_release */
+  hipStream_t s;
+  hipError_t e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+  if (e != hipSuccess) return hip_fail(e, "stream");
+  std::vector<float> tmp(P->esize == 4 ? V : 0);
+  for (int m = 0; m < 10 && e == hipSuccess; ++m) {
+    const char *src = base + (size_t)m * pitch * P->esize;
+    if (P->esize == 8) {
+      e = hipMemcpyAsync(host + (size_t)m * V, src, V * 8, hipMemcpyDeviceToHost, s);
+      if (e == hipSuccess) e = hipStreamSynchronize(s);
+    } else {
+      e = hipMemcpyAsync(tmp.data(), src, V * 4, hipMemcpyDeviceToHost, s);
+      if (e == hipSuccess) e = hipStreamSynchronize(s);
+      for (size_t i = 0; i < V && e == hipSuccess; ++i) host[(size_t)m * V + i] = tmp[i];
+    }
+  }
+  hipStreamDestroy(s);
+  return e == hipSuccess ? AX_OK : hip_fail(e, "pool get_state");
+}
+
+int ax_pool_release(ax_pool *P, int64_t id) {
+  if (!P) return fail(AX_EINVAL, "NULL pool");
+  std::lock_guard<std::mutex> g(P->mu);
+  auto f = P->reqs.find(id);
+  if (f == P->reqs.end()) return fail(AX_ENOTFOUND, "request %lld", (long long)id);
+  Req &r = *f->second;
+  if (r.status < AX_REQ_DONE) return fail(AX_EINVAL, "request %lld is not finished", (long long)id);
+  P->free_slot(r);
+  P->reqs.erase(f);
+  P->cv_sched.notify_all();
+  return AX_OK;
+}
+
+int ax_pool_stats_get(ax_pool *P, ax_pool_stats *out) {
+  if (!P || !out) return fail(AX_EINVAL, "NULL argument");
+  std::lock_guard<std::mutex> g(P->mu);
+  *out = P->st;
+  out->resident = (int32_t)P->resident.size();
+  out->queued = (int32_t)P->queue.size();
+  out->elapsed = now_s(P->t0);
+  out->in_use = P->arena.in_use();
+  out->peak_in_use = P->arena.peak();
+  return AX_OK;
+}
+
+int ax_pool_tune(ax_pool *P, int32_t n, int32_t *xseg, double *ms) {
+  if (!P || !xseg || !ms || n < 3) return fail(AX_EINVAL, "bad tune request");
+  if ((double)n * n * n * (double)P->esize >= 4294967296.0) return fail(AX_EINVAL, "n=%d too large", n);
+  int xs = 0;
+  double t = 0;
+  int rc = P->tune(n, &xs, &t);
+  if (rc) return rc;
+  *xseg = xs;
+  *ms = t;
+  return AX_OK;
+}
+
+}  // extern "C"

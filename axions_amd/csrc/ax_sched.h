@@ -1,0 +1,115 @@
+/*
+ * ax_sched.h - host-side bookkeeping of the ensemble pool, free of HIP so it
+ * can be unit-tested on a machine without a GPU.
+ *
+ *  Arena       first-fit sub-allocator over one device allocation of the
+ *              pool's HBM budget; blocks are 256-byte aligned and freed
+ *              blocks coalesce with their neighbours.
+ *  lattice_bytes  device bytes one request of edge N occupies.
+ *  xi_of       string-length density from the three plaquette-plane counts,
+ *              the expression ax_diagnostics uses.
+ */
+#ifndef AXIONS_AX_SCHED_H
+#define AXIONS_AX_SCHED_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include <iterator>
+#include <map>
+
+namespace ax {
+
+class Arena {
+ public:
+  static constexpr size_t kAlign = 256;
+
+  explicit Arena(size_t capacity = 0) { reset(capacity); }
+
+  void reset(size_t capacity) {
+    cap_ = capacity / kAlign * kAlign;
+    free_.clear();
+    used_.clear();
+    if (cap_) free_[0] = cap_;
+    in_use_ = peak_ = 0;
+  }
+
+  /* offset of a block of at least `bytes`, or -1 when no free range fits */
+  int64_t alloc(size_t bytes) {
+    if (bytes == 0) bytes = 1;
+    const size_t need = (bytes + kAlign - 1) / kAlign * kAlign;
+    for (auto it = free_.begin(); it != free_.end(); ++it) {
+      if (it->second < need) continue;
+      const size_t off = it->first, len = it->second;
+      free_.erase(it);
+      if (len > need) free_[off + need] = len - need;
+      used_[off] = need;
+      in_use_ += need;
+      if (in_use_ > peak_) peak_ = in_use_;
+      return (int64_t)off;
+    }
+    return -1;
+  }
+
+  /* returns false for an offset that is not an allocated block */
+  bool release(size_t off) {
+    auto u = used_.find(off);
+    if (u == used_.end()) return false;
+    size_t len = u->second;
+    used_.erase(u);
+    in_use_ -= len;
+    auto nx = free_.lower_bound(off);
+    if (nx != free_.end() && off + len == nx->first) {   /* merge with the next free range */
+      len += nx->second;
+      nx = free_.erase(nx);
+    }
+    if (nx != free_.begin()) {                            /* merge with the previous one */
+      auto pv = std::prev(nx);
+      if (pv->first + pv->second == off) {
+        pv->second += len;
+        return true;
+      }
+    }
+    free_[off] = len;
+    return true;
+  }
+
+  size_t capacity() const { return cap_; }
+  size_t in_use() const { return in_use_; }
+  size_t peak() const { return peak_; }
+  size_t largest_free() const {
+    size_t m = 0;
+    for (auto &f : free_) m = f.second > m ? f.second : m;
+    return m;
+  }
+  size_t free_ranges() const { return free_.size(); }
+  size_t blocks() const { return used_.size(); }
+
+ private:
+  size_t cap_ = 0, in_use_ = 0, peak_ = 0;
+  std::map<size_t, size_t> free_;   /* offset -> length, disjoint, never adjacent */
+  std::map<size_t, size_t> used_;   /* offset -> length */
+};
+
+/* Abelian-Higgs state of one n^3 lattice: 5 components x 2 levels of
+ * planes, each plane padded by pad_bytes (rounded to the element size) and
+ * the whole block rounded to the arena alignment. */
+inline size_t lattice_pitch(size_t sites, size_t esize, size_t pad_bytes) {
+  return sites + (pad_bytes + esize - 1) / esize;
+}
+inline size_t lattice_bytes(int n, size_t esize, size_t pad_bytes) {
+  const size_t V = (size_t)n * n * n;
+  const size_t b = 10 * lattice_pitch(V, esize, pad_bytes) * esize;
+  return (b + Arena::kAlign - 1) / Arena::kAlign * Arena::kAlign;
+}
+
+/* xi = L / V_phys with L = (2/3)(dz W_xy + dy W_xz + dx W_yz), the same
+ * expression and operand order as ax_diagnostics. */
+inline double xi_of(double wxy, double wxz, double wyz, double sites, double dx, double dy, double dz) {
+  const double len = (2.0 / 3.0) * (wxy * dz + wxz * dy + wyz * dx);
+  return len / (sites * dx * dy * dz);
+}
+
+}  // namespace ax
+
+#endif

@@ -168,6 +168,95 @@ int ax_winding_field(ax_lattice *lat, int32_t field, int8_t *out, size_t n);
  * total count to *count. */
 int ax_crossings(ax_lattice *lat, double *xyz, size_t cap, size_t *count);
 
+/* ------------------------------------------------------------------------
+ * Ensemble pool: many independent realizations (seed, N, steps) of one
+ * periodic Abelian-Higgs parameter set, served with continuous batching.
+ * No counterpart in the reference, which runs one realization per job
+ * (mpi_evolution.cpp:48 seed, :354 TimeStep loop) and leaves ensembles to
+ * job scripts.
+ *
+ *  - Requests are admitted in arrival order whenever their state fits in the
+ *    pool's HBM budget and a slot is free; admitted lattices stay resident
+ *    until they finish.
+ *  - All resident lattices advance together: each chunk of chunk_steps
+ *    TimeSteps is one replay of a captured hipGraph holding chunk_steps
+ *    launches of one persistent kernel that walks every resident lattice's
+ *    work items, so lattices of different N share each launch.
+ *  - After every chunk the wound-plaquette counts of every resident lattice
+ *    are copied back; each request accumulates its xi(t) series
+ *    (t = 0, the end of every chunk, and its last step).
+ *  - Each lattice evolves exactly as it would alone through ax_step() with
+ *    the same parameters (same kernels' arithmetic per site), so results do
+ *    not depend on what else is resident.
+ * The pool runs its own scheduler thread; the calls below are thread-safe.
+ * ---------------------------------------------------------------------- */
+typedef struct ax_pool ax_pool;
+
+typedef struct ax_pool_config {
+  size_t hbm_budget;       /* device bytes for lattice state; 0 = 80% of free memory */
+  int32_t max_lattices;    /* resident at once, 1..4096 (0 = 1024) */
+  int32_t chunk_steps;     /* TimeSteps per graph replay and xi cadence (0 = 10) */
+  int32_t ic_threads;      /* host threads building initial conditions (0 = 8) */
+  int32_t use_graph;       /* 1 replay a captured hipGraph, 0 plain launches */
+  int32_t autotune;        /* 1 time tile shapes per N on first use, 0 use xseg_default */
+  int32_t xseg_default;    /* tile x extent when not tuned (0 = 32) */
+  int32_t reserved;
+} ax_pool_config;
+
+#define AX_POOL_KEEP_STATE 1   /* keep the lattice after it finishes (ax_pool_get_state) */
+
+#define AX_REQ_QUEUED 0
+#define AX_REQ_RUNNING 1
+#define AX_REQ_DONE 2
+#define AX_REQ_FAILED 3        /* non-finite state found; its series stops there */
+
+typedef struct ax_request_info {
+  int64_t id;
+  uint32_t seed;
+  int32_t n, steps, status;
+  int32_t nsamples;        /* xi samples so far */
+  int32_t xseg;            /* tile x extent used */
+  double t_submit, t_admit, t_done;   /* seconds since pool creation (0 if not yet) */
+} ax_request_info;
+
+typedef struct ax_pool_stats {
+  int64_t submitted, admitted, completed, failed;
+  int64_t chunks;                /* graph replays */
+  int32_t resident, queued, max_resident, reserved;
+  double site_updates;           /* sum over completed steps of N^3 */
+  double elapsed;                /* seconds since pool creation */
+  size_t budget, in_use, peak_in_use;
+} ax_pool_stats;
+
+void ax_pool_default_config(ax_pool_config *cfg);
+/* tmpl: model AH, periodic, precision 32 or 64; its nx/ny/nz are ignored
+ * (each request brings its own N).  Allocates the budget up front. */
+int ax_pool_create(const ax_params *tmpl, const ax_pool_config *cfg, ax_pool **out);
+/* Stops the scheduler (pending requests are dropped) and frees everything. */
+void ax_pool_destroy(ax_pool *pool);
+/* Queue a realization: random-phase IC from seed (as ax_init_random_phase),
+ * then steps >= 1 TimeSteps on an n^3 lattice.  AX_EBUDGET if its state
+ * alone exceeds the budget. */
+int ax_pool_submit(ax_pool *pool, uint32_t seed, int32_t n, int32_t steps, int32_t flags, int64_t *id);
+/* Wait until request id (id < 0: every request not yet released) is done
+ * or failed: AX_OK, or 1 if timeout_s (< 0: no limit) passed first. */
+int ax_pool_wait(ax_pool *pool, int64_t id, double timeout_s);
+int ax_pool_info(ax_pool *pool, int64_t id, ax_request_info *out);
+/* The xi(t) series so far: timestep, wound plaquettes (sum over the three
+ * planes) and xi per sample; writes min(count, cap) entries. */
+int ax_pool_xi(ax_pool *pool, int64_t id, int32_t *timestep, int64_t *wound, double *xi, size_t cap,
+               size_t *count);
+/* Final state of a finished AX_POOL_KEEP_STATE request, reference layout
+ * (as ax_get_state). */
+int ax_pool_get_state(ax_pool *pool, int64_t id, double *host, size_t n);
+/* Forget a finished request (frees a kept state). */
+int ax_pool_release(ax_pool *pool, int64_t id);
+int ax_pool_stats_get(ax_pool *pool, ax_pool_stats *out);
+/* Autotuner: time each candidate tile x extent on a scratch n^3 lattice with
+ * the batched kernel and keep the fastest for requests of that N.  Runs on
+ * the scheduler's stream; returns the chosen xseg and its time per step. */
+int ax_pool_tune(ax_pool *pool, int32_t n, int32_t *xseg, double *ms_per_step);
+
 #ifdef __cplusplus
 }
 #endif

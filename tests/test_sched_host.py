@@ -1,0 +1,110 @@
+"""Host bookkeeping of the ensemble pool (axions_amd/csrc/ax_sched.h), on the
+CPU: the HBM arena allocator against a brute-force model, the per-request
+byte count and the xi expression."""
+import ctypes
+import os
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+
+from axions_amd.abi import lattice_bytes
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+@pytest.fixture(scope="module")
+def h():
+    d = tempfile.mkdtemp(prefix="ax_sched_")
+    so = os.path.join(d, "libsched.so")
+    subprocess.check_call(["g++", "-O1", "-std=c++17", "-shared", "-fPIC", "-o", so,
+                           os.path.join(HERE, "csrc", "sched_harness.cpp")])
+    L = ctypes.CDLL(so)
+    vp, sz = ctypes.c_void_p, ctypes.c_size_t
+    L.arena_new.restype = vp
+    L.arena_new.argtypes = [sz]
+    L.arena_delete.argtypes = [vp]
+    L.arena_alloc.restype = ctypes.c_longlong
+    L.arena_alloc.argtypes = [vp, sz]
+    L.arena_release.argtypes = [vp, sz]
+    for f in ("arena_capacity", "arena_in_use", "arena_peak", "arena_largest_free", "arena_free_ranges"):
+        getattr(L, f).restype = sz
+        getattr(L, f).argtypes = [vp]
+    L.lattice_bytes.restype = sz
+    L.lattice_bytes.argtypes = [ctypes.c_int, sz, sz]
+    L.xi_of.restype = ctypes.c_double
+    L.xi_of.argtypes = [ctypes.c_double] * 7
+    return L
+
+
+def test_arena_random_against_model(h):
+    cap = 1 << 20
+    a = h.arena_new(cap + 100)          # rounded down to the 256 B alignment
+    assert h.arena_capacity(a) == cap
+    rng = np.random.default_rng(0)
+    live = {}                           # offset -> size (rounded)
+    peak = 0
+    for it in range(4000):
+        if live and rng.random() < 0.45:
+            off = int(rng.choice(list(live)))
+            assert h.arena_release(a, off) == 1
+            del live[off]
+        else:
+            b = int(rng.integers(1, 40000))
+            off = h.arena_alloc(a, b)
+            need = (b + 255) // 256 * 256
+            # first fit: the lowest free gap that holds `need`
+            edges = sorted(live.items())
+            gaps, pos = [], 0
+            for o, s in edges:
+                if o > pos:
+                    gaps.append((pos, o - pos))
+                pos = o + s
+            if pos < cap:
+                gaps.append((pos, cap - pos))
+            fit = [g for g in gaps if g[1] >= need]
+            if not fit:
+                assert off == -1
+                continue
+            assert off == fit[0][0] and off % 256 == 0
+            live[off] = need
+        used = sum(live.values())
+        peak = max(peak, used)
+        assert h.arena_in_use(a) == used
+        assert h.arena_peak(a) == peak
+        # free ranges are coalesced: one per maximal gap
+        edges = sorted(live.items())
+        ngaps, pos = 0, 0
+        for o, s in edges:
+            ngaps += o > pos
+            pos = o + s
+        ngaps += pos < cap
+        assert h.arena_free_ranges(a) == ngaps
+    assert h.arena_release(a, 12345) == 0  # not a block
+    for off in list(live):
+        h.arena_release(a, off)
+    assert h.arena_in_use(a) == 0 and h.arena_largest_free(a) == cap and h.arena_free_ranges(a) == 1
+    h.arena_delete(a)
+
+
+def test_zero_size_and_exhaustion(h):
+    a = h.arena_new(1024)
+    offs = [h.arena_alloc(a, 0) for _ in range(4)]
+    assert offs == [0, 256, 512, 768] and h.arena_alloc(a, 1) == -1
+    h.arena_delete(a)
+
+
+@pytest.mark.parametrize("n,prec", [(64, 32), (96, 32), (256, 32), (64, 64), (17, 64)])
+def test_lattice_bytes(h, n, prec):
+    es = 8 if prec == 64 else 4
+    assert h.lattice_bytes(n, es, 4352) == lattice_bytes(n, prec)
+    assert lattice_bytes(n, prec) >= 10 * n ** 3 * es
+
+
+def test_xi_expression(h):
+    # the expression of ax_diagnostics: (2/3)(dz Wxy + dy Wxz + dx Wyz) / (V dx dy dz)
+    for w in [(0, 0, 0), (10, 20, 30), (123456, 7, 99)]:
+        V, dx, dy, dz = 64.0 ** 3, 0.5, 0.6, 0.7
+        ref = (2.0 / 3.0) * (w[0] * dz + w[1] * dy + w[2] * dx) / (V * dx * dy * dz)
+        assert h.xi_of(*w, V, dx, dy, dz) == ref
