@@ -1,23 +1,33 @@
 #!/usr/bin/env python3
 """bench.py - throughput of the lattice-evolution hot path on one MI355X.
 
-Default workload (the reference-scale single-GPU configuration): plain
-Abelian-Higgs leapfrog, 512^3 periodic lattice, FP64, dx=0.5, dt=0.1,
-lambda=2, g=eta=1, seeded random-phase IC (mt19937 seed 42), energy +
-Gauss-law + winding-number string count every 20 steps.  A "step" is one
-TimeStep of the fused in-place update over the whole lattice.
+Headline workload (`value`): plain Abelian-Higgs leapfrog, 512^3 periodic
+lattice, FP64, dx=0.5, dt=0.1, lambda=2, g=eta=1, seeded random-phase IC
+(mt19937 seed 42), 1000 steps with energy + Gauss-law + winding-number string
+count every 20 steps.  A "step" is one TimeStep of the fused in-place update
+over the whole lattice.
 
 Prints ONE JSON line:
-  value      whole-job site-updates/s over the timed steps (state resident in
-             HBM; the diagnostic passes every --diag-every steps are inside
-             the timed region)
-  roofline   the step kernel: algorithmic bytes per site-update (120 B FP64,
-             60 B FP32: 10 field values read, 5 written) x sites / the
-             kernel's average duration from HIP events on its stream,
-             against 8 TB/s
-  cpu_baseline  the oracle (C++/OpenMP restatement of the reference step,
-             glibc sin/cos like the reference) timed on this host over a
-             bounded sample of the same lattice
+  value         whole-job site-updates/s over the timed steps (state resident
+                in HBM; the diagnostic passes every --diag-every steps are
+                inside the timed region)
+  roofline      the step kernel: algorithmic bytes per site-update (120 B
+                FP64, 60 B FP32: 10 field values read, 5 written) x sites / the
+                kernel's mean duration from HIP events on its stream, against
+                8 TB/s; traffic = HBM bytes per launch measured by rocprofv3
+                (committed summary under profiles/), null if none
+  cpu_baseline  the oracle (C++/OpenMP restatement of the reference step, glibc
+                sin/cos like the reference) timed on this host over a bounded
+                sample of the same lattice
+  fp32          the same run in FP32 storage/arithmetic (FP64 reductions), its
+                throughput and roofline, and its deviation from the FP64 run in
+                energy, Gauss residual and xi(t)
+  ensemble      ensemble serving: Poisson arrivals of realizations with N drawn
+                from {64,96,128,192,256}, steps = N*dx/(2*dt), served by the
+                FP32 pool (continuous batching, persistent batched kernel,
+                hipGraph chunks); realizations/hour, site-updates/s, p50/p99
+                latency; the same stream run one by one through the FP64
+                single-lattice path for comparison (realizations/hour and xi)
 """
 import argparse
 import json
@@ -30,6 +40,7 @@ sys.path.insert(0, REPO)
 
 HBM_PEAK_GBS = 8000.0
 METRIC = "site-updates/s at 512³ FP64/FP32, % HBM roofline; ensemble p50/p99 latency"
+ENSEMBLE_NS = (64, 96, 128, 192, 256)
 
 
 def measured_traffic(n, precision):
@@ -91,6 +102,190 @@ def copy_calibration(nbytes, reps):
     print(json.dumps({"copy_calibration_bytes": nbytes, "reps": reps}))
 
 
+def run_lattice(ax, n, precision, steps, warmup, diag_every, seed, log):
+    """The headline loop on one lattice: warmup, then `steps` timed steps with
+    diagnostics every diag_every steps; returns throughput, the step kernel's
+    mean time and the diagnostic series."""
+    p = ax.ah_params(n, precision=precision)
+    log(f"lattice {n}^3 FP{precision}: IC seed {seed}")
+    series = []
+    with ax.Lattice(p) as lat:
+        lat.init_random_phase(seed)
+
+        def diag():
+            d = lat.diagnostics(winding=True)
+            if not d["finite"]:
+                raise RuntimeError(f"non-finite state at step {d['timestep']}")
+            series.append((d["timestep"], d["energy"], d["gauss_mean"], d["wound_plaquettes"], d["xi"]))
+
+        def run(nsteps):
+            done = 0
+            while done < nsteps:
+                k = min(diag_every - (lat.timestep % diag_every), nsteps - done)
+                if lat.timestep % diag_every == 0:
+                    diag()
+                lat.step(k)
+                done += k
+            lat.sync()
+
+        log(f"warmup {warmup} steps")
+        run(warmup)
+        lat.kernel_timer(True)
+        lat.kernel_time()
+        log(f"timed {steps} steps (diagnostics every {diag_every})")
+        t0 = time.perf_counter()
+        run(steps)
+        t1 = time.perf_counter()
+        kms, ksteps = lat.kernel_time()
+        lat.kernel_timer(False)
+        diag()
+    elapsed = t1 - t0
+    sites = n ** 3
+    bps = 120 if precision == 64 else 60
+    avg_kernel_s = kms / 1e3 / max(ksteps, 1)
+    achieved = bps * sites / avg_kernel_s / 1e9
+    return {
+        "value": sites * steps / elapsed,
+        "ms_per_step": elapsed / steps * 1e3,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(n, precision),
+                     "kernel_ms": avg_kernel_s * 1e3, "bytes_per_site": bps},
+        "series": series,
+    }
+
+
+def compare_series(a, b):
+    """FP32 series b against FP64 series a at the same timesteps."""
+    import numpy as np
+    A = {r[0]: r for r in a}
+    rows = [(A[r[0]], r) for r in b if r[0] in A]
+    E0a, E0b = rows[0][0][1], rows[0][1][1]
+    drift_a = np.array([(x[1] - E0a) / E0a for x, _ in rows])
+    drift_b = np.array([(y[1] - E0b) / E0b for _, y in rows])
+    eg = np.array([abs(y[1] - x[1]) / abs(x[1]) for x, y in rows])
+    xa = np.array([x[4] for x, _ in rows])
+    xb = np.array([y[4] for _, y in rows])
+    xr = np.abs(xb - xa) / np.maximum(np.abs(xa), 1e-300)
+    return {
+        "timesteps_compared": len(rows),
+        "energy_rel_diff_max": float(eg.max()),
+        "energy_drift_fp64_final": float(drift_a[-1]), "energy_drift_fp32_final": float(drift_b[-1]),
+        "gauss_fp64_final": rows[-1][0][2], "gauss_fp32_final": rows[-1][1][2],
+        "xi_rel_diff_max": float(xr.max()), "xi_rel_diff_final": float(xr[-1]),
+        "xi_fp64_final": float(xa[-1]), "xi_fp32_final": float(xb[-1]),
+    }
+
+
+def ensemble(ax, count, rate, seed, budget_gb, chunk, fp64_compare, log):
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    dx, dt = 0.5, 0.1
+    Ns = [int(v) for v in rng.choice(ENSEMBLE_NS, size=count)]
+    seeds = [int(v) for v in rng.integers(1, 2 ** 31, size=count)]
+    steps = [int(round(N * dx / (2 * dt))) for N in Ns]
+    gaps = rng.exponential(1.0 / rate, size=count)
+    arrivals = np.cumsum(gaps) - gaps[0]
+    total_su = float(sum(N ** 3 * s for N, s in zip(Ns, steps)))
+    threads = max(2, min(12, physical_cores() - 2))
+    log(f"ensemble: {count} requests, Poisson rate {rate}/s, {total_su:.3e} site-updates, {threads} IC threads")
+    out = {"requests": count, "n_choices": list(ENSEMBLE_NS), "arrival_rate_per_s": rate, "chunk_steps": chunk,
+           "site_updates_total": total_su, "dtype": "f32", "reductions": "f64"}
+    with ax.Pool(ax.ah_params(8, dx=dx, dt=dt, precision=32), hbm_budget=int(budget_gb * (1 << 30)),
+                 chunk_steps=chunk, autotune=1, ic_threads=threads) as pool:
+        tuned = {N: pool.tune(N) for N in ENSEMBLE_NS}
+        out["tuned_xseg"] = {str(N): x for N, (x, _) in tuned.items()}
+        log(f"tuned tiles: {out['tuned_xseg']}")
+        ids = []
+        t0 = time.perf_counter()
+        for i in range(count):
+            d = t0 + arrivals[i] - time.perf_counter()
+            if d > 0:
+                time.sleep(d)
+            ids.append(pool.submit(seeds[i], Ns[i], steps[i]))
+        while not pool.wait(timeout=20):
+            s = pool.stats()
+            log(f"  pool: {s['completed']}/{count} done, {s['resident']} resident, {s['queued']} queued")
+        wall = time.perf_counter() - t0
+        infos = [pool.info(i) for i in ids]
+        xi32 = [pool.xi(i) for i in ids]
+        st = pool.stats()
+    lat = np.array([r["t_done"] - r["t_submit"] for r in infos])
+    failed = sum(1 for r in infos if r["status"] != 2)
+    out.update({
+        "failed": failed,
+        "wall_s": wall,
+        "realizations_per_hour": count / wall * 3600,
+        "site_updates_per_s": total_su / wall,
+        "latency_p50_s": float(np.percentile(lat, 50)), "latency_p99_s": float(np.percentile(lat, 99)),
+        "latency_mean_s": float(lat.mean()),
+        "max_resident": st["max_resident"], "peak_hbm_bytes": st["peak_in_use"], "graph_chunks": st["chunks"],
+    })
+    log(f"pool: {out['realizations_per_hour']:.0f} realizations/h, {out['site_updates_per_s']:.3e} SU/s, "
+        f"p50 {out['latency_p50_s']:.2f} s, p99 {out['latency_p99_s']:.2f} s")
+    if not fp64_compare:
+        return out
+    # the same stream, one realization at a time through the FP64 plain path
+    log("fp64 plain path: the same requests one by one")
+    xi64 = []
+    service = []
+    t0 = time.perf_counter()
+    for i in range(count):
+        a = time.perf_counter()
+        ts_, x_ = [], []
+        with ax.Lattice(ax.ah_params(Ns[i], dx=dx, dt=dt)) as L:
+            L.init_random_phase(seeds[i])
+            t = 0
+            while True:
+                d = L.diagnostics(winding=True)
+                ts_.append(t)
+                x_.append(d["xi"])
+                if t == steps[i]:
+                    break
+                k = min(chunk, steps[i] - t)
+                L.step(k)
+                t += k
+        xi64.append((np.array(ts_), np.array(x_)))
+        service.append(time.perf_counter() - a)
+        if (i + 1) % 32 == 0:
+            log(f"  fp64 plain: {i + 1}/{count} done, {time.perf_counter() - t0:.1f} s")
+    wall64 = time.perf_counter() - t0
+    # FIFO single-server latencies for the same arrival times
+    fin, lat64 = 0.0, []
+    for i in range(count):
+        fin = max(fin, arrivals[i]) + service[i]
+        lat64.append(fin - arrivals[i])
+    lat64 = np.array(lat64)
+    rel = []
+    by_n = {}
+    for i in range(count):
+        ts32, _, x32 = xi32[i]
+        ts64, x64 = xi64[i]
+        assert np.array_equal(ts32, ts64)
+        rel.append(abs(x32[-1] - x64[-1]) / x64[-1])
+        by_n.setdefault(Ns[i], []).append((x32, x64))
+    mean_rel = {}
+    for N, rows in by_n.items():
+        m32 = np.mean([r[0] for r in rows], axis=0)
+        m64 = np.mean([r[1] for r in rows], axis=0)
+        mean_rel[str(N)] = float(np.max(np.abs(m32 - m64) / m64))
+    all32 = np.mean([x[2][-1] for x in xi32])
+    all64 = np.mean([x[1][-1] for x in xi64])
+    out["fp64_plain"] = {
+        "wall_s": wall64, "realizations_per_hour": count / wall64 * 3600,
+        "site_updates_per_s": total_su / wall64,
+        "latency_p50_s": float(np.percentile(lat64, 50)), "latency_p99_s": float(np.percentile(lat64, 99)),
+    }
+    out["speedup_vs_fp64_plain"] = out["realizations_per_hour"] / out["fp64_plain"]["realizations_per_hour"]
+    out["xi_vs_fp64"] = {
+        "final_rel_diff_median": float(np.median(rel)), "final_rel_diff_max": float(np.max(rel)),
+        "ensemble_mean_final_rel_diff": float(abs(all32 - all64) / all64),
+        "per_n_ensemble_mean_rel_diff_max_over_t": mean_rel,
+    }
+    log(f"fp64 plain: {out['fp64_plain']['realizations_per_hour']:.0f} realizations/h; "
+        f"speedup {out['speedup_vs_fp64_plain']:.2f}x; xi ensemble diff {out['xi_vs_fp64']['ensemble_mean_final_rel_diff']:.4f}")
+    return out
+
+
 def main():
     if "--copy-calibration" in sys.argv:
         copy_calibration(1 << 30, 3)
@@ -106,6 +301,14 @@ def main():
     ap.add_argument("--cpu-baseline", choices=("on", "off"), default="on")
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--cpu-n", type=int, default=0, help="lattice for the CPU sample (default: --n)")
+    ap.add_argument("--fp32", choices=("on", "off"), default="on", help="FP32 run of the same workload")
+    ap.add_argument("--ensemble", choices=("on", "off"), default="on")
+    ap.add_argument("--ensemble-count", type=int, default=256)
+    ap.add_argument("--ensemble-rate", type=float, default=50.0, help="Poisson arrivals per second")
+    ap.add_argument("--ensemble-seed", type=int, default=2024)
+    ap.add_argument("--ensemble-budget-gb", type=float, default=96.0)
+    ap.add_argument("--ensemble-chunk", type=int, default=10)
+    ap.add_argument("--ensemble-fp64", choices=("on", "off"), default="on")
     ap.add_argument("--quiet", action="store_true")
     a = ap.parse_args()
 
@@ -120,54 +323,29 @@ def main():
         sys.exit(2)
 
     import axions_amd as ax
-    p = ax.ah_params(a.n, precision=a.precision)
-    log(f"lattice {a.n}^3 FP{a.precision}: IC seed {a.seed}")
-    lat = ax.Lattice(p)
-    lat.init_random_phase(a.seed)
-
-    def run(nsteps):
-        done = 0
-        while done < nsteps:
-            k = min(a.diag_every - (lat.timestep % a.diag_every), nsteps - done)
-            if lat.timestep % a.diag_every == 0:
-                d = lat.diagnostics(winding=True)
-                if not d["finite"]:
-                    raise RuntimeError(f"non-finite state at step {d['timestep']}")
-            lat.step(k)
-            done += k
-        lat.sync()
-
-    log(f"warmup {a.warmup} steps")
-    run(a.warmup)
-    lat.kernel_timer(True)
-    lat.kernel_time()
-    log(f"timed {a.steps} steps (diagnostics every {a.diag_every})")
-    t0 = time.perf_counter()
-    run(a.steps)
-    t1 = time.perf_counter()
-    kms, ksteps = lat.kernel_time()
-    lat.kernel_timer(False)
-    dlast = lat.diagnostics(winding=True)
-    elapsed = t1 - t0
-    sites = a.n ** 3
-    value = sites * a.steps / elapsed
-    bytes_per_site = 120 if a.precision == 64 else 60
-    avg_kernel_s = kms / 1e3 / max(ksteps, 1)
-    achieved = bytes_per_site * sites / avg_kernel_s / 1e9
-    lat.close()
-
+    main_run = run_lattice(ax, a.n, a.precision, a.steps, a.warmup, a.diag_every, a.seed, log)
+    fp32 = None
+    if a.fp32 == "on" and a.precision == 64:
+        r32 = run_lattice(ax, a.n, 32, a.steps, a.warmup, a.diag_every, a.seed, log)
+        fp32 = {"value": r32["value"], "unit": "site-updates/s", "ms_per_step": r32["ms_per_step"],
+                "roofline": r32["roofline"], "vs_fp64": compare_series(main_run["series"], r32["series"])}
+    ens = None
+    if a.ensemble == "on":
+        ens = ensemble(ax, a.ensemble_count, a.ensemble_rate, a.ensemble_seed, a.ensemble_budget_gb,
+                       a.ensemble_chunk, a.ensemble_fp64 == "on", log)
     cpu = None
     if a.cpu_baseline == "on":
         cpu = cpu_baseline(a.cpu_n or a.n, a.cpu_steps, min(16, physical_cores()), log)
 
+    last = main_run["series"][-1]
     out = {
         "metric": METRIC,
-        "value": value,
+        "value": main_run["value"],
         "unit": "site-updates/s",
         "n_gpus": 1,
         "steps": a.steps,
         "warmup": a.warmup,
-        "ms_per_step": elapsed / a.steps * 1e3,
+        "ms_per_step": main_run["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -176,12 +354,12 @@ def main():
         "config": {"workload": f"Abelian-Higgs leapfrog {a.n}^3 periodic FP{a.precision}, dx=0.5 dt=0.1 "
                                f"lambda=2 g=eta=1, energy+Gauss+winding every {a.diag_every} steps",
                    "n": a.n, "precision": a.precision, "diag_every": a.diag_every},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(a.n, a.precision),
-                     "kernel_ms": avg_kernel_s * 1e3, "bytes_per_site": bytes_per_site},
+        "roofline": main_run["roofline"],
         "cpu_baseline": cpu,
-        "final_diag": {"timestep": dlast["timestep"], "energy": dlast["energy"],
-                       "gauss_mean": dlast["gauss_mean"], "wound_plaquettes": dlast["wound_plaquettes"]},
+        "fp32": fp32,
+        "ensemble": ens,
+        "final_diag": {"timestep": last[0], "energy": last[1], "gauss_mean": last[2], "wound_plaquettes": last[3],
+                       "xi": last[4]},
     }
     print(json.dumps(out), flush=True)
 
