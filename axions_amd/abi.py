@@ -18,8 +18,10 @@ AX_ENODEV = -7
 
 AX_MODEL_AH = 0
 AX_MODEL_MOORE = 1
+AX_MODEL_GLOBAL = 2
 AX_BC_PERIODIC = 0
 AX_BC_FIXED = 1
+AX_BC_NEUMANN_XY = 2
 
 
 class AxParams(ctypes.Structure):
@@ -98,12 +100,31 @@ def moore_params(n=64, ny=None, nz=None, dx=0.7, dt=0.3, lambda1=2.0, lambda2=2.
     return p
 
 
+def global_params(n=64, ny=None, nz=None, dx=0.25, dt=0.1, dy=None, dz=None, boundary=AX_BC_NEUMANN_XY,
+                  precision=64):
+    """Global-string parameters at leapfrog.cpp:74-81 defaults (dx = 0.25,
+    dt = 0.1, Neumann x/y faces, periodic z); the potential is the program's
+    fixed (|phi|^2 - 1) phi."""
+    p = AxParams()
+    p.model = AX_MODEL_GLOBAL
+    p.nx, p.ny, p.nz = n, (ny or n), (nz or n)
+    p.dx, p.dy, p.dz = dx, (dy if dy is not None else dx), (dz if dz is not None else dx)
+    p.dt = dt
+    p.lambda_, p.eta, p.g = 2.0, 1.0, 0.0
+    p.scaling = 1.0
+    p.boundary, p.precision = boundary, precision
+    return p
+
+
 def ncomp(p):
-    return 5 if p.model == AX_MODEL_AH else 7
+    return {AX_MODEL_AH: 5, AX_MODEL_MOORE: 7, AX_MODEL_GLOBAL: 4}[p.model]
 
 
 def state_len(p):
-    return ncomp(p) * 2 * p.nx * p.ny * p.nz
+    """Host state length: two levels per component, except the global model
+    (field and momentum, one level each)."""
+    levels = 1 if p.model == AX_MODEL_GLOBAL else 2
+    return ncomp(p) * levels * p.nx * p.ny * p.nz
 
 
 AX_POOL_KEEP_STATE = 1

@@ -45,14 +45,17 @@ int hip_fail(hipError_t e, const char *what) {
 
 int check_params(const ax_params *p) {
   if (!p) return fail(AX_EINVAL, "params is NULL");
-  if (p->model != AX_MODEL_AH && p->model != AX_MODEL_MOORE) return fail(AX_EINVAL, "model %d", p->model);
+  if (p->model != AX_MODEL_AH && p->model != AX_MODEL_MOORE && p->model != AX_MODEL_GLOBAL)
+    return fail(AX_EINVAL, "model %d", p->model);
   if (p->nx < 3 || p->ny < 3 || p->nz < 3) return fail(AX_EINVAL, "lattice %dx%dx%d: each axis needs >= 3 sites",
                                                         p->nx, p->ny, p->nz);
   if ((long long)p->nx * p->ny * p->nz > 2147483647LL / 2)
     return fail(AX_EINVAL, "lattice %dx%dx%d exceeds 2^30 sites", p->nx, p->ny, p->nz);
   if (!(p->dx > 0 && p->dy > 0 && p->dz > 0 && p->dt > 0)) return fail(AX_EINVAL, "spacings must be > 0");
   if (p->precision != 64 && p->precision != 32) return fail(AX_EINVAL, "precision %d (64 or 32)", p->precision);
-  if (p->boundary != AX_BC_PERIODIC && p->boundary != AX_BC_FIXED) return fail(AX_EINVAL, "boundary %d", p->boundary);
+  if (p->model == AX_MODEL_GLOBAL ? (p->boundary != AX_BC_PERIODIC && p->boundary != AX_BC_NEUMANN_XY)
+                                   : (p->boundary != AX_BC_PERIODIC && p->boundary != AX_BC_FIXED))
+    return fail(AX_EINVAL, "boundary %d for model %d", p->boundary, p->model);
   if (p->theta_damp_form != 0 && p->theta_damp_form != 1) return fail(AX_EINVAL, "theta_damp_form");
   if (p->energy_form != 0 && p->energy_form != 1) return fail(AX_EINVAL, "energy_form");
   return AX_OK;
@@ -164,9 +167,10 @@ const char *ax_last_error(void) { return g_last_error.c_str(); }
 int ax_state_size(const ax_params *p, int32_t *ncomp, size_t *n) {
   int rc = check_params(p);
   if (rc) return rc;
-  int nc = p->model == AX_MODEL_AH ? 5 : 7;
+  const bool glob = p->model == AX_MODEL_GLOBAL;
+  int nc = glob ? 4 : (p->model == AX_MODEL_AH ? 5 : 7);
   if (ncomp) *ncomp = nc;
-  if (n) *n = (size_t)nc * 2 * p->nx * p->ny * p->nz;
+  if (n) *n = (size_t)nc * (glob ? 1 : 2) * p->nx * p->ny * p->nz;
   return AX_OK;
 }
 
@@ -180,7 +184,9 @@ int ax_create(const ax_params *p, ax_lattice **out) {
   ax_lattice *L = new (std::nothrow) ax_lattice;
   if (!L) return fail(AX_ENOMEM, "host allocation");
   L->p = *p;
-  L->ncomp = p->model == AX_MODEL_AH ? 5 : 7;
+  /* device components: the global model's phi (two buffers) and pi take
+   * 3 x 2 planes */
+  L->ncomp = p->model == AX_MODEL_AH ? 5 : (p->model == AX_MODEL_MOORE ? 7 : 3);
   L->V = (size_t)p->nx * p->ny * p->nz;
   L->esize = p->precision == 64 ? 8 : 4;
   L->xseg = default_xseg(*p);
@@ -261,8 +267,17 @@ int ax_sync(ax_lattice *L) {
 }
 
 /* Upload one field plane (V values) from doubles into level lev of comp. */
-static int upload_plane(ax_lattice *L, int comp, int lev, const double *src, std::vector<float> &tmp) {
-  char *dst = (char *)L->state + ((size_t)comp * 2 + lev) * L->pitch * L->esize;
+/* Host state plane m -> device plane.  AH/Moore: identity ([comp][level]).
+ * Global model: host (Re phi, Im phi, Re pi, Im pi); phi lives in the buffer
+ * of the current TimeStep's parity. */
+static int host_planes(const ax_lattice *L) { return L->p.model == AX_MODEL_GLOBAL ? 4 : 2 * L->ncomp; }
+static int device_plane(const ax_lattice *L, int m) {
+  if (L->p.model != AX_MODEL_GLOBAL) return m;
+  return m < 2 ? 2 * m + L->timestep % 2 : 4 + (m - 2);
+}
+
+static int upload_plane(ax_lattice *L, int plane, const double *src, std::vector<float> &tmp) {
+  char *dst = (char *)L->state + (size_t)plane * L->pitch * L->esize;
   hipError_t e;
   if (L->esize == 8) {
     e = hipMemcpyAsync(dst, src, L->V * 8, hipMemcpyHostToDevice, L->stream);
@@ -277,24 +292,25 @@ static int upload_plane(ax_lattice *L, int comp, int lev, const double *src, std
 
 int ax_set_state(ax_lattice *L, const double *host, size_t n) {
   if (!L || !host) return fail(AX_EINVAL, "NULL argument");
-  if (n != (size_t)L->ncomp * 2 * L->V) return fail(AX_EINVAL, "state length %zu != %zu", n, (size_t)L->ncomp * 2 * L->V);
+  const size_t want = (size_t)host_planes(L) * L->V;
+  if (n != want) return fail(AX_EINVAL, "state length %zu != %zu", n, want);
   std::vector<float> tmp;
-  for (int c = 0; c < L->ncomp; ++c)
-    for (int t = 0; t < 2; ++t) {
-      int rc = upload_plane(L, c, t, host + ((size_t)c * 2 + t) * L->V, tmp);
-      if (rc) return rc;
-    }
+  for (int m = 0; m < host_planes(L); ++m) {
+    int rc = upload_plane(L, device_plane(L, m), host + (size_t)m * L->V, tmp);
+    if (rc) return rc;
+  }
   hipError_t e = hipStreamSynchronize(L->stream);
   return e == hipSuccess ? AX_OK : hip_fail(e, "set_state");
 }
 
 int ax_get_state(ax_lattice *L, double *host, size_t n) {
   if (!L || !host) return fail(AX_EINVAL, "NULL argument");
-  if (n != (size_t)L->ncomp * 2 * L->V) return fail(AX_EINVAL, "state length %zu != %zu", n, (size_t)L->ncomp * 2 * L->V);
+  const size_t want = (size_t)host_planes(L) * L->V;
+  if (n != want) return fail(AX_EINVAL, "state length %zu != %zu", n, want);
   hipError_t e = hipSuccess;
   std::vector<float> tmp(L->esize == 4 ? L->V : 0);
-  for (int m = 0; m < 2 * L->ncomp && e == hipSuccess; ++m) {
-    const char *src = (const char *)L->state + (size_t)m * L->pitch * L->esize;
+  for (int m = 0; m < host_planes(L) && e == hipSuccess; ++m) {
+    const char *src = (const char *)L->state + (size_t)device_plane(L, m) * L->pitch * L->esize;
     double *dst = host + (size_t)m * L->V;
     if (L->esize == 8) {
       e = hipMemcpyAsync(dst, src, L->V * 8, hipMemcpyDeviceToHost, L->stream);
@@ -357,8 +373,8 @@ int ax_init_random_phase(ax_lattice *L, uint32_t seed, int32_t mode) {
     double amp = !moore ? L->p.eta : (f == 0 ? L->p.eta1 : L->p.eta2);
     fill_sincos(ph, per, (per == 2) ? (size_t)f : 0, amp, re, im);
     for (int t = 0; t < 2; ++t) {
-      int rc = upload_plane(L, 2 * f, t, re.data(), tmp);
-      if (!rc) rc = upload_plane(L, 2 * f + 1, t, im.data(), tmp);
+      int rc = upload_plane(L, (2 * f) * 2 + t, re.data(), tmp);
+      if (!rc) rc = upload_plane(L, (2 * f + 1) * 2 + t, im.data(), tmp);
       if (rc) return rc;
     }
     e = hipStreamSynchronize(L->stream);  /* re/im are reused for the next field */
@@ -429,12 +445,23 @@ int ax_kernel_time(ax_lattice *L, double *ms, int64_t *steps) {
 int ax_diagnostics(ax_lattice *L, int32_t want_winding, ax_diag *out) {
   if (!L || !out) return fail(AX_EINVAL, "NULL argument");
   const ax_params &p = L->p;
-  DiagConsts k = diag_consts(&p);
+  const bool glob = p.model == AX_MODEL_GLOBAL;
   hipError_t e;
-  if (L->esize == 8)
-    e = launch_diag<double>(k, levels_of<double>(L, L->timestep), L->part, L->dres, L->stream);
-  else
-    e = launch_diag<float>(k, levels_of<float>(L, L->timestep), L->part, L->dres, L->stream);
+  if (glob) {
+    /* the energy leapfrog.cpp prints for the last TimeStep (its pre-step
+     * field is still in the other buffer); none before the first step */
+    if (L->timestep > 0)
+      e = L->esize == 8 ? launch_global_energy<double>(p, L->state, L->pitch, L->timestep - 1, L->part, L->dres, L->stream)
+                        : launch_global_energy<float>(p, L->state, L->pitch, L->timestep - 1, L->part, L->dres, L->stream);
+    else
+      e = hipMemsetAsync(L->dres, 0, 3 * sizeof(double), L->stream);
+  } else {
+    DiagConsts k = diag_consts(&p);
+    if (L->esize == 8)
+      e = launch_diag<double>(k, levels_of<double>(L, L->timestep), L->part, L->dres, L->stream);
+    else
+      e = launch_diag<float>(k, levels_of<float>(L, L->timestep), L->part, L->dres, L->stream);
+  }
   if (e != hipSuccess) return hip_fail(e, "diag launch");
   if (want_winding) {
     int rc = ensure_winding_buffer(L);
@@ -447,13 +474,13 @@ int ax_diagnostics(ax_lattice *L, int32_t want_winding, ax_diag *out) {
   if (e != hipSuccess) return hip_fail(e, "diag readback");
   memset(out, 0, sizeof *out);
   out->timestep = L->timestep;
-  out->energy = L->hres[0];
+  out->energy = (glob && L->timestep == 0) ? NAN : L->hres[0];
   double dev = L->hres[1];
   if (p.boundary == AX_BC_PERIODIC) {
     dev = dev / (p.nx * p.ny);                             /* evolution.cpp:1404-1408 */
     dev = dev / p.nz;
   }
-  out->gauss_mean = (p.model == AX_MODEL_AH && p.energy_form == 1) ? 0.0 : dev;
+  out->gauss_mean = ((p.model == AX_MODEL_AH && p.energy_form == 1) || glob) ? 0.0 : dev;
   out->finite = L->hres[2] == 0 ? 1 : 0;
   if (want_winding) {
     const double wxy = L->hres[4], wxz = L->hres[5], wyz = L->hres[6];
@@ -502,7 +529,7 @@ int ax_crossings(ax_lattice *L, double *xyz, size_t cap, size_t *count) {
     }
     L->cxyz_cap = cap;
   }
-  const int tN = (L->timestep + 1) % 2;
+  const int tN = p.model == AX_MODEL_GLOBAL ? L->timestep % 2 : (L->timestep + 1) % 2;
   if (L->esize == 8) {
     const double *b = (const double *)L->state;
     e = launch_crossings<double>(p.nx, p.ny, p.nz, p.dx, p.dy, p.dz, b + (size_t)tN * L->pitch, b + (size_t)(2 + tN) * L->pitch,
@@ -542,12 +569,15 @@ int ensure_winding_buffer(ax_lattice *L) {
 
 int launch_winding_field(ax_lattice *L, int field, int8_t *out, double *part, double *res3) {
   const ax_params &p = L->p;
-  const int tN = (L->timestep + 1) % 2;
-  const int c0 = p.model == AX_MODEL_AH ? 0 : 2 * field;
-  const int t0 = p.model == AX_MODEL_AH ? 2 : 4;
-  const double q = p.model == AX_MODEL_AH ? 1.0 : (double)(field == 0 ? p.q1 : p.q2);
+  const bool glob = p.model == AX_MODEL_GLOBAL;
+  const int tN = glob ? L->timestep % 2 : (L->timestep + 1) % 2;
+  const int c0 = p.model == AX_MODEL_MOORE ? 2 * field : 0;
+  const int t0 = p.model == AX_MODEL_MOORE ? 4 : 2;
+  /* global model: no links; charge 0 makes every link phase exp(0) (the
+   * momentum planes only stand in as finite link operands) */
+  const double q = p.model == AX_MODEL_AH ? 1.0 : glob ? 0.0 : (double)(field == 0 ? p.q1 : p.q2);
   hipError_t e;
-  auto P = [&](int c) { return (size_t)(c * 2 + tN) * L->pitch; };
+  auto P = [&](int c) { return glob && c >= 2 ? (size_t)4 * L->pitch : (size_t)(c * 2 + tN) * L->pitch; };
   if (L->esize == 8) {
     const double *b = (const double *)L->state;
     e = launch_winding<double>(p.nx, p.ny, p.nz, b + P(c0), b + P(c0 + 1), b + P(t0), b + P(t0 + 1), b + P(t0 + 2), q,
@@ -561,8 +591,13 @@ int launch_winding_field(ax_lattice *L, int field, int8_t *out, double *part, do
 }
 
 int launch_step(ax_lattice *L, int n) {
-  StepConsts k = step_consts(&L->p, n);
   hipError_t e;
+  if (L->p.model == AX_MODEL_GLOBAL) {
+    e = L->esize == 8 ? launch_global_step<double>(L->p, L->state, L->pitch, n, L->stream)
+                      : launch_global_step<float>(L->p, L->state, L->pitch, n, L->stream);
+    return e == hipSuccess ? AX_OK : hip_fail(e, "global step launch");
+  }
+  StepConsts k = step_consts(&L->p, n);
   const bool tiled = use_tiled_ah(L->p);
   if (L->esize == 8) {
     Levels<double> lv = levels_of<double>(L, n);

@@ -43,6 +43,12 @@ int default_xseg(const ax_params &p);
 template <typename R>
 hipError_t launch_step_tiled_ah(const StepConsts &k, const Levels<R> &L, int xseg, hipStream_t st);
 
+template <typename R>
+hipError_t launch_global_step(const ax_params &p, void *state, size_t pitch, int n, hipStream_t st);
+template <typename R>
+hipError_t launch_global_energy(const ax_params &p, const void *state, size_t pitch, int n, double *part,
+                                double *out3, hipStream_t st);
+
 template <typename T>
 hipError_t launch_diag(const DiagConsts &k, const Levels<T> &L, double *part, double *out3, hipStream_t st);
 template <typename T>

@@ -44,9 +44,13 @@ extern "C" {
 
 #define AX_MODEL_AH 0      /* Abelian-Higgs: one complex scalar + U(1) links */
 #define AX_MODEL_MOORE 1   /* two complex scalars, charges q1,q2, one U(1) */
+#define AX_MODEL_GLOBAL 2  /* one complex scalar, no gauge field, momentum (pi)
+                              form of the leapfrog (leapfrog.cpp:200-347) */
 
 #define AX_BC_PERIODIC 0   /* periodic in x, y, z (mpi_evolution.cpp:385-405) */
 #define AX_BC_FIXED 1      /* edge sites never updated (mooreEvol.cpp:187-189) */
+#define AX_BC_NEUMANN_XY 2 /* global model: x and y faces copied from their inner
+                              neighbours each step, z periodic (leapfrog.cpp:168-193) */
 
 /* Run parameters.  Names and meanings follow the reference's file-scope
  * consts (evolution.cpp:23-59, mpi_evolution.cpp:17-60, mooreEvol.cpp:19-42).
@@ -73,7 +77,8 @@ typedef struct ax_params {
   /* 0: energy as evolution.cpp:403-427 (E_i = dtheta/(g dt dx_i), zero gauge
    *    terms when g == 0);  1: as mpi_evolution.cpp:513-530. */
   int32_t energy_form;
-  int32_t boundary;         /* AX_BC_PERIODIC or AX_BC_FIXED */
+  int32_t boundary;         /* AX_BC_PERIODIC, AX_BC_FIXED (AH, Moore) or
+                               AX_BC_NEUMANN_XY (global) */
   int32_t precision;        /* 64: FP64 drop-in; 32: FP32 storage+arithmetic */
 } ax_params;
 
@@ -95,7 +100,10 @@ void ax_default_params(ax_params *p);
 const char *ax_strerror(int code);
 const char *ax_last_error(void);
 /* Number of stored components (5 AH, 7 Moore) and the host array length
- * ncomp * 2 * nx * ny * nz that ax_set_state/ax_get_state exchange. */
+ * ncomp * 2 * nx * ny * nz that ax_set_state/ax_get_state exchange.  The
+ * global model stores its field and momentum instead of two levels: 4
+ * components (Re phi, Im phi, Re pi, Im pi), one level each, n = 4 nx ny nz
+ * (phi and phit of leapfrog.cpp:83). */
 int ax_state_size(const ax_params *p, int32_t *ncomp, size_t *n);
 
 /* Allocate device state (zeroed) on the current HIP device.

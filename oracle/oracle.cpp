@@ -18,11 +18,13 @@
  *   face crossings ........................... mpi_evolution.cpp:536-789
  *   Moore step, energy, Gauss ................ mooreEvol.cpp:184-395, 647-697
  *   periodic neighbour indices ............... mpi_evolution.cpp:385-405
+ *   global string, momentum-form leapfrog .... leapfrog.cpp:166-347
  * The per-plaquette winding number has no reference counterpart.
  *
  * Pinning (what checks this file, see tests/): compiled reference runs in
  * tests/golden (bit-exact state hashes and sampled sites for AH g=0 and g=1,
- * and for Moore with fixed edges), the std::mt19937 known-answer value, the
+ * for Moore with fixed edges and for the global leapfrog), the std::mt19937
+ * known-answer value, the
  * libstdc++ <random> routines themselves, and invariants (vacuum, lattice
  * translation, gauge covariance, straight string winding, cube sums).
  */
@@ -704,6 +706,88 @@ void run_steps_r(const ax_params *p, R *st, int ts0, int nsteps, int nthreads) {
   }
 }
 
+/* ---------------- global string, momentum-form leapfrog ------------------
+ * leapfrog.cpp:166-347, one TimeStep:
+ *   Neumann copies of the x and y faces (y faces first, then x faces over
+ *   all j, :168-193) unless periodic;
+ *   phitt = phixx + phiyy + phizz - (|phi|^2 - 1) phi over the updated sites
+ *   from the pre-step field (:198-260);
+ *   then, site by site in loop order (:266-347): phit += phitt*dt, the energy
+ *   density from central differences of the field as the array holds it at
+ *   that moment (neighbours earlier in the loop order already updated) with
+ *   the momentum rolled back by dt/2, then phi += phit*dt.
+ * Single-threaded here, so the energy has the loop order of a one-thread
+ * run (with several OpenMP threads the reference's mix of updated and
+ * not-yet-updated neighbours depends on the thread schedule).  State layout
+ * [Re phi, Im phi, Re pi, Im pi] x sites; R is the storage/arithmetic type,
+ * the energy is accumulated in double. */
+template <class R>
+void global_steps(const ax_params *p, R *st, int nsteps, double *energy) {
+  Lat L(p);
+  const int nx = L.nx, ny = L.ny, nz = L.nz;
+  const bool per = p->boundary == AX_BC_PERIODIC;
+  R *ph[2] = {st, st + L.V}, *pt[2] = {st + 2 * L.V, st + 3 * L.V};
+  std::vector<R> tt(2 * L.V);
+  const R dx = (R)p->dx, dy = (R)p->dy, dz = (R)p->dz, dt = (R)p->dt;
+  const R dx2 = (R)(p->dx * p->dx), dy2 = (R)(p->dy * p->dy), dz2 = (R)(p->dz * p->dz);
+  const int i0 = per ? 0 : 1, i1 = per ? nx : nx - 1, j0 = per ? 0 : 1, j1 = per ? ny : ny - 1;
+  for (int n = 0; n < nsteps; ++n) {
+    if (!per) {
+      for (int c = 0; c < 2; ++c) {
+        R *f = ph[c];
+        for (int i = 1; i < nx - 1; ++i)
+          for (int k = 0; k < nz; ++k) {
+            f[L.id(i, 0, k)] = f[L.id(i, 1, k)];
+            f[L.id(i, ny - 1, k)] = f[L.id(i, ny - 2, k)];
+          }
+        for (int j = 0; j < ny; ++j)
+          for (int k = 0; k < nz; ++k) {
+            f[L.id(0, j, k)] = f[L.id(1, j, k)];
+            f[L.id(nx - 1, j, k)] = f[L.id(nx - 2, j, k)];
+          }
+      }
+    }
+    for (int i = i0; i < i1; ++i)
+      for (int j = j0; j < j1; ++j)
+        for (int k = 0; k < nz; ++k) {
+          Nb b(L, i, j, k);
+          const size_t s = L.id(i, j, k);
+          const R mag = ph[0][s] * ph[0][s] + ph[1][s] * ph[1][s];
+          for (int c = 0; c < 2; ++c) {
+            const R *f = ph[c];
+            R xx = (f[L.id(b.ip, j, k)] - 2 * f[s] + f[L.id(b.im, j, k)]) / dx2;
+            R yy = (f[L.id(i, b.jp, k)] - 2 * f[s] + f[L.id(i, b.jm, k)]) / dy2;
+            R zz = (f[L.id(i, j, b.kp)] - 2 * f[s] + f[L.id(i, j, b.km)]) / dz2;
+            tt[c * L.V + s] = xx + yy + zz - (mag - 1) * f[s];
+          }
+        }
+    double E = 0;
+    for (int i = i0; i < i1; ++i)
+      for (int j = j0; j < j1; ++j)
+        for (int k = 0; k < nz; ++k) {
+          Nb b(L, i, j, k);
+          const size_t s = L.id(i, j, k);
+          const R mag = ph[0][s] * ph[0][s] + ph[1][s] * ph[1][s];
+          double gx[2], gy[2], gz[2];
+          for (int c = 0; c < 2; ++c) {
+            pt[c][s] += tt[c * L.V + s] * dt;
+            const R *f = ph[c];
+            gx[c] = (double)((f[L.id(b.ip, j, k)] - f[L.id(b.im, j, k)]) / (2 * dx));
+            gy[c] = (double)((f[L.id(i, b.jp, k)] - f[L.id(i, b.jm, k)]) / (2 * dy));
+            gz[c] = (double)((f[L.id(i, j, b.kp)] - f[L.id(i, j, b.km)]) / (2 * dz));
+          }
+          double r0 = (double)pt[0][s] - 0.5 * (double)tt[s] * p->dt;
+          double r1 = (double)pt[1][s] - 0.5 * (double)tt[L.V + s] * p->dt;
+          double dm = (double)mag - 1;
+          double rho = r0 * r0 + r1 * r1 + gx[0] * gx[0] + gx[1] * gx[1] + gy[0] * gy[0] + gy[1] * gy[1] +
+                       gz[0] * gz[0] + gz[1] * gz[1] + 0.5 * (dm * dm);
+          E += p->dx * p->dy * p->dz * rho;
+          for (int c = 0; c < 2; ++c) ph[c][s] += pt[c][s] * dt;
+        }
+    if (energy) energy[n] = E;
+  }
+}
+
 }  // namespace
 
 /* ================================ C ABI =================================== */
@@ -730,6 +814,18 @@ int orc_init_random_phase(const ax_params *p, uint32_t seed, int mode, double *s
   Lat L(p);
   MT m(seed);
   const double pi = 4.0 * atan(1.0);
+  if (p->model == AX_MODEL_GLOBAL) {
+    /* [Re phi, Im phi, Re pi, Im pi]: the same phases, momentum zero */
+    memset(st, 0, sizeof(double) * 4 * L.V);
+    for (size_t s = 0; s < L.V; ++s) {
+      double ph = m.canonical() * (2 * pi - 0.0) + 0.0;
+      double sn, cs;
+      sincos(ph, &sn, &cs);
+      st[s] = p->eta * cs;
+      st[L.V + s] = p->eta * sn;
+    }
+    return 0;
+  }
   const int nc = (p->model == AX_MODEL_AH) ? 5 : 7;
   memset(st, 0, sizeof(double) * (size_t)nc * 2 * L.V);
   for (size_t s = 0; s < L.V; ++s) {
@@ -761,6 +857,7 @@ int orc_init_random_phase(const ax_params *p, uint32_t seed, int mode, double *s
 
 /* trig_mode 0: the shared ax_trig.h functions; 1: libm sin/cos. */
 int orc_step(const ax_params *p, double *st, int ts0, int nsteps, int trig_mode, int nthreads) {
+  if (p->model == AX_MODEL_GLOBAL) return -1;
   if (trig_mode == 1)
     run_steps<TrigLibm>(p, st, ts0, nsteps, nthreads);
   else
@@ -779,7 +876,20 @@ int orc_step_r(const ax_params *p, void *st, int ts0, int nsteps, int nthreads) 
   return 0;
 }
 
+/* global model (AX_MODEL_GLOBAL): nsteps TimeSteps of leapfrog.cpp on st
+ * ([Re phi, Im phi, Re pi, Im pi], float* for precision 32), the energy of
+ * each step into energy[0..nsteps) (may be NULL). */
+int orc_global_step(const ax_params *p, void *st, int nsteps, double *energy) {
+  if (p->model != AX_MODEL_GLOBAL) return -1;
+  if (p->precision == 32)
+    global_steps<float>(p, (float *)st, nsteps, energy);
+  else
+    global_steps<double>(p, (double *)st, nsteps, energy);
+  return 0;
+}
+
 int orc_diag(const ax_params *p, const double *st, int n, int trig_mode, double *out) {
+  if (p->model == AX_MODEL_GLOBAL) return -1;
   if (p->model == AX_MODEL_AH) {
     if (trig_mode == 1) diag_ah<TrigLibm>(p, st, n, out);
     else diag_ah<TrigAx>(p, st, n, out);
@@ -798,11 +908,13 @@ int orc_diag(const ax_params *p, const double *st, int n, int trig_mode, double 
 
 /* field: 0 = phi (AH) / phi1 (Moore), 1 = phi2 (Moore) */
 int orc_winding(const ax_params *p, const double *st, int n, int field, int8_t *out) {
+  if (p->model == AX_MODEL_GLOBAL) return -1;
   winding_impl(p, st, n, out, field);
   return 0;
 }
 
 int orc_crossings(const ax_params *p, const double *st, int n, double *xyz, size_t cap, size_t *count) {
+  if (p->model == AX_MODEL_GLOBAL) return -1;
   Pts P{xyz, cap, 0};
   crossings_impl(p, st, n, P);
   *count = P.n;

@@ -31,6 +31,7 @@ def lib():
         L.orc_init_random_phase.argtypes = [P, ctypes.c_uint32, ctypes.c_int, vp]
         L.orc_step.argtypes = [P, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]
         L.orc_step_r.argtypes = [P, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        L.orc_global_step.argtypes = [P, vp, ctypes.c_int, vp]
         L.orc_diag.argtypes = [P, vp, ctypes.c_int, ctypes.c_int, vp]
         L.orc_winding.argtypes = [P, vp, ctypes.c_int, ctypes.c_int, vp]
         L.orc_crossings.argtypes = [P, vp, ctypes.c_int, vp, ctypes.c_size_t,
@@ -60,7 +61,7 @@ def init_random_phase(p, seed, mode=0):
 
 def step(p, st, ts0, nsteps, trig_mode=0, nthreads=1):
     assert st.dtype == np.float64 and st.flags.c_contiguous and st.size == state_len(p)
-    lib().orc_step(ctypes.byref(p), st.ctypes.data, ts0, nsteps, trig_mode, nthreads)
+    assert lib().orc_step(ctypes.byref(p), st.ctypes.data, ts0, nsteps, trig_mode, nthreads) == 0
 
 
 def step_r(p, st, ts0, nsteps, nthreads=1):
@@ -71,9 +72,19 @@ def step_r(p, st, ts0, nsteps, nthreads=1):
     assert rc == 0
 
 
+def global_step(p, st, nsteps):
+    """Global-model steps (leapfrog.cpp) on st = [Re phi, Im phi, Re pi, Im pi];
+    returns the energy of each step."""
+    dt = np.float32 if p.precision == 32 else np.float64
+    assert st.dtype == dt and st.flags.c_contiguous and st.size == 4 * p.nx * p.ny * p.nz
+    e = np.empty(max(nsteps, 1))
+    assert lib().orc_global_step(ctypes.byref(p), st.ctypes.data, nsteps, e.ctypes.data) == 0
+    return e[:nsteps]
+
+
 def diag(p, st, n, trig_mode=0):
     out = np.zeros(3)
-    lib().orc_diag(ctypes.byref(p), st.ctypes.data, n, trig_mode, out.ctypes.data)
+    assert lib().orc_diag(ctypes.byref(p), st.ctypes.data, n, trig_mode, out.ctypes.data) == 0
     return {"energy": out[0], "gauss": out[1], "finite": bool(out[2])}
 
 

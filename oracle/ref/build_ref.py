@@ -59,6 +59,10 @@ CONFIGS = {
     # Moore model as shipped (fixed edges), reference couplings
     "moore_shared": ("mooreEvol.cpp", dict(nx=12, ny=12, nz=12, nt=30, makeGif="false"), True),
     "moore_libm": ("mooreEvol.cpp", dict(nx=12, ny=12, nz=12, nt=30, makeGif="false"), False),
+    # global string, momentum-form leapfrog (no trigonometry): the program's
+    # own spacings, and an anisotropic set with non-power-of-two spacings
+    "global_neumann": ("leapfrog.cpp", dict(nx=12, ny=10, nz=14, nt=40), False),
+    "global_aniso": ("leapfrog.cpp", dict(nx=9, ny=11, nz=13, nt=30, dx=0.3, dy=0.35, dz=0.28, dt=0.07), False),
 }
 
 

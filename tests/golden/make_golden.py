@@ -27,7 +27,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 REPO = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, REPO)
 
-from axions_amd.abi import ah_params, moore_params, AX_BC_FIXED  # noqa: E402
+from axions_amd.abi import ah_params, global_params, moore_params, AX_BC_FIXED  # noqa: E402
 from oracle import oracle_py as orc  # noqa: E402
 sys.path.insert(0, os.path.join(REPO, "oracle", "ref"))
 import build_ref  # noqa: E402
@@ -49,6 +49,9 @@ SPECS = {
                                         damped_nt=0, dampFac=0.5, theta_damp_form=1), 200, 10, 0, "shared"),
     "moore_shared": (lambda: moore_params(12, boundary=AX_BC_FIXED), 30, None, None, "shared"),
     "moore_libm": (lambda: moore_params(12, boundary=AX_BC_FIXED), 30, None, None, "libm"),
+    "global_neumann": (lambda: global_params(12, ny=10, nz=14), 40, None, None, "none"),
+    "global_aniso": (lambda: global_params(9, ny=11, nz=13, dx=0.3, dy=0.35, dz=0.28, dt=0.07), 30, None, None,
+                     "none"),
 }
 
 
@@ -66,9 +69,40 @@ def write_ic(p, st, path):
     np.savetxt(path, cols, fmt="%.17g")
 
 
+def run_global(name, p, nt):
+    """leapfrog.cpp: reads ./ic.txt (Re phi, Im phi per site), writes one
+    energy per TimeStep to ./valsPerLoop.txt and ./finalField.txt."""
+    dest = os.path.join(REPO, "oracle", "_ref", name)
+    if not os.path.exists(os.path.join(dest, "prog")):
+        build_ref.build(name, "/root/reference")
+    st = orc.init_random_phase(p, SEED, 0)      # phi = (cos, sin) of the seeded phases
+    V = p.nx * p.ny * p.nz
+    np.savetxt(os.path.join(dest, "ic.txt"), st[:2 * V].reshape(2, V).T, fmt="%.17g")
+    env = dict(os.environ, OMP_NUM_THREADS="1")
+    subprocess.run(["./prog"], cwd=dest, text=True, env=env, check=True, stdout=subprocess.DEVNULL)
+    energy = [float(line) for line in open(os.path.join(dest, "valsPerLoop.txt"))]
+    assert len(energy) == nt, (name, len(energy))
+    fin = np.loadtxt(os.path.join(dest, "finalField.txt"), dtype=np.float64).T.copy()
+    rng = np.random.default_rng(7)
+    idx = sorted(int(i) for i in rng.choice(V, size=min(64, V), replace=False))
+    fx = {
+        "name": name, "program": build_ref.CONFIGS[name][0], "trig": "none", "seed": SEED,
+        "nt": nt, "params": params_dict(p), "energy": energy,
+        "final_sha256": sha(fin), "final_sample_idx": idx, "final_sample": fin[:, idx].T.tolist(),
+        "frames": [],
+    }
+    with open(os.path.join(HERE, f"{name}.json"), "w") as f:
+        json.dump(fx, f)
+    for f in ("ic.txt", "valsPerLoop.txt", "finalField.txt"):
+        os.remove(os.path.join(dest, f))
+    print(name, "energy[0..2]", energy[:3], "final", fx["final_sha256"][:12])
+
+
 def run(name):
     builder, nt, save, det0, trig = SPECS[name]
     p = builder()
+    if p.model == 2:
+        return run_global(name, p, nt)
     dest = os.path.join(REPO, "oracle", "_ref", name)
     if not os.path.exists(os.path.join(dest, "prog")):
         build_ref.build(name, "/root/reference")
