@@ -52,7 +52,23 @@ def build(verbose=False):
     tmp = LIB + ".tmp"
     subprocess.run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs, "-lpthread"], check=True)
     os.replace(tmp, LIB)
+    build_cli(verbose)
     return LIB
+
+
+CLI = os.path.join(HERE, "axions_run")
+
+
+def build_cli(verbose=False):
+    """axions_run: the reference programs' main() over libaxions.so (plain C++)."""
+    cmd = [os.environ.get("CXX", "g++"), "-O2", "-std=c++17", "-Wall", "-I", os.path.join(REPO, "include"),
+           os.path.join(HERE, "cli", "axions_run.cpp"), "-o", CLI + ".tmp", "-L", HERE, "-laxions",
+           "-Wl,-rpath,$ORIGIN", "-Wl,-rpath-link,/opt/rocm/lib"]
+    if verbose:
+        print(" ".join(cmd))
+    subprocess.run(cmd, check=True)
+    os.replace(CLI + ".tmp", CLI)
+    return CLI
 
 
 if __name__ == "__main__":

@@ -1,0 +1,136 @@
+"""axions_run (the reference programs' main() over the library) against the
+reference's own outputs: the fixture runs of evolution.cpp, mooreEvol.cpp
+and leapfrog.cpp (energy/Gauss series to 1e-12, crossing frames and final
+fields bit for bit after a %.17g round trip), and mpi_evolution's random-IC
+path against the oracle."""
+import hashlib
+import json
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+from axions_amd.abi import AxParams, ah_params
+from oracle import oracle_py as orc
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLDEN = os.path.join(HERE, "golden")
+CLI = os.path.join(os.path.dirname(HERE), "axions_amd", "axions_run")
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def sha(a):
+    return hashlib.sha256(np.ascontiguousarray(a, dtype=np.float64).tobytes()).hexdigest()
+
+
+def rel(a, b):
+    return abs(a - b) / max(abs(b), 1e-300)
+
+
+def load(name):
+    fx = json.load(open(os.path.join(GOLDEN, name + ".json")))
+    p = AxParams()
+    for k, v in fx["params"].items():
+        setattr(p, k, v)
+    return fx, p
+
+
+def run(args, cwd):
+    say("[cli]", " ".join(args))
+    r = subprocess.run([CLI, "--dir", cwd, "--digits", "17", *args], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+
+
+def write_ic(path, st, p, ncomp):
+    V = p.nx * p.ny * p.nz
+    cur = st.reshape(ncomp, 2, V)[:, 0]            # level 0 = the stationary IC
+    np.savetxt(path, cur.T, fmt="%.17g")
+
+
+def consts_ah(p, fx, extra):
+    return [f"nx={p.nx}", f"ny={p.ny}", f"nz={p.nz}", f"nt={fx['nt']}", f"dx={p.dx!r}", f"dy={p.dy!r}",
+            f"dz={p.dz!r}", f"dt={p.dt!r}", f"lambda={p.lambda_!r}", f"eta={p.eta!r}", f"g={p.g!r}",
+            f"damped_nt={p.damped_nt}", f"dampFac={p.dampFac!r}", f"alpha={p.alpha!r}",
+            f"scaling={p.scaling!r}", *extra]
+
+
+def test_evolution_program(tmp_path):
+    fx, p = load("evo_ah_g1_shared")
+    d = str(tmp_path)
+    os.makedirs(os.path.join(d, "Data"), exist_ok=True)
+    st = orc.init_random_phase(p, fx["seed"])
+    write_ic(os.path.join(d, "Data", "ic.txt"), st, p, 5)
+    run(["--program", "evolution", "--tag", "golden",
+         *consts_ah(p, fx, ["saveFreq=5", "stationary_ic=true", "finalOut=true"])], d)
+    vals = np.loadtxt(os.path.join(d, "Data", "valsPerLoop_golden.txt"))
+    assert vals.shape == (fx["nt"], 3)
+    for n in range(fx["nt"]):
+        assert rel(vals[n, 0], fx["energy"][n]) < 1e-12, n
+        assert rel(vals[n, 1], fx["gauss"][n]) < 1e-12, n
+    for k, fr in enumerate(fx["frames"]):
+        path = os.path.join(d, "GifData", f"gifStringPosData_{k}.txt")
+        pts = np.loadtxt(path, ndmin=2) if os.path.getsize(path) else np.zeros((0, 3))
+        assert pts.shape[0] == fr["count"] and sha(pts) == fr["sha256"], k
+    fin = np.loadtxt(os.path.join(d, "Data", "finalField.txt")).T
+    assert sha(fin) == fx["final_sha256"]
+
+
+def test_moore_program(tmp_path):
+    fx, p = load("moore_shared")
+    d = str(tmp_path)
+    os.makedirs(os.path.join(d, "Data"), exist_ok=True)
+    st = orc.init_random_phase(p, fx["seed"])
+    write_ic(os.path.join(d, "Data", "moore_ic.txt"), st, p, 7)
+    run(["--program", "moore", f"nx={p.nx}", f"ny={p.ny}", f"nz={p.nz}", f"nt={fx['nt']}", "makeGif=false"], d)
+    vals = np.loadtxt(os.path.join(d, "Data", "valsPerLoop.txt"))
+    for n in range(fx["nt"]):
+        assert rel(vals[n, 0], fx["energy"][n]) < 1e-12, n
+        if fx["gauss"][n] != 0:
+            assert rel(vals[n, 1], fx["gauss"][n]) < 1e-12, n
+    fin = np.loadtxt(os.path.join(d, "Data", "finalField.txt")).T
+    assert sha(fin) == fx["final_sha256"]
+
+
+def test_leapfrog_program(tmp_path):
+    fx, p = load("global_neumann")
+    d = str(tmp_path)
+    st = orc.init_random_phase(p, fx["seed"])
+    V = p.nx * p.ny * p.nz
+    np.savetxt(os.path.join(d, "ic.txt"), st[:2 * V].reshape(2, V).T, fmt="%.17g")
+    run(["--program", "leapfrog", f"nx={p.nx}", f"ny={p.ny}", f"nz={p.nz}", f"nt={fx['nt']}"], d)
+    e = np.loadtxt(os.path.join(d, "valsPerLoop.txt"))
+    for n in range(fx["nt"]):
+        assert rel(e[n], fx["energy"][n]) < 1e-12, n
+    fin = np.loadtxt(os.path.join(d, "finalField.txt")).T
+    assert sha(fin) == fx["final_sha256"]
+
+
+def test_mpi_evolution_random_ic(tmp_path):
+    """Seeded random IC, MPI-form energy every step, crossing frames after the
+    damping window, file names with outTag/nx/seed."""
+    d = str(tmp_path)
+    n, nt, seed = 12, 16, 7
+    p = ah_params(n, dx=0.5, dt=0.1, lam=2.0, g=1.0, alpha=3.0, damped_nt=5, dampFac=0.5, energy_form=1)
+    run(["--program", "mpi_evolution", f"nx={n}", f"ny={n}", f"nz={n}", f"nt={nt}", "dx=0.5", "dy=0.5", "dz=0.5",
+         "dt=0.1", "lambda=2", "g=1", "damped_nt=5", f"seed={seed}", "saveFreq=3", "calcEnergy=true",
+         "outTag=t"], d)
+    tag = f"t_nx{n}_seed{seed}"
+    e = np.loadtxt(os.path.join(d, "Data", f"valsPerLoop_{tag}.txt"))
+    st = orc.init_random_phase(p, seed)
+    frame = 0
+    for t in range(nt):
+        o = orc.diag(p, st, t, 0)
+        assert rel(e[t], o["energy"]) < 1e-12, t
+        if t % 3 == 0 and t >= 5:
+            pts = np.loadtxt(os.path.join(d, "GifData", f"gifStringPosData_{tag}_{frame}.txt"), ndmin=2)
+            po, co = orc.crossings(p, st, t)
+            assert pts.shape[0] == co and np.array_equal(pts, po)
+            frame += 1
+        orc.step(p, st, t, 1, 0, 16)
