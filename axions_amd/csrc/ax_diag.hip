@@ -167,24 +167,26 @@ __device__ __forceinline__ void site_energy_moore(const DiagConsts &k, const Lev
   absG = k.edge ? fabs(G) / k.inner : fabs(G);
 }
 
-/* Sites handled by one 256-thread diagnostic block (each thread takes
- * kSitesPerThread sites 256 apart, so loads stay coalesced). */
+/* Sites per thread of the winding pass (each thread takes sites 256 apart,
+ * so loads stay coalesced); the energy pass takes one site per thread. */
 constexpr int kSitesPerThread = 2;
 constexpr int kSitesPerBlock = 256 * kSitesPerThread;
+constexpr int kDiagSitesPerThread = 1;
+constexpr int kDiagSitesPerBlock = 256 * kDiagSitesPerThread;
 
 /* out: per block [energy, sum|G|, nonfinite count] */
-template <typename T>
+template <typename T, int MODEL>
 __global__ void __launch_bounds__(256) diag_kernel(DiagConsts k, Levels<T> L, int V, double *part) {
   __shared__ double sh[4];
   double rho = 0, aG = 0, bad = 0;
-  for (int q = 0; q < kSitesPerThread; ++q) {
-    const int s = blockIdx.x * kSitesPerBlock + q * 256 + threadIdx.x;
+  for (int q = 0; q < kDiagSitesPerThread; ++q) {
+    const int s = blockIdx.x * kDiagSitesPerBlock + q * 256 + threadIdx.x;
     if (s >= V) break;
     const int kk = s % k.nz, r = s / k.nz, j = r % k.ny, i = r / k.ny;
     const bool inner = !k.edge || (i > 0 && i < k.nx - 1 && j > 0 && j < k.ny - 1 && kk > 0 && kk < k.nz - 1);
     if (inner) {
       double r1 = 0, g1 = 0;
-      if (k.model == AX_MODEL_AH)
+      if (MODEL == AX_MODEL_AH)
         site_energy_ah<T>(k, L, i, j, kk, r1, g1);
       else
         site_energy_moore<T>(k, L, i, j, kk, r1, g1);
@@ -401,8 +403,11 @@ __global__ void __launch_bounds__(256) cross_write_kernel(int nx, int ny, int nz
 template <typename T>
 hipError_t launch_diag(const DiagConsts &k, const Levels<T> &L, double *part, double *out3, hipStream_t st) {
   const int V = k.nx * k.ny * k.nz;
-  const int nb = (V + kSitesPerBlock - 1) / kSitesPerBlock;
-  hipLaunchKernelGGL((diag_kernel<T>), dim3(nb), dim3(256), 0, st, k, L, V, part);
+  const int nb = (V + kDiagSitesPerBlock - 1) / kDiagSitesPerBlock;
+  if (k.model == AX_MODEL_AH)
+    hipLaunchKernelGGL((diag_kernel<T, AX_MODEL_AH>), dim3(nb), dim3(256), 0, st, k, L, V, part);
+  else
+    hipLaunchKernelGGL((diag_kernel<T, AX_MODEL_MOORE>), dim3(nb), dim3(256), 0, st, k, L, V, part);
   hipLaunchKernelGGL(sum3_kernel, dim3(1), dim3(1024), 0, st, part, nb, out3);
   return hipGetLastError();
 }

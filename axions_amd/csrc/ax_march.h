@@ -122,12 +122,191 @@ struct PlaneOps {
   R TXzm, TYzm, TZzm, TZxpzm, TZypzm;
 };
 
+/* Values carried from plane x-1 into plane x of the march. */
+template <typename R>
+struct MarchCarry {
+  R PRm, PIm;          /* phi(s-x) */
+  R cxm, sxm;          /* cos/sin theta_x(s-x) */
+  R Qyx_m, Qzx_m;      /* plaquette sines at s-x needed by F_yx, F_zx */
+};
+
+/* The new previous-level values (level n+1) of one site. */
+template <typename R>
+struct SiteOut {
+  R X, Y, Z, PR, PI;
+};
+
+/* Operands of one site update, by name.  The site is s = (x, y, z); "n"
+ * is plane x+1, "o" the previous level at s, yp/ym/zp/zm the neighbour
+ * rows/columns (xp_ym = (x+1, y-1), ym_zp = (x, y-1, z+1), ...). */
+enum Op {
+  kPRc, kPIc, kTXc, kTYc, kTZc,
+  kPRn, kPIn, kTXn, kTYn, kTZn,
+  koPR, koPI, koX, koY, koZ,
+  kPRyp, kPIyp, kPRym, kPIym, kPRzp, kPIzp, kPRzm, kPIzm,
+  kTXyp, kTZyp, kTXzp, kTYzp, kTXym, kTYym, kTZym, kTYxpym, kTYymzp,
+  kTXzm, kTYzm, kTZzm, kTZxpzm, kTZypzm
+};
+
+/* Operands held in registers (the register march). */
+template <typename R>
+struct RegOps {
+  const PlaneOps<R> &D;
+  R PRc, PIc, TXc, TYc, TZc;
+  __device__ __forceinline__ R operator()(int id) const {
+    switch (id) {
+      case kPRc: return PRc; case kPIc: return PIc; case kTXc: return TXc; case kTYc: return TYc; case kTZc: return TZc;
+      case kPRn: return D.PRn; case kPIn: return D.PIn; case kTXn: return D.TXn; case kTYn: return D.TYn;
+      case kTZn: return D.TZn; case koPR: return D.oPR; case koPI: return D.oPI; case koX: return D.oX;
+      case koY: return D.oY; case koZ: return D.oZ; case kPRyp: return D.PRyp; case kPIyp: return D.PIyp;
+      case kPRym: return D.PRym; case kPIym: return D.PIym; case kPRzp: return D.PRzp; case kPIzp: return D.PIzp;
+      case kPRzm: return D.PRzm; case kPIzm: return D.PIzm; case kTXyp: return D.TXyp; case kTZyp: return D.TZyp;
+      case kTXzp: return D.TXzp; case kTYzp: return D.TYzp; case kTXym: return D.TXym; case kTYym: return D.TYym;
+      case kTZym: return D.TZym; case kTYxpym: return D.TYxpym; case kTYymzp: return D.TYymzp;
+      case kTXzm: return D.TXzm; case kTYzm: return D.TYzm; case kTZzm: return D.TZzm;
+      case kTZxpzm: return D.TZxpzm; default: return D.TZypzm;
+    }
+  }
+};
+
+/* Keeps the compiler from moving instructions across phase boundaries, so
+ * operands fetched on demand (LDS) are not all hoisted to the top. */
+template <bool PHASED>
+__device__ __forceinline__ void phase_fence() {
+  if (PHASED) __builtin_amdgcn_sched_barrier(0);
+}
+
+/* One site of the fused in-place update (mpi_evolution.cpp:418-501) from its
+ * operands g(Op) and the carry from s-x, which it advances to s.  Every lane
+ * of the wave must call it (it votes with __all/__any). */
+template <typename R, bool POW2, bool PHASED, typename G>
+__device__ __forceinline__ void ah_site_update(const MarchConsts<R> &k, bool plain, const G &g, MarchCarry<R> &m,
+                                               SiteOut<R> &o) {
+  typedef TrigR<R> T;
+  const R dx2 = k.dx2, dy2 = k.dy2, dz2 = k.dz2, dt = k.dt, dt2 = k.dt2;
+  const R hl = k.hl, eta2 = k.eta2, m2gg = k.m2gg, fric = k.fric, damp = k.damp;
+  /* links at s, s-y and s-z */
+  const R TXc = g(kTXc), TYc = g(kTYc), TZc = g(kTZc), TYym = g(kTYym), TZzm = g(kTZzm);
+  R cx, sx, cy, sy, cz, sz, cym, sym_, czm, szm;
+  T::sc(TXc, &sx, &cx);
+  T::sc(TYc, &sy, &cy);
+  T::sc(TZc, &sz, &cz);
+  T::sc(TYym, &sym_, &cym);
+  T::sc(TZzm, &szm, &czm);
+  phase_fence<PHASED>();
+  /* plaquette angles, reference operand order */
+  const R TYn = g(kTYn), TZn = g(kTZn), TXyp = g(kTXyp), TXzp = g(kTXzp), TZyp = g(kTZyp), TYzp = g(kTYzp);
+  const R a_xy = TXc + TYn - TXyp - TYc;
+  const R a_xz = TXc + TZn - TXzp - TZc;
+  const R a_yx = TYc + TXyp - TYn - TXc;
+  const R a_yz = TYc + TZyp - TYzp - TZc;
+  const R a_zx = TZc + TXzp - TZn - TXc;
+  const R a_zy = TZc + TYzp - TZyp - TYc;
+  const R b_xy = g(kTXym) + g(kTYxpym) - TXc - TYym;
+  const R b_zy = g(kTZym) + g(kTYymzp) - TZc - TYym;
+  const R b_xz = g(kTXzm) + g(kTZxpzm) - TXc - TZzm;
+  const R b_yz = g(kTYzm) + g(kTZypzm) - TYc - TZzm;
+  R Qxy, Qxz, Qyx, Qyz, Qzx, Qzy, Qxy_m, Qzy_m, Qxz_m, Qyz_m;
+  const bool small = T::small(a_xy) && T::small(a_xz) && T::small(a_yx) && T::small(a_yz) &&
+                     T::small(a_zx) && T::small(a_zy) && T::small(b_xy) && T::small(b_zy) &&
+                     T::small(b_xz) && T::small(b_yz);
+  if (__all(small)) {
+    Qxy = T::sn_small(a_xy); Qxz = T::sn_small(a_xz); Qyx = T::sn_small(a_yx);
+    Qyz = T::sn_small(a_yz); Qzx = T::sn_small(a_zx); Qzy = T::sn_small(a_zy);
+    Qxy_m = T::sn_small(b_xy); Qzy_m = T::sn_small(b_zy);
+    Qxz_m = T::sn_small(b_xz); Qyz_m = T::sn_small(b_yz);
+  } else {
+    Qxy = T::sn(a_xy); Qxz = T::sn(a_xz); Qyx = T::sn(a_yx);
+    Qyz = T::sn(a_yz); Qzx = T::sn(a_zx); Qzy = T::sn(a_zy);
+    Qxy_m = T::sn(b_xy); Qzy_m = T::sn(b_zy);
+    Qxz_m = T::sn(b_xz); Qyz_m = T::sn(b_yz);
+  }
+  phase_fence<PHASED>();
+
+  /* gauge field first (frees the plaquette sines early) */
+  const R fxy = mdiv<R, POW2>(Qxy - Qxy_m, dy2, k.inv_dy2);
+  const R fxz = mdiv<R, POW2>(Qxz - Qxz_m, dz2, k.inv_dz2);
+  const R fyx = mdiv<R, POW2>(Qyx - m.Qyx_m, dx2, k.inv_dx2);
+  const R fyz = mdiv<R, POW2>(Qyz - Qyz_m, dz2, k.inv_dz2);
+  const R fzx = mdiv<R, POW2>(Qzx - m.Qzx_m, dx2, k.inv_dx2);
+  const R fzy = mdiv<R, POW2>(Qzy - Qzy_m, dy2, k.inv_dy2);
+  const R PRc = g(kPRc), PIc = g(kPIc), PRn = g(kPRn), PIn = g(kPIn);
+  const R PRyp = g(kPRyp), PIyp = g(kPIyp), PRzp = g(kPRzp), PIzp = g(kPIzp);
+  const R oX = g(koX), oY = g(koY), oZ = g(koZ), oPR = g(koPR), oPI = g(koPI);
+  const R jx = cx * (PIc * PRn - PRc * PIn) + sx * (PRc * PRn + PIc * PIn);
+  const R jy = cy * (PIc * PRyp - PRc * PIyp) + sy * (PRc * PRyp + PIc * PIyp);
+  const R jz = cz * (PIc * PRzp - PRc * PIzp) + sz * (PRc * PRzp + PIc * PIzp);
+  R gX, gY, gZ, f0, f1;
+  bool literal = !plain;
+  if (plain) {
+    bool ovf = false;
+    gX = zero_rate(TXc - oX, k.lim, &ovf);
+    gY = zero_rate(TYc - oY, k.lim, &ovf);
+    gZ = zero_rate(TZc - oZ, k.lim, &ovf);
+    f0 = zero_rate(PRc - oPR, k.lim, &ovf);
+    f1 = zero_rate(PIc - oPI, k.lim, &ovf);
+    /* some |a-b|/dt overflows: take the literal products (never in a sane state) */
+    literal = __any(ovf);
+  }
+  if (literal) {
+    f0 = fric * ((PRc - oPR) / dt);
+    f1 = fric * ((PIc - oPI) / dt);
+    if (k.theta_damp_form == 0) {
+      gX = damp * ((TXc - oX) / dt); gY = damp * ((TYc - oY) / dt); gZ = damp * ((TZc - oZ) / dt);
+    } else {
+      gX = damp * (TXc - oX) / dt; gY = damp * (TYc - oY) / dt; gZ = damp * (TZc - oZ) / dt;
+    }
+  }
+  const R ax_ = m2gg * jx - fxy - fxz - gX;
+  const R ay_ = m2gg * jy - fyx - fyz - gY;
+  const R az_ = m2gg * jz - fzx - fzy - gZ;
+  o.X = 2 * TXc - oX + dt2 * ax_;
+  o.Y = 2 * TYc - oY + dt2 * ay_;
+  o.Z = 2 * TZc - oZ + dt2 * az_;
+  phase_fence<PHASED>();
+
+  /* scalar field */
+  const R PRym = g(kPRym), PIym = g(kPIym), PRzm = g(kPRzm), PIzm = g(kPIzm);
+  const R mag2 = PRc * PRc + PIc * PIc;
+  const R pot = hl * (mag2 - eta2);
+  const R lxx0 = mdiv<R, POW2>(cx * PRn + sx * PIn - 2 * PRc + m.cxm * m.PRm - m.sxm * m.PIm, dx2, k.inv_dx2);
+  const R lyy0 = mdiv<R, POW2>(cy * PRyp + sy * PIyp - 2 * PRc + cym * PRym - sym_ * PIym, dy2, k.inv_dy2);
+  const R lzz0 = mdiv<R, POW2>(cz * PRzp + sz * PIzp - 2 * PRc + czm * PRzm - szm * PIzm, dz2, k.inv_dz2);
+  const R lxx1 = mdiv<R, POW2>(cx * PIn + (-sx) * PRn - 2 * PIc + m.cxm * m.PIm - (-m.sxm) * m.PRm, dx2, k.inv_dx2);
+  const R lyy1 = mdiv<R, POW2>(cy * PIyp + (-sy) * PRyp - 2 * PIc + cym * PIym - (-sym_) * PRym, dy2, k.inv_dy2);
+  const R lzz1 = mdiv<R, POW2>(cz * PIzp + (-sz) * PRzp - 2 * PIc + czm * PIzm - (-szm) * PRzm, dz2, k.inv_dz2);
+  const R acc0 = lxx0 + lyy0 + lzz0 - pot * PRc - f0;
+  const R acc1 = lxx1 + lyy1 + lzz1 - pot * PIc - f1;
+  o.PR = 2 * PRc - oPR + dt2 * acc0;
+  o.PI = 2 * PIc - oPI + dt2 * acc1;
+
+  /* carry to x+1 */
+  m.PRm = PRc; m.PIm = PIc;
+  m.cxm = cx; m.sxm = sx; m.Qyx_m = Qyx; m.Qzx_m = Qzx;
+}
+
+/* The prologue of a march starting at plane x0: phi, cos/sin theta_x and
+ * the plaquette sines F_yx, F_zx need at s-x, from plane x0-1 and x0.  The
+ * accessor G(array, plane, row, zoff) returns the current-level value at
+ * (plane, y + row, z + zoff) for this lane. */
+template <typename R, typename Get>
+__device__ __forceinline__ void march_prologue(Get G, int xm0, int x0, MarchCarry<R> &m) {
+  typedef TrigR<R> T;
+  m.PRm = G(0, xm0, 0, 0);
+  m.PIm = G(1, xm0, 0, 0);
+  const R X = G(2, xm0, 0, 0), Y = G(3, xm0, 0, 0), Z = G(4, xm0, 0, 0);
+  T::sc(X, &m.sxm, &m.cxm);
+  const R ayx = Y + G(2, xm0, 1, 0) - G(3, x0, 0, 0) - X;
+  const R azx = Z + G(2, xm0, 0, 1) - G(4, x0, 0, 0) - X;
+  m.Qyx_m = T::sn(ayx);
+  m.Qzx_m = T::sn(azx);
+}
+
 /* PLAIN: 0 friction products always evaluated, 1 fric = damp = +0 known at
  * compile time, 2 decided at run time from k.plain (wave-uniform). */
-template <typename R, bool POW2, int PLAIN>
+template <typename R, bool POW2, int PLAIN, bool PHASED = false>
 __device__ __forceinline__ void ah_march(const MarchConsts<R> &k, const Levels<R> &L, int NX, int NY, int NZ,
                                          int y, int zc, int x0, int x1) {
-  typedef TrigR<R> T;
   const int lane = (int)(threadIdx.x & 63);
   const int zr = zc * 64 + lane;
   const bool act = zr < NZ;
@@ -141,8 +320,6 @@ __device__ __forceinline__ void ah_march(const MarchConsts<R> &k, const Levels<R
   const uint32_t R0 = (uint32_t)y * NZ * E, RM = (uint32_t)ym * NZ * E, RP = (uint32_t)yp * NZ * E;
   const R *PR = L.cur[0], *PI = L.cur[1], *TX = L.cur[2], *TY = L.cur[3], *TZ = L.cur[4];
   R *OPR = L.prv[0], *OPI = L.prv[1], *OTX = L.prv[2], *OTY = L.prv[3], *OTZ = L.prv[4];
-  const R dx2 = k.dx2, dy2 = k.dy2, dz2 = k.dz2, dt = k.dt, dt2 = k.dt2;
-  const R hl = k.hl, eta2 = k.eta2, m2gg = k.m2gg, fric = k.fric, damp = k.damp;
   const bool plain = PLAIN == 1 || (PLAIN == 2 && k.plain);
 
   auto load_plane = [&](int x, PlaneOps<R> &d) {
@@ -166,18 +343,11 @@ __device__ __forceinline__ void ah_march(const MarchConsts<R> &k, const Levels<R
 
   /* ---- prologue: plane x0-1 quantities needed by plane x0 ---- */
   const int xm0 = x0 == 0 ? NX - 1 : x0 - 1;
-  R PRm, PIm, cxm, sxm, Qyx_m, Qzx_m;
-  {
-    const uint32_t b = (uint32_t)xm0 * PLB, bn = (uint32_t)x0 * PLB;
-    PRm = ldo(PR, b + R0, oz);
-    PIm = ldo(PI, b + R0, oz);
-    const R X = ldo(TX, b + R0, oz), Y = ldo(TY, b + R0, oz), Z = ldo(TZ, b + R0, oz);
-    T::sc(X, &sxm, &cxm);
-    const R ayx = Y + ldo(TX, b + RP, oz) - ldo(TY, bn + R0, oz) - X;
-    const R azx = Z + ldo(TX, b + R0, ozp) - ldo(TZ, bn + R0, oz) - X;
-    Qyx_m = T::sn(ayx);
-    Qzx_m = T::sn(azx);
-  }
+  const R *F[5] = {PR, PI, TX, TY, TZ};
+  MarchCarry<R> m;
+  march_prologue<R>([&](int a, int x, int dy, int dz) {
+    return ldo(F[a], (uint32_t)x * PLB + (dy ? RP : R0), dz ? ozp : oz);
+  }, xm0, x0, m);
   const uint32_t b0 = (uint32_t)x0 * PLB;
   R PRc = ldo(PR, b0 + R0, oz), PIc = ldo(PI, b0 + R0, oz);
   R TXc = ldo(TX, b0 + R0, oz), TYc = ldo(TY, b0 + R0, oz), TZc = ldo(TZ, b0 + R0, oz);
@@ -186,100 +356,15 @@ __device__ __forceinline__ void ah_march(const MarchConsts<R> &k, const Levels<R
     PlaneOps<R> D;
     load_plane(x, D);
     const uint32_t so = (uint32_t)x * PLB + R0;
-
-    /* links at s, s-y and s-z */
-    R cx, sx, cy, sy, cz, sz, cym, sym_, czm, szm;
-    T::sc(TXc, &sx, &cx);
-    T::sc(TYc, &sy, &cy);
-    T::sc(TZc, &sz, &cz);
-    T::sc(D.TYym, &sym_, &cym);
-    T::sc(D.TZzm, &szm, &czm);
-    /* plaquette angles, reference operand order */
-    const R a_xy = TXc + D.TYn - D.TXyp - TYc;
-    const R a_xz = TXc + D.TZn - D.TXzp - TZc;
-    const R a_yx = TYc + D.TXyp - D.TYn - TXc;
-    const R a_yz = TYc + D.TZyp - D.TYzp - TZc;
-    const R a_zx = TZc + D.TXzp - D.TZn - TXc;
-    const R a_zy = TZc + D.TYzp - D.TZyp - TYc;
-    const R b_xy = D.TXym + D.TYxpym - TXc - D.TYym;
-    const R b_zy = D.TZym + D.TYymzp - TZc - D.TYym;
-    const R b_xz = D.TXzm + D.TZxpzm - TXc - D.TZzm;
-    const R b_yz = D.TYzm + D.TZypzm - TYc - D.TZzm;
-    R Qxy, Qxz, Qyx, Qyz, Qzx, Qzy, Qxy_m, Qzy_m, Qxz_m, Qyz_m;
-    const bool small = T::small(a_xy) && T::small(a_xz) && T::small(a_yx) && T::small(a_yz) &&
-                       T::small(a_zx) && T::small(a_zy) && T::small(b_xy) && T::small(b_zy) &&
-                       T::small(b_xz) && T::small(b_yz);
-    if (__all(small)) {
-      Qxy = T::sn_small(a_xy); Qxz = T::sn_small(a_xz); Qyx = T::sn_small(a_yx);
-      Qyz = T::sn_small(a_yz); Qzx = T::sn_small(a_zx); Qzy = T::sn_small(a_zy);
-      Qxy_m = T::sn_small(b_xy); Qzy_m = T::sn_small(b_zy);
-      Qxz_m = T::sn_small(b_xz); Qyz_m = T::sn_small(b_yz);
-    } else {
-      Qxy = T::sn(a_xy); Qxz = T::sn(a_xz); Qyx = T::sn(a_yx);
-      Qyz = T::sn(a_yz); Qzx = T::sn(a_zx); Qzy = T::sn(a_zy);
-      Qxy_m = T::sn(b_xy); Qzy_m = T::sn(b_zy);
-      Qxz_m = T::sn(b_xz); Qyz_m = T::sn(b_yz);
-    }
-
-    /* gauge field first (frees the plaquette sines early) */
-    const R fxy = mdiv<R, POW2>(Qxy - Qxy_m, dy2, k.inv_dy2);
-    const R fxz = mdiv<R, POW2>(Qxz - Qxz_m, dz2, k.inv_dz2);
-    const R fyx = mdiv<R, POW2>(Qyx - Qyx_m, dx2, k.inv_dx2);
-    const R fyz = mdiv<R, POW2>(Qyz - Qyz_m, dz2, k.inv_dz2);
-    const R fzx = mdiv<R, POW2>(Qzx - Qzx_m, dx2, k.inv_dx2);
-    const R fzy = mdiv<R, POW2>(Qzy - Qzy_m, dy2, k.inv_dy2);
-    const R jx = cx * (PIc * D.PRn - PRc * D.PIn) + sx * (PRc * D.PRn + PIc * D.PIn);
-    const R jy = cy * (PIc * D.PRyp - PRc * D.PIyp) + sy * (PRc * D.PRyp + PIc * D.PIyp);
-    const R jz = cz * (PIc * D.PRzp - PRc * D.PIzp) + sz * (PRc * D.PRzp + PIc * D.PIzp);
-    R gX, gY, gZ, f0, f1;
-    bool literal = !plain;
-    if (plain) {
-      bool ovf = false;
-      gX = zero_rate(TXc - D.oX, k.lim, &ovf);
-      gY = zero_rate(TYc - D.oY, k.lim, &ovf);
-      gZ = zero_rate(TZc - D.oZ, k.lim, &ovf);
-      f0 = zero_rate(PRc - D.oPR, k.lim, &ovf);
-      f1 = zero_rate(PIc - D.oPI, k.lim, &ovf);
-      /* some |a-b|/dt overflows: take the literal products (never in a sane state) */
-      literal = __any(ovf);
-    }
-    if (literal) {
-      f0 = fric * ((PRc - D.oPR) / dt);
-      f1 = fric * ((PIc - D.oPI) / dt);
-      if (k.theta_damp_form == 0) {
-        gX = damp * ((TXc - D.oX) / dt); gY = damp * ((TYc - D.oY) / dt); gZ = damp * ((TZc - D.oZ) / dt);
-      } else {
-        gX = damp * (TXc - D.oX) / dt; gY = damp * (TYc - D.oY) / dt; gZ = damp * (TZc - D.oZ) / dt;
-      }
-    }
-    const R ax_ = m2gg * jx - fxy - fxz - gX;
-    const R ay_ = m2gg * jy - fyx - fyz - gY;
-    const R az_ = m2gg * jz - fzx - fzy - gZ;
+    SiteOut<R> o;
+    ah_site_update<R, POW2, PHASED>(k, plain, RegOps<R>{D, PRc, PIc, TXc, TYc, TZc}, m, o);
     if (act) {
-      sto(OTX, so, oz, 2 * TXc - D.oX + dt2 * ax_);
-      sto(OTY, so, oz, 2 * TYc - D.oY + dt2 * ay_);
-      sto(OTZ, so, oz, 2 * TZc - D.oZ + dt2 * az_);
+      sto(OTX, so, oz, o.X);
+      sto(OTY, so, oz, o.Y);
+      sto(OTZ, so, oz, o.Z);
+      sto(OPR, so, oz, o.PR);
+      sto(OPI, so, oz, o.PI);
     }
-
-    /* scalar field */
-    const R mag2 = PRc * PRc + PIc * PIc;
-    const R pot = hl * (mag2 - eta2);
-    const R lxx0 = mdiv<R, POW2>(cx * D.PRn + sx * D.PIn - 2 * PRc + cxm * PRm - sxm * PIm, dx2, k.inv_dx2);
-    const R lyy0 = mdiv<R, POW2>(cy * D.PRyp + sy * D.PIyp - 2 * PRc + cym * D.PRym - sym_ * D.PIym, dy2, k.inv_dy2);
-    const R lzz0 = mdiv<R, POW2>(cz * D.PRzp + sz * D.PIzp - 2 * PRc + czm * D.PRzm - szm * D.PIzm, dz2, k.inv_dz2);
-    const R lxx1 = mdiv<R, POW2>(cx * D.PIn + (-sx) * D.PRn - 2 * PIc + cxm * PIm - (-sxm) * PRm, dx2, k.inv_dx2);
-    const R lyy1 = mdiv<R, POW2>(cy * D.PIyp + (-sy) * D.PRyp - 2 * PIc + cym * D.PIym - (-sym_) * D.PRym, dy2, k.inv_dy2);
-    const R lzz1 = mdiv<R, POW2>(cz * D.PIzp + (-sz) * D.PRzp - 2 * PIc + czm * D.PIzm - (-szm) * D.PRzm, dz2, k.inv_dz2);
-    const R acc0 = lxx0 + lyy0 + lzz0 - pot * PRc - f0;
-    const R acc1 = lxx1 + lyy1 + lzz1 - pot * PIc - f1;
-    if (act) {
-      sto(OPR, so, oz, 2 * PRc - D.oPR + dt2 * acc0);
-      sto(OPI, so, oz, 2 * PIc - D.oPI + dt2 * acc1);
-    }
-
-    /* carry to x+1 */
-    PRm = PRc; PIm = PIc;
-    cxm = cx; sxm = sx; Qyx_m = Qyx; Qzx_m = Qzx;
     PRc = D.PRn; PIc = D.PIn; TXc = D.TXn; TYc = D.TYn; TZc = D.TZn;
   }
 }
