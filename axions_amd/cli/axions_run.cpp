@@ -35,10 +35,11 @@
  * reference's ostream default; 17 round-trips doubles).
  *
  * Differences, all deliberate: periodic boundaries for the gauged programs
- * unless xyBC=fixed (mooreEvol.cpp's default); the string-length column of
- * evolution.cpp (uninitialised in the reference, evolution.cpp:1229-1330 is
- * commented out) holds the winding-count length (2/3)(dz Wxy + dy Wxz +
- * dx Wyz); all gauged runs use the library's shared sin/cos.
+ * unless xyBC=fixed (mooreEvol.cpp's default); the string-length column(s)
+ * of evolution.cpp (uninitialised there: its chaining, evolution.cpp:1229-1330,
+ * is commented out) hold the chained length of ic_w_evolution.cpp:961-1087
+ * (total, or inner/outer with splitLength) via ax_string_length; all gauged
+ * runs use the library's shared sin/cos.
  */
 #include <errno.h>
 #include <stdio.h>
@@ -88,7 +89,7 @@ std::map<std::string, std::string> defaults(const std::string &prog) {
             {"dz", "0.5"}, {"dt", "0.05"}, {"lambda", "1"}, {"eta", "1"}, {"g", "0"}, {"damped_nt", "200"},
             {"dampFac", "0.5"}, {"alpha", "3"}, {"scaling", "1"}, {"makeStringPosGif", "true"},
             {"saveFreq", "5"}, {"stationary_ic", "true"}, {"stringDetect", "true"}, {"detectBuffer", "true"},
-            {"finalOut", "false"}, {"xyBC", "periodic"}, {"precision", "64"}};
+            {"finalOut", "false"}, {"xyBC", "periodic"}, {"splitLength", "false"}, {"precision", "64"}};
   if (prog == "mpi_evolution")   /* mpi_evolution.cpp:17-60 */
     return {{"nx", "1001"}, {"ny", "1001"}, {"nz", "1001"}, {"nt", "2501"}, {"dx", "0.25"}, {"dy", "0.25"},
             {"dz", "0.25"}, {"dt", "0.05"}, {"lambda", "1"}, {"eta", "1"}, {"g", "0"}, {"damped_nt", "200"},
@@ -211,11 +212,17 @@ struct Run {
       for (int k = 0; k < ncomp; ++k) f.num(cols[(size_t)k * V + s], k == ncomp - 1);
   }
 
-  size_t frame(const std::string &path) {
+  std::vector<double> crossings() {
     size_t n = 0;
     check(ax_crossings(lat, nullptr, 0, &n), "ax_crossings");
     std::vector<double> xyz(3 * n + 3);
     check(ax_crossings(lat, xyz.data(), n, &n), "ax_crossings");
+    xyz.resize(3 * n);
+    return xyz;
+  }
+
+  size_t frame(const std::string &path, const std::vector<double> &xyz) {
+    const size_t n = xyz.size() / 3;
     Out f(path, o.digits);
     for (size_t i = 0; i < n; ++i) {
       f.num(xyz[3 * i], false);
@@ -239,7 +246,7 @@ struct Run {
     p.boundary = C.s("xyBC") == "periodic" ? AX_BC_PERIODIC : AX_BC_FIXED;
     const int nt = C.i("nt"), save = C.i("saveFreq");
     const bool detect = C.b("stringDetect"), buffer = C.b("detectBuffer"), gif = C.b("makeStringPosGif");
-    const bool stationary = C.b("stationary_ic"), fin = C.b("finalOut");
+    const bool stationary = C.b("stationary_ic"), fin = C.b("finalOut"), split = C.b("splitLength");
     if (o.tag.empty()) {
       printf("Enter a tag for output files: ");
       fflush(stdout);
@@ -254,12 +261,21 @@ struct Run {
     for (int n = 0; n < nt; ++n) {
       const bool active = detect && (!buffer || n >= p.damped_nt);
       ax_diag d;
-      check(ax_diagnostics(lat, active ? 1 : 0, &d), "ax_diagnostics");
-      if (active && gif && n % save == 0)
-        frame(D("GifData/gifStringPosData_" + std::to_string(frameNo++) + ".txt"));
+      check(ax_diagnostics(lat, 0, &d), "ax_diagnostics");
+      double len[3] = {0, 0, 0};
+      if (active) {
+        const std::vector<double> xyz = crossings();
+        if (gif && n % save == 0) frame(D("GifData/gifStringPosData_" + std::to_string(frameNo++) + ".txt"), xyz);
+        check(ax_string_length(xyz.data(), xyz.size() / 3, p.nx, p.ny, p.nz, p.dx, p.dy, p.dz,
+                               p.boundary == AX_BC_PERIODIC, p.boundary == AX_BC_PERIODIC, len), "ax_string_length");
+      }
       vals.num(d.energy, false);
       vals.num(d.gauss_mean, !detect);
-      if (detect) vals.num(active ? d.string_length : 0.0, true);
+      if (detect && !split) vals.num(len[0], true);
+      if (detect && split) {
+        vals.num(len[1], false);
+        vals.num(len[2], true);
+      }
       check(ax_step(lat, 1), "ax_step");
     }
     if (fin) write_rows(D("Data/finalField.txt"), newest(5), 5);
@@ -297,7 +313,7 @@ struct Run {
         vals.num(d.energy, true);
       }
       if (detect && n % save == 0 && gif && exist && (!buffer || n >= p.damped_nt)) {
-        if (frame(D("GifData/gifStringPosData_" + tag + "_" + std::to_string(frameNo++) + ".txt")) == 0)
+        if (frame(D("GifData/gifStringPosData_" + tag + "_" + std::to_string(frameNo++) + ".txt"), crossings()) == 0)
           exist = false;   /* mpi_evolution.cpp:854 */
       }
       check(ax_step(lat, 1), "ax_step");

@@ -52,6 +52,8 @@ def _load():
         "ax_random_phases": (ctypes.c_int, [ctypes.c_uint32, sz, vp]),
         "ax_kernel_timer": (ctypes.c_int, [vp, i32]),
         "ax_set_tile": (ctypes.c_int, [vp, i32]),
+        "ax_string_length": (ctypes.c_int, [vp, sz, i32, i32, i32, ctypes.c_double, ctypes.c_double,
+                                            ctypes.c_double, i32, i32, vp]),
         "ax_get_tile": (ctypes.c_int, [vp, ctypes.POINTER(i32), ctypes.POINTER(i32)]),
         "ax_kernel_time": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int64)]),
     }
@@ -67,6 +69,16 @@ lib = _load()
 def _check(rc):
     if rc != AX_OK:
         raise AxError(rc, lib.ax_last_error().decode() or lib.ax_strerror(rc).decode())
+
+
+def string_length(xyz, nx, ny, nz, dx, dy, dz, periodic_xy=True, periodic_z=True):
+    """(total, inner, outer) chained string length of crossing points
+    (ic_w_evolution.cpp:961-1087)."""
+    pts = np.ascontiguousarray(np.asarray(xyz, dtype=np.float64).reshape(-1, 3))
+    out = np.zeros(3)
+    _check(lib.ax_string_length(pts.ctypes.data if pts.size else None, pts.shape[0], nx, ny, nz, dx, dy, dz,
+                                1 if periodic_xy else 0, 1 if periodic_z else 0, out.ctypes.data))
+    return float(out[0]), float(out[1]), float(out[2])
 
 
 def random_phases(seed, n):

@@ -176,6 +176,16 @@ int ax_winding_field(ax_lattice *lat, int32_t field, int8_t *out, size_t n);
  * total count to *count. */
 int ax_crossings(ax_lattice *lat, double *xyz, size_t cap, size_t *count);
 
+/* String length from a list of crossing points (x,y,z triples, e.g. from
+ * ax_crossings) by the greedy nearest-neighbour chaining of
+ * ic_w_evolution.cpp:961-1087: out[0] = total length, out[1] / out[2] = the
+ * parts whose starting point lies inside / outside the central half-box
+ * (its splitLength).  Host only; O(n) with a cell grid instead of the
+ * reference's O(n^2) search, same result bits.  periodic_xy / periodic_z
+ * select the minimum-image distance per axis as the reference's xyBC / zBC. */
+int ax_string_length(const double *xyz, size_t n, int32_t nx, int32_t ny, int32_t nz, double dx, double dy,
+                     double dz, int32_t periodic_xy, int32_t periodic_z, double *out);
+
 /* ------------------------------------------------------------------------
  * Ensemble pool: many independent realizations (seed, N, steps) of one
  * periodic Abelian-Higgs parameter set, served with continuous batching.

@@ -19,6 +19,7 @@
  *   Moore step, energy, Gauss ................ mooreEvol.cpp:184-395, 647-697
  *   periodic neighbour indices ............... mpi_evolution.cpp:385-405
  *   global string, momentum-form leapfrog .... leapfrog.cpp:166-347
+ *   string length by chaining crossings ...... ic_w_evolution.cpp:961-1087
  * The per-plaquette winding number has no reference counterpart.
  *
  * Pinning (what checks this file, see tests/): compiled reference runs in
@@ -873,6 +874,64 @@ int orc_step_r(const ax_params *p, void *st, int ts0, int nsteps, int nthreads) 
     run_steps_r<float>(p, (float *)st, ts0, nsteps, nthreads);
   else
     run_steps_r<double>(p, (double *)st, ts0, nsteps, nthreads);
+  return 0;
+}
+
+/* String length by chaining crossing points, ic_w_evolution.cpp:961-1087,
+ * with its all-pairs search over a list it erases from (one thread: ties go
+ * to the later point).  out: total, inside the central half-box, outside. */
+int orc_string_length(const double *xyz, size_t n, int nx, int ny, int nz, double dx, double dy, double dz,
+                      int pxy, int pz, double *out) {
+  std::vector<double> X(n), Y(n), Z(n);
+  for (size_t i = 0; i < n; ++i) { X[i] = xyz[3 * i]; Y[i] = xyz[3 * i + 1]; Z[i] = xyz[3 * i + 2]; }
+  auto d2 = [&](double ax, double ay, double az, double bx, double by, double bz) {
+    double d = 0;
+    if (fabs(ax - bx) <= 0.5 * nx * dx) d = (ax - bx) * (ax - bx);
+    else if (pxy) d = (nx * dx - fabs(ax - bx)) * (nx * dx - fabs(ax - bx));
+    if (fabs(ay - by) <= 0.5 * ny * dy) d += (ay - by) * (ay - by);
+    else if (pxy) d += (ny * dy - fabs(ay - by)) * (ny * dy - fabs(ay - by));
+    if (fabs(az - bz) <= 0.5 * nz * dz) d += (az - bz) * (az - bz);
+    else if (pz) d += (nz * dz - fabs(az - bz)) * (nz * dz - fabs(az - bz));
+    return d;
+  };
+  auto inner = [&](size_t f) {
+    return X[f] >= -0.25 * nx * dx && X[f] <= 0.25 * nx * dx && Y[f] >= -0.25 * ny * dy && Y[f] <= 0.25 * ny * dy &&
+           Z[f] >= -0.25 * nz * dz && Z[f] <= 0.25 * nz * dz;
+  };
+  double L0 = 0, Lin = 0, Lout = 0;
+  size_t f = 0;
+  const size_t n0 = n;
+  double b[3] = {0, 0, 0};
+  if (n0) { b[0] = X[0]; b[1] = Y[0]; b[2] = Z[0]; }
+  for (size_t it = 0; it < n0; ++it) {
+    bool found = false;
+    const double c = dx * dx + dy * dy + dz * dz;
+    double best = c * c;
+    size_t bi = 0;
+    for (size_t j = 0; j < X.size(); ++j) {
+      if (j == f) continue;
+      const double d = d2(X[f], Y[f], Z[f], X[j], Y[j], Z[j]);
+      if (d <= best) { found = true; best = d; bi = j; }
+    }
+    if (found) {
+      const double seg = sqrt(best);
+      L0 += seg;
+      (inner(f) ? Lin : Lout) += seg;
+      X.erase(X.begin() + f); Y.erase(Y.begin() + f); Z.erase(Z.begin() + f);
+      f = bi < f ? bi : bi - 1;
+    } else {
+      if (d2(X[f], Y[f], Z[f], b[0], b[1], b[2]) <= best) {
+        const double seg = sqrt(best);
+        L0 += seg;
+        (inner(f) ? Lin : Lout) += seg;
+      }
+      X.erase(X.begin() + f); Y.erase(Y.begin() + f); Z.erase(Z.begin() + f);
+      f = 0;
+      if (X.empty()) break;
+      b[0] = X[0]; b[1] = Y[0]; b[2] = Z[0];
+    }
+  }
+  out[0] = L0; out[1] = Lin; out[2] = Lout;
   return 0;
 }
 

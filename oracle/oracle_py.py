@@ -32,6 +32,9 @@ def lib():
         L.orc_step.argtypes = [P, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]
         L.orc_step_r.argtypes = [P, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
         L.orc_global_step.argtypes = [P, vp, ctypes.c_int, vp]
+        L.orc_string_length.argtypes = [vp, ctypes.c_size_t, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                         ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                         ctypes.c_int, vp]
         L.orc_diag.argtypes = [P, vp, ctypes.c_int, ctypes.c_int, vp]
         L.orc_winding.argtypes = [P, vp, ctypes.c_int, ctypes.c_int, vp]
         L.orc_crossings.argtypes = [P, vp, ctypes.c_int, vp, ctypes.c_size_t,
@@ -80,6 +83,15 @@ def global_step(p, st, nsteps):
     e = np.empty(max(nsteps, 1))
     assert lib().orc_global_step(ctypes.byref(p), st.ctypes.data, nsteps, e.ctypes.data) == 0
     return e[:nsteps]
+
+
+def string_length(xyz, nx, ny, nz, dx, dy, dz, pxy=True, pz=True):
+    """(total, inner, outer) by the reference's all-pairs chaining."""
+    pts = np.ascontiguousarray(np.asarray(xyz, dtype=np.float64).reshape(-1, 3))
+    out = np.zeros(3)
+    lib().orc_string_length(pts.ctypes.data if pts.size else None, pts.shape[0], nx, ny, nz, dx, dy, dz,
+                            int(pxy), int(pz), out.ctypes.data)
+    return float(out[0]), float(out[1]), float(out[2])
 
 
 def diag(p, st, n, trig_mode=0):

@@ -63,6 +63,18 @@ CONFIGS = {
     # own spacings, and an anisotropic set with non-power-of-two spacings
     "global_neumann": ("leapfrog.cpp", dict(nx=12, ny=10, nz=14, nt=40), False),
     "global_aniso": ("leapfrog.cpp", dict(nx=9, ny=11, nz=13, nt=30, dx=0.3, dy=0.35, dz=0.28, dt=0.07), False),
+    # string length by chaining (ic_w_evolution.cpp:961-1087) on the crossings
+    # of ic_func.cpp's seeded "random" initial condition, first TimeStep only
+    "icw_len_split": ("ic_w_evolution.cpp", dict(nx=15, ny=14, nz=16, nt=1, dx=0.35, dy=0.35, dz=0.35,
+                                                 splitLength="true"), False),
+    "icw_len_total": ("ic_w_evolution.cpp", dict(nx=12, ny=13, nz=11, nt=1, dx=0.5, dy=0.45, dz=0.4,
+                                                 splitLength="false"), False),
+}
+
+# extra translation units compiled from their own substituted text
+EXTRA = {
+    "icw_len_split": {"ic_func.cpp": dict(ic_type='"random"', seed=3)},
+    "icw_len_total": {"ic_func.cpp": dict(ic_type='"random"', seed=5)},
 }
 
 
@@ -71,7 +83,7 @@ def substituted_source(path, overrides):
     for name, val in overrides.items():
         if isinstance(val, float) and val == int(val):
             val = repr(val)
-        pat = re.compile(r"^(const\s+(?:int|double|bool)\s+" + re.escape(name) + r"\s*=\s*)([^;]*)(;)",
+        pat = re.compile(r"^(\s*const\s+(?:int|double|bool|string)\s+" + re.escape(name) + r"\s*=\s*)([^;]*)(;)",
                          re.M)
         text, n = pat.subn(lambda m: m.group(1) + str(val) + m.group(3), text)
         if n != 1:
@@ -86,7 +98,18 @@ def build(name, ref, cxx="g++"):
     os.makedirs(os.path.join(dest, "Data"), exist_ok=True)
     os.makedirs(os.path.join(dest, "GifData"), exist_ok=True)
     text = substituted_source(path, over)
-    cmd = [cxx, "-x", "c++", "-", "-x", "none", os.path.join(ref, "array.cpp"),
+    objs = []
+    for extra_src, extra_over in EXTRA.get(name, {}).items():
+        obj = os.path.join(dest, extra_src.replace(".cpp", ".o"))
+        r = subprocess.run([cxx, "-x", "c++", "-", "-c", "-I", ref, "-I", os.path.join(REPO, "include"),
+                            "-include", os.path.join(HERE, "ref_shim.h"), "-O3", "-fopenmp", "-o", obj],
+                           input=substituted_source(os.path.join(ref, extra_src), extra_over), text=True,
+                           capture_output=True)
+        if r.returncode != 0:
+            sys.stderr.write(r.stderr)
+            raise SystemExit(f"build of {name}/{extra_src} failed")
+        objs.append(obj)
+    cmd = [cxx, "-x", "c++", "-", "-x", "none", os.path.join(ref, "array.cpp"), *objs,
            "-I", ref, "-I", os.path.join(REPO, "include"),
            "-include", os.path.join(HERE, "ref_shim.h"),
            f"-fmacro-prefix-map=<stdin>=./{src}",

@@ -98,7 +98,44 @@ def run_global(name, p, nt):
     print(name, "energy[0..2]", energy[:3], "final", fx["final_sha256"][:12])
 
 
+ICW = {   # name -> (nx, ny, nz, dx, dy, dz, ic seed, split)
+    "icw_len_split": (15, 14, 16, 0.35, 0.35, 0.35, 3, True),
+    "icw_len_total": (12, 13, 11, 0.5, 0.45, 0.4, 5, False),
+}
+
+
+def run_icw(name):
+    """ic_w_evolution.cpp with ic_func.cpp's seeded "random" IC (minstd_rand,
+    uniform in (-1, 1), Re then Im per site in x-major order, links zero):
+    the string length it prints for TimeStep 0."""
+    nx, ny, nz, dx, dy, dz, seed, split = ICW[name]
+    dest = os.path.join(REPO, "oracle", "_ref", name)
+    if not os.path.exists(os.path.join(dest, "prog")):
+        build_ref.build(name, "/root/reference")
+    for d in ("Data", "GifData"):
+        os.makedirs(os.path.join(dest, d), exist_ok=True)
+    env = dict(os.environ, OMP_NUM_THREADS="1")
+    subprocess.run(["./prog"], cwd=dest, text=True, env=env, check=True, stdout=subprocess.DEVNULL)
+    f = open(os.path.join(dest, "Data", "valsPerLoop_test.txt")).readline().split()
+    length = [float(v) for v in f[2:]]
+    assert len(length) == (2 if split else 1)
+    fx = {"name": name, "program": "ic_w_evolution.cpp + ic_func.cpp", "lattice": [nx, ny, nz],
+          "spacing": [dx, dy, dz], "ic": {"engine": "minstd_rand", "seed": seed, "uniform": [-1.0, 1.0]},
+          "split": split, "length": length}
+    with open(os.path.join(HERE, f"{name}.json"), "w") as fo:
+        json.dump(fx, fo)
+    for d in ("Data", "GifData"):
+        for fn in os.listdir(os.path.join(dest, d)):
+            os.remove(os.path.join(dest, d, fn))
+    for fn in ("test1.txt", "test2.txt"):
+        if os.path.exists(os.path.join(dest, fn)):
+            os.remove(os.path.join(dest, fn))
+    print(name, "length", length)
+
+
 def run(name):
+    if name in ICW:
+        return run_icw(name)
     builder, nt, save, det0, trig = SPECS[name]
     p = builder()
     if p.model == 2:
@@ -158,7 +195,7 @@ def run(name):
 
 
 def main():
-    for n in (sys.argv[1:] or list(SPECS)):
+    for n in (sys.argv[1:] or list(SPECS) + list(ICW)):
         run(n)
 
 
