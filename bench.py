@@ -22,6 +22,10 @@ Prints ONE JSON line:
   fp32          the same run in FP32 storage/arithmetic (FP64 reductions), its
                 throughput and roofline, and its deviation from the FP64 run in
                 energy, Gauss residual and xi(t)
+  moore         the Moore model (two scalars, charges 2 and 1, one U(1) field)
+                at 256^3 FP64, mooreEvol.cpp couplings, 800 steps with winding
+                counts every 20: site-updates/s and the step kernel's roofline
+                (168 B per site-update)
   ensemble      ensemble serving: Poisson arrivals of realizations with N drawn
                 from {64,96,128,192,256}, steps = N*dx/(2*dt), served by the
                 FP32 pool (continuous batching, persistent batched kernel,
@@ -102,21 +106,21 @@ def copy_calibration(nbytes, reps):
     print(json.dumps({"copy_calibration_bytes": nbytes, "reps": reps}))
 
 
-def run_lattice(ax, n, precision, steps, warmup, diag_every, seed, log):
+def run_lattice(ax, n, precision, steps, warmup, diag_every, seed, log, moore=False, moore_dt=0.3):
     """The headline loop on one lattice: warmup, then `steps` timed steps with
     diagnostics every diag_every steps; returns throughput, the step kernel's
     mean time and the diagnostic series."""
-    p = ax.ah_params(n, precision=precision)
-    log(f"lattice {n}^3 FP{precision}: IC seed {seed}")
+    p = ax.moore_params(n, dt=moore_dt, precision=precision) if moore else ax.ah_params(n, precision=precision)
+    log(f"lattice {n}^3 FP{precision}{' Moore' if moore else ''}: IC seed {seed}")
     series = []
     with ax.Lattice(p) as lat:
         lat.init_random_phase(seed)
 
         def diag():
             d = lat.diagnostics(winding=True)
-            if not d["finite"]:
+            if not d["finite"] and not moore:
                 raise RuntimeError(f"non-finite state at step {d['timestep']}")
-            series.append((d["timestep"], d["energy"], d["gauss_mean"], d["wound_plaquettes"], d["xi"]))
+            series.append((d["timestep"], d["energy"], d["gauss_mean"], d["wound_plaquettes"], d["xi"], d["finite"]))
 
         def run(nsteps):
             done = 0
@@ -141,14 +145,15 @@ def run_lattice(ax, n, precision, steps, warmup, diag_every, seed, log):
         diag()
     elapsed = t1 - t0
     sites = n ** 3
-    bps = 120 if precision == 64 else 60
+    bps = (168 if moore else 120) if precision == 64 else (84 if moore else 60)
     avg_kernel_s = kms / 1e3 / max(ksteps, 1)
     achieved = bps * sites / avg_kernel_s / 1e9
     return {
         "value": sites * steps / elapsed,
         "ms_per_step": elapsed / steps * 1e3,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(n, precision),
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": None if moore else measured_traffic(n, precision),
                      "kernel_ms": avg_kernel_s * 1e3, "bytes_per_site": bps},
         "series": series,
     }
@@ -302,6 +307,11 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--cpu-n", type=int, default=0, help="lattice for the CPU sample (default: --n)")
     ap.add_argument("--fp32", choices=("on", "off"), default="on", help="FP32 run of the same workload")
+    ap.add_argument("--moore", choices=("on", "off"), default="on", help="Moore-model 256^3 run")
+    ap.add_argument("--moore-n", type=int, default=256)
+    ap.add_argument("--moore-steps", type=int, default=800)
+    ap.add_argument("--moore-dt", type=float, default=0.2,
+                    help="Moore time step (the program's 0.3 is unstable from a random-phase IC)")
     ap.add_argument("--ensemble", choices=("on", "off"), default="on")
     ap.add_argument("--ensemble-count", type=int, default=256)
     ap.add_argument("--ensemble-rate", type=float, default=50.0, help="Poisson arrivals per second")
@@ -329,6 +339,15 @@ def main():
         r32 = run_lattice(ax, a.n, 32, a.steps, a.warmup, a.diag_every, a.seed, log)
         fp32 = {"value": r32["value"], "unit": "site-updates/s", "ms_per_step": r32["ms_per_step"],
                 "roofline": r32["roofline"], "vs_fp64": compare_series(main_run["series"], r32["series"])}
+    moore = None
+    if a.moore == "on":
+        rm = run_lattice(ax, a.moore_n, 64, a.moore_steps, 10, 20, a.seed, log, moore=True, moore_dt=a.moore_dt)
+        last_m = rm["series"][-1]
+        moore = {"workload": f"Moore {a.moore_n}^3 periodic FP64, mooreEvol.cpp couplings (q=2,1, dx=0.7), "
+                             f"dt={a.moore_dt}, {a.moore_steps} steps, energy+Gauss+winding every 20",
+                 "value": rm["value"], "unit": "site-updates/s", "ms_per_step": rm["ms_per_step"],
+                 "roofline": rm["roofline"], "final_wound_plaquettes": last_m[3],
+                 "finite_to_end": bool(all(r[5] for r in rm["series"]))}
     ens = None
     if a.ensemble == "on":
         ens = ensemble(ax, a.ensemble_count, a.ensemble_rate, a.ensemble_seed, a.ensemble_budget_gb,
@@ -357,6 +376,7 @@ def main():
         "roofline": main_run["roofline"],
         "cpu_baseline": cpu,
         "fp32": fp32,
+        "moore": moore,
         "ensemble": ens,
         "final_diag": {"timestep": last[0], "energy": last[1], "gauss_mean": last[2], "wound_plaquettes": last[3],
                        "xi": last[4]},
