@@ -598,15 +598,17 @@ int launch_step(ax_lattice *L, int n) {
     return e == hipSuccess ? AX_OK : hip_fail(e, "global step launch");
   }
   StepConsts k = step_consts(&L->p, n);
-  const bool tiled = use_tiled_ah(L->p);
+  const bool tiled = use_tiled_ah(L->p), moore = L->p.model == AX_MODEL_MOORE;
   if (L->esize == 8) {
     Levels<double> lv = levels_of<double>(L, n);
-    e = tiled ? launch_step_tiled_ah<double>(k, lv, L->xseg, L->stream)
-              : launch_step_site<double>(k, lv, L->p.model, L->stream);
+    e = !tiled ? launch_step_site<double>(k, lv, L->p.model, L->stream)
+               : moore ? launch_step_tiled_moore<double>(k, lv, L->xseg, L->stream)
+                       : launch_step_tiled_ah<double>(k, lv, L->xseg, L->stream);
   } else {
     Levels<float> lv = levels_of<float>(L, n);
-    e = tiled ? launch_step_tiled_ah<float>(k, lv, L->xseg, L->stream)
-              : launch_step_site<float>(k, lv, L->p.model, L->stream);
+    e = !tiled ? launch_step_site<float>(k, lv, L->p.model, L->stream)
+               : moore ? launch_step_tiled_moore<float>(k, lv, L->xseg, L->stream)
+                       : launch_step_tiled_ah<float>(k, lv, L->xseg, L->stream);
   }
   return e == hipSuccess ? AX_OK : hip_fail(e, "step launch");
 }

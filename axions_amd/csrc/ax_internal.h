@@ -42,6 +42,8 @@ bool use_tiled_ah(const ax_params &p);
 int default_xseg(const ax_params &p);
 template <typename R>
 hipError_t launch_step_tiled_ah(const StepConsts &k, const Levels<R> &L, int xseg, hipStream_t st);
+template <typename R>
+hipError_t launch_step_tiled_moore(const StepConsts &k, const Levels<R> &L, int xseg, hipStream_t st);
 
 template <typename R>
 hipError_t launch_global_step(const ax_params &p, void *state, size_t pitch, int n, hipStream_t st);

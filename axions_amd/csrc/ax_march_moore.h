@@ -1,0 +1,172 @@
+/*
+ * ax_march_moore.h - the Moore-model update (mooreEvol.cpp:202-337: two
+ * complex scalars with charges q1, q2 on one U(1) link field, no friction)
+ * as a 64-lane wave marching along x over a z chunk of one row, like
+ * ah_march.  Same arithmetic and operand order as moore_update in
+ * ax_step.hip; the cos/sin of q_f theta_x(s-x) and the plaquette sines at
+ * s-x come from the previous plane of the march, y/z-backward values are
+ * recomputed, the six plaquette sines at s are computed once and take the
+ * short k = 0 form when the whole wave qualifies.
+ */
+#ifndef AXIONS_AX_MARCH_MOORE_H
+#define AXIONS_AX_MARCH_MOORE_H
+
+#include "ax_march.h"
+
+namespace ax {
+
+template <typename R>
+struct MooreConsts {
+  R dx2, dy2, dz2, dt2, m2gg;
+  R hl1, hl2, hl12, eta1sq, eta2sq, q1, q2;
+  R inv_dx2, inv_dy2, inv_dz2;
+};
+
+template <typename R>
+inline void moore_consts_from(const StepConsts &k, MooreConsts<R> &m) {
+  m.dx2 = (R)k.dx2; m.dy2 = (R)k.dy2; m.dz2 = (R)k.dz2; m.dt2 = (R)k.dt2; m.m2gg = (R)k.m2gg;
+  m.hl1 = (R)k.hl1; m.hl2 = (R)k.hl2; m.hl12 = (R)k.hl12; m.eta1sq = (R)k.eta1sq; m.eta2sq = (R)k.eta2sq;
+  m.q1 = (R)k.q1; m.q2 = (R)k.q2;
+  m.inv_dx2 = (R)1 / m.dx2; m.inv_dy2 = (R)1 / m.dy2; m.inv_dz2 = (R)1 / m.dz2;
+}
+
+template <typename R, bool POW2>
+__device__ __forceinline__ void moore_march(const MooreConsts<R> &k, const Levels<R> &L, int NX, int NY, int NZ, int y,
+                                            int zc, int x0, int x1) {
+  typedef TrigR<R> T;
+  const int lane = (int)(threadIdx.x & 63);
+  const int zr = zc * 64 + lane;
+  const bool act = zr < NZ;
+  const int z = act ? zr : NZ - 1;
+  const int ym = y == 0 ? NY - 1 : y - 1, yp = y == NY - 1 ? 0 : y + 1;
+  const uint32_t E = (uint32_t)sizeof(R);
+  const uint32_t oz = (uint32_t)z * E;
+  const uint32_t ozm = (uint32_t)(z == 0 ? NZ - 1 : z - 1) * E;
+  const uint32_t ozp = (uint32_t)(z == NZ - 1 ? 0 : z + 1) * E;
+  const uint32_t PLB = (uint32_t)NY * (uint32_t)NZ * E;
+  const uint32_t R0 = (uint32_t)y * NZ * E, RM = (uint32_t)ym * NZ * E, RP = (uint32_t)yp * NZ * E;
+  const R *F[7] = {L.cur[0], L.cur[1], L.cur[2], L.cur[3], L.cur[4], L.cur[5], L.cur[6]};
+  R *O[7] = {L.prv[0], L.prv[1], L.prv[2], L.prv[3], L.prv[4], L.prv[5], L.prv[6]};
+  const R *TX = F[4], *TY = F[5], *TZ = F[6];
+  const R q[2] = {k.q1, k.q2};
+
+  /* carry from plane x-1: phi_f(s-x), cos/sin(q_f theta_x(s-x)), sines at s-x */
+  const int xm0 = x0 == 0 ? NX - 1 : x0 - 1;
+  R Pm[4], cxm[2], sxm[2], Qyx_m, Qzx_m;
+  {
+    const uint32_t b = (uint32_t)xm0 * PLB, bn = (uint32_t)x0 * PLB;
+    for (int c = 0; c < 4; ++c) Pm[c] = ldo(F[c], b + R0, oz);
+    const R X = ldo(TX, b + R0, oz), Y = ldo(TY, b + R0, oz), Z = ldo(TZ, b + R0, oz);
+    for (int f = 0; f < 2; ++f) T::sc(q[f] * X, &sxm[f], &cxm[f]);
+    Qyx_m = T::sn(Y + ldo(TX, b + RP, oz) - ldo(TY, bn + R0, oz) - X);
+    Qzx_m = T::sn(Z + ldo(TX, b + R0, ozp) - ldo(TZ, bn + R0, oz) - X);
+  }
+  const uint32_t b0 = (uint32_t)x0 * PLB;
+  R C[7];
+  for (int c = 0; c < 7; ++c) C[c] = ldo(F[c], b0 + R0, oz);
+
+  for (int x = x0; x < x1; ++x) {
+    const int xp = x == NX - 1 ? 0 : x + 1;
+    const uint32_t b = (uint32_t)x * PLB, bn = (uint32_t)xp * PLB, so = b + R0;
+    R Nx[7];
+    for (int c = 0; c < 7; ++c) Nx[c] = ldo(F[c], bn + R0, oz);
+    const R X = C[4], Y = C[5], Z = C[6];
+    const R TXyp = ldo(TX, b + RP, oz), TZyp = ldo(TZ, b + RP, oz);
+    const R TXzp = ldo(TX, b + R0, ozp), TYzp = ldo(TY, b + R0, ozp);
+    const R TXym = ldo(TX, b + RM, oz), TYym = ldo(TY, b + RM, oz), TZym = ldo(TZ, b + RM, oz);
+    const R TYxpym = ldo(TY, bn + RM, oz), TYymzp = ldo(TY, b + RM, ozp);
+    const R TXzm = ldo(TX, b + R0, ozm), TYzm = ldo(TY, b + R0, ozm), TZzm = ldo(TZ, b + R0, ozm);
+    const R TZxpzm = ldo(TZ, bn + R0, ozm), TZypzm = ldo(TZ, b + RP, ozm);
+
+    /* plaquette sines (mooreEvol.cpp:310-333 operand order) */
+    const R a_xy = X + Nx[5] - TXyp - Y;
+    const R a_xz = X + Nx[6] - TXzp - Z;
+    const R a_yx = Y + TXyp - Nx[5] - X;
+    const R a_yz = Y + TZyp - TYzp - Z;
+    const R a_zx = Z + TXzp - Nx[6] - X;
+    const R a_zy = Z + TYzp - TZyp - Y;
+    const R b_xy = TXym + TYxpym - X - TYym;
+    const R b_zy = TZym + TYymzp - Z - TYym;
+    const R b_xz = TXzm + TZxpzm - X - TZzm;
+    const R b_yz = TYzm + TZypzm - Y - TZzm;
+    R Qxy, Qxz, Qyx, Qyz, Qzx, Qzy, Qxy_m, Qzy_m, Qxz_m, Qyz_m;
+    const bool small = T::small(a_xy) && T::small(a_xz) && T::small(a_yx) && T::small(a_yz) &&
+                       T::small(a_zx) && T::small(a_zy) && T::small(b_xy) && T::small(b_zy) &&
+                       T::small(b_xz) && T::small(b_yz);
+    if (__all(small)) {
+      Qxy = T::sn_small(a_xy); Qxz = T::sn_small(a_xz); Qyx = T::sn_small(a_yx);
+      Qyz = T::sn_small(a_yz); Qzx = T::sn_small(a_zx); Qzy = T::sn_small(a_zy);
+      Qxy_m = T::sn_small(b_xy); Qzy_m = T::sn_small(b_zy); Qxz_m = T::sn_small(b_xz); Qyz_m = T::sn_small(b_yz);
+    } else {
+      Qxy = T::sn(a_xy); Qxz = T::sn(a_xz); Qyx = T::sn(a_yx);
+      Qyz = T::sn(a_yz); Qzx = T::sn(a_zx); Qzy = T::sn(a_zy);
+      Qxy_m = T::sn(b_xy); Qzy_m = T::sn(b_zy); Qxz_m = T::sn(b_xz); Qyz_m = T::sn(b_yz);
+    }
+    const R fxy = mdiv<R, POW2>(Qxy - Qxy_m, k.dy2, k.inv_dy2);
+    const R fxz = mdiv<R, POW2>(Qxz - Qxz_m, k.dz2, k.inv_dz2);
+    const R fyx = mdiv<R, POW2>(Qyx - Qyx_m, k.dx2, k.inv_dx2);
+    const R fyz = mdiv<R, POW2>(Qyz - Qyz_m, k.dz2, k.inv_dz2);
+    const R fzx = mdiv<R, POW2>(Qzx - Qzx_m, k.dx2, k.inv_dx2);
+    const R fzy = mdiv<R, POW2>(Qzy - Qzy_m, k.dy2, k.inv_dy2);
+
+    const R m1 = C[0] * C[0] + C[1] * C[1];
+    const R m2 = C[2] * C[2] + C[3] * C[3];
+    const R pot[2] = {k.hl1 * (m1 - k.eta1sq) + k.hl12 * (m2 - k.eta2sq),
+                      k.hl2 * (m2 - k.eta2sq) + k.hl12 * (m1 - k.eta1sq)};
+    R cur[2][3], acc[2][2], cxs[2], sxs[2];
+#pragma unroll
+    for (int f = 0; f < 2; ++f) {
+      const int cr = 2 * f, ci = 2 * f + 1;
+      R sx, cx, sy, cy, sz, cz, sym_, cym, szm, czm;
+      T::sc(q[f] * X, &sx, &cx);
+      T::sc(q[f] * Y, &sy, &cy);
+      T::sc(q[f] * Z, &sz, &cz);
+      T::sc(q[f] * TYym, &sym_, &cym);
+      T::sc(q[f] * TZzm, &szm, &czm);
+      const R ar = C[cr], ai = C[ci];
+      const R rxp = Nx[cr], ixp = Nx[ci];
+      const R ryp = ldo(F[cr], b + RP, oz), iyp = ldo(F[ci], b + RP, oz);
+      const R rym = ldo(F[cr], b + RM, oz), iym = ldo(F[ci], b + RM, oz);
+      const R rzp = ldo(F[cr], b + R0, ozp), izp = ldo(F[ci], b + R0, ozp);
+      const R rzm = ldo(F[cr], b + R0, ozm), izm = ldo(F[ci], b + R0, ozm);
+      {
+        R lxx = mdiv<R, POW2>(cx * rxp + sx * ixp - 2 * ar + cxm[f] * Pm[cr] - sxm[f] * Pm[ci], k.dx2, k.inv_dx2);
+        R lyy = mdiv<R, POW2>(cy * ryp + sy * iyp - 2 * ar + cym * rym - sym_ * iym, k.dy2, k.inv_dy2);
+        R lzz = mdiv<R, POW2>(cz * rzp + sz * izp - 2 * ar + czm * rzm - szm * izm, k.dz2, k.inv_dz2);
+        acc[f][0] = lxx + lyy + lzz - pot[f] * ar;
+      }
+      {
+        R lxx = mdiv<R, POW2>(cx * ixp + (-sx) * rxp - 2 * ai + cxm[f] * Pm[ci] - (-sxm[f]) * Pm[cr], k.dx2, k.inv_dx2);
+        R lyy = mdiv<R, POW2>(cy * iyp + (-sy) * ryp - 2 * ai + cym * iym - (-sym_) * rym, k.dy2, k.inv_dy2);
+        R lzz = mdiv<R, POW2>(cz * izp + (-sz) * rzp - 2 * ai + czm * izm - (-szm) * rzm, k.dz2, k.inv_dz2);
+        acc[f][1] = lxx + lyy + lzz - pot[f] * ai;
+      }
+      cur[f][0] = q[f] * (cx * (ai * rxp - ar * ixp) + sx * (ar * rxp + ai * ixp));
+      cur[f][1] = q[f] * (cy * (ai * ryp - ar * iyp) + sy * (ar * ryp + ai * iyp));
+      cur[f][2] = q[f] * (cz * (ai * rzp - ar * izp) + sz * (ar * rzp + ai * izp));
+      cxs[f] = cx;
+      sxs[f] = sx;
+    }
+    const R atx = k.m2gg * (cur[0][0] + cur[1][0]) - fxy - fxz;
+    const R aty = k.m2gg * (cur[0][1] + cur[1][1]) - fyx - fyz;
+    const R atz = k.m2gg * (cur[0][2] + cur[1][2]) - fzx - fzy;
+    R Onew[7];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) Onew[c] = 2 * C[c] - ldo(O[c], so, oz) + k.dt2 * acc[c >> 1][c & 1];
+    Onew[4] = 2 * X - ldo(O[4], so, oz) + k.dt2 * atx;
+    Onew[5] = 2 * Y - ldo(O[5], so, oz) + k.dt2 * aty;
+    Onew[6] = 2 * Z - ldo(O[6], so, oz) + k.dt2 * atz;
+    if (act)
+      for (int c = 0; c < 7; ++c) sto(O[c], so, oz, Onew[c]);
+    /* carry */
+    for (int c = 0; c < 4; ++c) Pm[c] = C[c];
+    for (int f = 0; f < 2; ++f) { cxm[f] = cxs[f]; sxm[f] = sxs[f]; }
+    Qyx_m = Qyx;
+    Qzx_m = Qzx;
+    for (int c = 0; c < 7; ++c) C[c] = Nx[c];
+  }
+}
+
+}  // namespace ax
+
+#endif

@@ -12,6 +12,7 @@
 
 #include "ax_internal.h"
 #include "ax_march.h"
+#include "ax_march_moore.h"
 
 namespace ax {
 
@@ -33,9 +34,40 @@ __global__ void __launch_bounds__(64) ah_tiled_kernel(MarchConsts<R> k, Levels<R
   ah_march<R, POW2, PLAIN ? 1 : 0>(k, L, nx, ny, nz, y, zc, x0, x1);
 }
 
+template <typename R, bool POW2>
+__global__ void __launch_bounds__(64) moore_tiled_kernel(MooreConsts<R> k, Levels<R> L, int nx, int ny, int nz,
+                                                         TileGrid g) {
+  const int w = (int)blockIdx.x;
+  if (w >= g.work) return;
+  const int zc = w % g.nw, r = w / g.nw;
+  const int y = r % ny, seg = r / ny;
+  const int x0 = seg * g.xseg;
+  const int x1 = min(nx, x0 + g.xseg);
+  moore_march<R, POW2>(k, L, nx, ny, nz, y, zc, x0, x1);
+}
+
+template <typename R>
+hipError_t launch_step_tiled_moore(const StepConsts &k, const Levels<R> &L, int xseg, hipStream_t st) {
+  TileGrid g;
+  g.xseg = (xseg <= 0 || xseg > k.nx) ? k.nx : xseg;
+  g.nw = (k.nz + 63) / 64;
+  g.work = g.nw * k.ny * ((k.nx + g.xseg - 1) / g.xseg);
+  MooreConsts<R> m;
+  moore_consts_from<R>(k, m);
+  const bool pow2 = is_pow2(k.dx2) && is_pow2(k.dy2) && is_pow2(k.dz2);
+  dim3 grid(g.work), block(64);
+  if (pow2) hipLaunchKernelGGL((moore_tiled_kernel<R, true>), grid, block, 0, st, m, L, k.nx, k.ny, k.nz, g);
+  else hipLaunchKernelGGL((moore_tiled_kernel<R, false>), grid, block, 0, st, m, L, k.nx, k.ny, k.nz, g);
+  return hipGetLastError();
+}
+
+template hipError_t launch_step_tiled_moore<double>(const StepConsts &, const Levels<double> &, int, hipStream_t);
+template hipError_t launch_step_tiled_moore<float>(const StepConsts &, const Levels<float> &, int, hipStream_t);
+
 bool use_tiled_ah(const ax_params &p) {
   const int tail = p.nz % 64;
-  return p.model == AX_MODEL_AH && p.boundary == AX_BC_PERIODIC && (tail == 0 || tail >= 32) &&
+  return (p.model == AX_MODEL_AH || p.model == AX_MODEL_MOORE) && p.boundary == AX_BC_PERIODIC &&
+         (tail == 0 || tail >= 32) &&
          p.nx >= 3 && p.ny >= 3 &&
          (double)p.nx * p.ny * p.nz * (p.precision == 64 ? 8.0 : 4.0) < 4294967296.0 &&
          getenv("AX_FORCE_SITE_KERNEL") == nullptr;

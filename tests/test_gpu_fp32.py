@@ -98,6 +98,27 @@ def test_tile_shape_does_not_change_results(precision):
         assert np.array_equal(o.view(np.uint64), outs[0].view(np.uint64))
 
 
+def test_fp32_moore_march_equals_site_kernel():
+    """FP32 Moore has no oracle of its own: the marching kernel must equal
+    the site kernel (the FP32 definition) bit for bit."""
+    import os
+    p = ax.moore_params(12, ny=10, nz=64, dt=0.2, precision=32)
+    say("[fp32 moore] march vs site")
+    outs = []
+    for force in ("", "1"):
+        if force:
+            os.environ["AX_FORCE_SITE_KERNEL"] = "1"
+        try:
+            with ax.Lattice(p) as lat:
+                assert lat.tile()[1] == (not force)
+                lat.init_random_phase(21)
+                lat.step(12)
+                outs.append(lat.get_state())
+        finally:
+            os.environ.pop("AX_FORCE_SITE_KERNEL", None)
+    assert np.array_equal(outs[0].view(np.uint64), outs[1].view(np.uint64))
+
+
 def test_fp32_diagnostics_and_winding_vs_oracle():
     """FP64 reductions over the FP32 state: the oracle's FP64 diagnostics of
     the same (exactly widened) state."""

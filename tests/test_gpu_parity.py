@@ -57,6 +57,9 @@ CASES = {
     "tiled_aniso_plain": (lambda: ax.ah_params(70, ny=5, nz=64, dx=0.7, dy=0.6, dz=0.55, dt=0.13), 12),
     "tiled_512_row": (lambda: ax.ah_params(33, ny=4, nz=512), 6),
     "moore_periodic_12": (lambda: ax.moore_params(12), 25),
+    # periodic Moore with nz >= 32 takes the marching kernel
+    "moore_tiled_64": (lambda: ax.moore_params(20, ny=12, nz=64, dt=0.2), 12),
+    "moore_tiled_96_aniso": (lambda: ax.moore_params(9, ny=8, nz=96, dx=0.6, dt=0.17), 10),
     "moore_fixed_12": (lambda: ax.moore_params(12, boundary=AX_BC_FIXED), 25),
 }
 
