@@ -44,6 +44,21 @@ class AxParams(ctypes.Structure):
     ]
 
 
+class AxSorParams(ctypes.Structure):
+    """ax_sor_params: constants of singleSemiAnalytic.cpp:19-32."""
+    _fields_ = [
+        ("nx", ctypes.c_int32),
+        ("max_iter", ctypes.c_int32),
+        ("h", ctypes.c_double),
+        ("w", ctypes.c_double),
+        ("tol", ctypes.c_double),
+        ("n", ctypes.c_double),
+        ("lambda_", ctypes.c_double),
+        ("eta", ctypes.c_double),
+        ("g", ctypes.c_double),
+    ]
+
+
 class AxDiag(ctypes.Structure):
     _fields_ = [
         ("timestep", ctypes.c_int32),

@@ -13,7 +13,8 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 REPO = os.path.dirname(HERE)
-SRC = ["ax_api.hip", "ax_step.hip", "ax_step_tiled.hip", "ax_diag.hip", "ax_pool.hip", "ax_global.hip", "ax_length.hip"]
+SRC = ["ax_api.hip", "ax_step.hip", "ax_step_tiled.hip", "ax_diag.hip", "ax_pool.hip", "ax_global.hip", "ax_length.hip",
+       "ax_profile.hip"]
 LIB = os.path.join(HERE, "libaxions.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("AX_OFFLOAD_ARCH", "gfx950")

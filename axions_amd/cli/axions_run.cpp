@@ -27,6 +27,13 @@
  *   --program leapfrog       leapfrog.cpp: ic.txt (2 columns), valsPerLoop.txt
  *                            (E per step), GifData/gifData_<frame>.txt (|phi|^2),
  *                            finalField.txt (leapfrog.cpp:74-112, 323-386)
+ *   --program sor            singleSemiAnalytic.cpp: straight-string radial
+ *                            profile to Data/SOR_Fields.txt "phi A" per point
+ *                            (ic_type 1 starts from that file)
+ *                            (singleSemiAnalytic.cpp:19-32, 54-142)
+ *   --program straight_ic    straight_ic.cpp: SOR_Fields.txt (SORnx rows) ->
+ *                            ic.txt, 5 columns "Re phi, Im phi, dx g Ax,
+ *                            dy g Ay, dz g Az" (straight_ic.cpp:20-31, 53-150)
  *
  * Constants are set with name=value arguments (e.g. nx=256 nt=1001 g=1);
  * --dir is the directory the reference's dir_path would be (default "."),
@@ -39,7 +46,8 @@
  * of evolution.cpp (uninitialised there: its chaining, evolution.cpp:1229-1330,
  * is commented out) hold the chained length of ic_w_evolution.cpp:961-1087
  * (total, or inner/outer with splitLength) via ax_string_length; all gauged
- * runs use the library's shared sin/cos.
+ * runs use the library's shared sin/cos; straight_ic reports the number of
+ * sites beyond its profile once instead of one line per site.
  */
 #include <errno.h>
 #include <stdio.h>
@@ -105,6 +113,12 @@ std::map<std::string, std::string> defaults(const std::string &prog) {
     return {{"nx", "200"}, {"ny", "200"}, {"nz", "200"}, {"nt", "6001"}, {"dx", "0.25"}, {"dy", "0.25"},
             {"dz", "0.25"}, {"dt", "0.1"}, {"makeGif", "false"}, {"saveFreq", "60"}, {"xyBC", "neumann"},
             {"precision", "64"}};
+  if (prog == "sor")   /* singleSemiAnalytic.cpp:19-32 */
+    return {{"nx", "30001"}, {"h", "0.01"}, {"w", "1.5"}, {"maxIter", "2000000"}, {"tol", "1e-3"},
+            {"ic_type", "0"}, {"n", "1"}, {"lambda", "1"}, {"eta", "1"}, {"g", "0"}};
+  if (prog == "straight_ic")   /* straight_ic.cpp:20-31 */
+    return {{"nx", "101"}, {"ny", "101"}, {"nz", "51"}, {"dx", "0.5"}, {"dy", "0.5"}, {"dz", "0.5"},
+            {"n", "1"}, {"g", "0.5"}, {"SORnx", "20001"}, {"SORa", "0.01"}, {"precision", "64"}};
   die("unknown --program " + prog);
 }
 
@@ -359,6 +373,55 @@ struct Run {
     write_rows(D("Data/finalField.txt"), newest(7), 7);
   }
 
+  void sor() {
+    ax_sor_params sp;
+    ax_sor_default_params(&sp);
+    sp.nx = C.i("nx");
+    sp.h = C.d("h");
+    sp.w = C.d("w");
+    sp.max_iter = C.i("maxIter");
+    sp.tol = C.d("tol");
+    sp.n = C.d("n");
+    sp.lambda = C.d("lambda");
+    sp.eta = C.d("eta");
+    sp.g = C.d("g");
+    std::vector<double> init;
+    if (C.i("ic_type") == 1) init = read_numbers(D("Data/SOR_Fields.txt"), 2 * (size_t)sp.nx);
+    std::vector<double> prof(2 * (size_t)sp.nx);
+    int32_t iters = 0;
+    double res = 0;
+    check(ax_sor_profile(&sp, init.empty() ? nullptr : init.data(), prof.data(), &iters, &res), "ax_sor_profile");
+    Out f(D("Data/SOR_Fields.txt"), o.digits);
+    for (int j = 0; j < sp.nx; ++j) {
+      f.num(prof[2 * (size_t)j], false);
+      f.num(prof[2 * (size_t)j + 1], true);
+    }
+    if (iters == sp.max_iter) printf("Maximum number of iterations reached\n");
+    printf("Number of iterations: %d\n", iters);
+  }
+
+  void straight_ic() {
+    ax_default_params(&p);                 /* the lattice is only written, not stepped */
+    p.nx = C.i("nx");
+    p.ny = C.i("ny");
+    p.nz = C.i("nz");
+    p.dx = C.d("dx");
+    p.dy = C.d("dy");
+    p.dz = C.d("dz");
+    p.precision = C.i("precision");
+    V = (size_t)p.nx * p.ny * p.nz;
+    p.g = C.d("g");
+    p.boundary = AX_BC_FIXED;
+    const int sn = C.i("SORnx");
+    const std::vector<double> prof = read_numbers(D("SOR_Fields.txt"), 2 * (size_t)sn);
+    check(ax_create(&p, &lat), "ax_create");
+    int64_t off = 0;
+    check(ax_init_straight_string(lat, prof.data(), sn, C.d("SORa"), C.d("n"), p.g, &off),
+          "ax_init_straight_string");
+    if (off) printf("Off straight string solution grid: %lld sites\n", (long long)off);
+    write_rows(D("ic.txt"), newest(5), 5);
+  }
+
   void leapfrog() {
     set_common(AX_MODEL_GLOBAL);
     p.boundary = C.s("xyBC") == "periodic" ? AX_BC_PERIODIC : AX_BC_NEUMANN_XY;
@@ -416,7 +479,7 @@ int main(int argc, char **argv) {
     else if (a == "--tag") r.o.tag = val("--tag");
     else if (a == "--digits") r.o.digits = atoi(val("--digits").c_str());
     else if (a == "-h" || a == "--help") {
-      printf("usage: axions_run --program evolution|mpi_evolution|moore|leapfrog [--dir D] [--tag T] "
+      printf("usage: axions_run --program evolution|mpi_evolution|moore|leapfrog|sor|straight_ic [--dir D] [--tag T] "
              "[--digits N] [name=value ...]\n");
       return 0;
     } else if (a.find('=') != std::string::npos) r.o.kv[a.substr(0, a.find('='))] = a.substr(a.find('=') + 1);
@@ -433,6 +496,8 @@ int main(int argc, char **argv) {
   if (r.o.program == "evolution") r.evolution();
   else if (r.o.program == "mpi_evolution") r.mpi_evolution();
   else if (r.o.program == "moore") r.moore();
+  else if (r.o.program == "sor") r.sor();
+  else if (r.o.program == "straight_ic") r.straight_ic();
   else r.leapfrog();
   if (r.lat) {
     check(ax_sync(r.lat), "ax_sync");

@@ -128,6 +128,8 @@ hipStream_t lattice_stream(ax_lattice *L) { return L->stream; }
 const ax_params &lattice_params(const ax_lattice *L) { return L->p; }
 void *lattice_state(ax_lattice *L) { return L->state; }
 size_t lattice_sites(const ax_lattice *L) { return L->V; }
+size_t lattice_pitch(const ax_lattice *L) { return L->pitch; }
+void lattice_reset_timestep(ax_lattice *L) { L->timestep = 0; }
 
 }  // namespace ax
 

@@ -31,6 +31,8 @@ hipStream_t lattice_stream(ax_lattice *L);
 const ax_params &lattice_params(const ax_lattice *L);
 void *lattice_state(ax_lattice *L);
 size_t lattice_sites(const ax_lattice *L);
+size_t lattice_pitch(const ax_lattice *L);
+void lattice_reset_timestep(ax_lattice *L);
 
 int launch_step(ax_lattice *L, int n);
 int ensure_winding_buffer(ax_lattice *L);

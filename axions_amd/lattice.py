@@ -11,7 +11,7 @@ import os
 
 import numpy as np
 
-from .abi import AxDiag, AxParams, AX_OK, ncomp, state_len
+from .abi import AxDiag, AxParams, AxSorParams, AX_OK, ncomp, state_len
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libaxions.so")
@@ -56,6 +56,11 @@ def _load():
                                             ctypes.c_double, i32, i32, vp]),
         "ax_get_tile": (ctypes.c_int, [vp, ctypes.POINTER(i32), ctypes.POINTER(i32)]),
         "ax_kernel_time": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int64)]),
+        "ax_sor_default_params": (None, [ctypes.POINTER(AxSorParams)]),
+        "ax_sor_profile": (ctypes.c_int, [ctypes.POINTER(AxSorParams), vp, vp, ctypes.POINTER(i32),
+                                          ctypes.POINTER(ctypes.c_double)]),
+        "ax_init_straight_string": (ctypes.c_int, [vp, vp, i32, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                                   ctypes.POINTER(ctypes.c_int64)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -79,6 +84,34 @@ def string_length(xyz, nx, ny, nz, dx, dy, dz, periodic_xy=True, periodic_z=True
     _check(lib.ax_string_length(pts.ctypes.data if pts.size else None, pts.shape[0], nx, ny, nz, dx, dy, dz,
                                 1 if periodic_xy else 0, 1 if periodic_z else 0, out.ctypes.data))
     return float(out[0]), float(out[1]), float(out[2])
+
+
+def sor_params(**kw):
+    """ax_sor_params with singleSemiAnalytic.cpp's constants, overridden by
+    keyword (nx, max_iter, h, w, tol, n, lam, eta, g)."""
+    p = AxSorParams()
+    lib.ax_sor_default_params(ctypes.byref(p))
+    for k, v in kw.items():
+        setattr(p, "lambda_" if k in ("lam", "lambda") else k, v)
+    return p
+
+
+def sor_profile(params=None, init=None, **kw):
+    """Straight-string radial profile by SOR (singleSemiAnalytic.cpp:54-134)
+    on the GPU: (profile [nx, 2] of (f, A), sweeps, last residual).  init:
+    a starting profile [nx, 2] (the reference's ic_type 1) or None (tanh)."""
+    p = params if params is not None else sor_params(**kw)
+    out = np.empty((p.nx, 2), dtype=np.float64)
+    ini = None
+    if init is not None:
+        ini = np.ascontiguousarray(init, dtype=np.float64)
+        if ini.shape != (p.nx, 2):
+            raise ValueError(f"init profile shape {ini.shape} != ({p.nx}, 2)")
+    it = ctypes.c_int32()
+    res = ctypes.c_double()
+    _check(lib.ax_sor_profile(ctypes.byref(p), None if ini is None else ini.ctypes.data, out.ctypes.data,
+                              ctypes.byref(it), ctypes.byref(res)))
+    return out, it.value, res.value
 
 
 def random_phases(seed, n):
@@ -122,6 +155,17 @@ class Lattice:
 
     def init_random_phase(self, seed=42, moore_phase_mode=0):
         _check(lib.ax_init_random_phase(self._h, seed, moore_phase_mode))
+
+    def init_straight_string(self, profile, dr, n=1.0, g=None):
+        """Straight string along z from a radial profile [nprof, 2] of
+        (f, A) spaced dr (straight_ic.cpp:59-146); g defaults to the
+        lattice's coupling.  Returns the number of sites beyond the
+        profile."""
+        pr = np.ascontiguousarray(profile, dtype=np.float64).reshape(-1, 2)
+        off = ctypes.c_int64()
+        gg = self.params.g if g is None else g
+        _check(lib.ax_init_straight_string(self._h, pr.ctypes.data, pr.shape[0], dr, n, gg, ctypes.byref(off)))
+        return off.value
 
     def set_state(self, st, timestep=0):
         st = np.ascontiguousarray(st, dtype=np.float64)

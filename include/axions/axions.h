@@ -187,6 +187,47 @@ int ax_string_length(const double *xyz, size_t n, int32_t nx, int32_t ny, int32_
                      double dz, int32_t periodic_xy, int32_t periodic_z, double *out);
 
 /* ------------------------------------------------------------------------
+ * Straight-string initial condition (singleSemiAnalytic.cpp ->
+ * straight_ic.cpp).
+ *
+ * ax_sor_profile: radial profile f(r_i), A(r_i), r_i = i*h, of a straight
+ * string of winding n by successive over-relaxation of the static field
+ * equations, singleSemiAnalytic.cpp:54-134 in its one-thread order (each
+ * sweep relaxes i = 1 .. nx-2 in place; f(0) = A(0) = 0; f(nx-1), A(nx-1)
+ * copy i = nx-2 before each sweep).  Sweeps stop after the first one whose
+ * largest |F| is not above tol, or after max_iter.  init: nx (f, A) pairs,
+ * or NULL for the reference's f = eta*tanh(r/2), A = 0 (its ic_type 0; 1
+ * reads the previous output, which is passing it here).  out: nx (f, A)
+ * pairs, the rows of Data/SOR_Fields.txt.  Runs on the current HIP device
+ * and waits for it; allocates and frees its own buffers. */
+typedef struct ax_sor_params {
+  int32_t nx;          /* grid points (singleSemiAnalytic.cpp:19) */
+  int32_t max_iter;    /* :22 */
+  double h;            /* spacing :20 */
+  double w;            /* relaxation factor :21 */
+  double tol;          /* :23 */
+  double n;            /* winding :29 */
+  double lambda, eta;  /* :30-31 */
+  double g;            /* gauge coupling; 0 = global string :32 */
+} ax_sor_params;
+void ax_sor_default_params(ax_sor_params *p);
+int ax_sor_profile(const ax_sor_params *p, const double *init, double *out, int32_t *iterations, double *residual);
+
+/* Straight string along z through (x0, y0) = ((nx-1)/2, (ny-1)/2) (integer
+ * division), interpolated from a profile of nprof (f, A) pairs spaced dr,
+ * as straight_ic.cpp:59-146: quadratic through the three profile points
+ * around the nearest one (linear next to r = 0); beyond the profile |phi| =
+ * 1 and A = n/g (0 if g = 0).  phi = |phi| (x, y)/r, A = A(r) (-y, x)/r^2,
+ * stored as the link variables (dx g A_x, dy g A_y, 0) at both levels
+ * (a stationary start, as the ic.txt reader of evolution.cpp:102-117); the
+ * global model gets phi and zero momentum.  TimeStep is reset to 0.
+ * *off_grid (may be NULL) counts sites beyond the profile.  The reference
+ * reads one entry past its table when the nearest point is the last one;
+ * here that point is also treated as beyond the profile. */
+int ax_init_straight_string(ax_lattice *lat, const double *profile, int32_t nprof, double dr, double n, double g,
+                            int64_t *off_grid);
+
+/* ------------------------------------------------------------------------
  * Ensemble pool: many independent realizations (seed, N, steps) of one
  * periodic Abelian-Higgs parameter set, served with continuous batching.
  * No counterpart in the reference, which runs one realization per job

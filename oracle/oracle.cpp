@@ -20,6 +20,8 @@
  *   periodic neighbour indices ............... mpi_evolution.cpp:385-405
  *   global string, momentum-form leapfrog .... leapfrog.cpp:166-347
  *   string length by chaining crossings ...... ic_w_evolution.cpp:961-1087
+ *   straight-string profile (SOR) ............ singleSemiAnalytic.cpp:54-134
+ *   straight-string lattice .................. straight_ic.cpp:59-146
  * The per-plaquette winding number has no reference counterpart.
  *
  * Pinning (what checks this file, see tests/): compiled reference runs in
@@ -933,6 +935,96 @@ int orc_string_length(const double *xyz, size_t n, int nx, int ny, int nz, doubl
   }
   out[0] = L0; out[1] = Lin; out[2] = Lout;
   return 0;
+}
+
+/* Straight-string profile by SOR, singleSemiAnalytic.cpp:54-134, one
+ * thread: each sweep relaxes j = 1 .. nx-2 in place in increasing j.  par =
+ * {h, w, tol, n, lambda, eta, g}; init = nx (phi, A) pairs or NULL for
+ * ic_type 0; out = nx (phi, A) pairs.  Returns the sweep count. */
+int orc_sor_profile(int nx, int max_iter, const double *par, const double *init, double *out, double *residual) {
+  const double h = par[0], w = par[1], tol = par[2], n = par[3], lambda = par[4], eta = par[5], g = par[6];
+  std::vector<double> phi(nx), A(nx);
+  for (int i = 0; i < nx; ++i) {
+    if (init) { phi[i] = init[2 * i]; A[i] = init[2 * i + 1]; }
+    else { phi[i] = eta * tanh(0.5 * h * i); A[i] = 0; }
+  }
+  phi[0] = 0;
+  A[0] = 0;
+  double tolTest = 1;
+  int iterNum = 0;
+  while (tolTest > tol && iterNum < max_iter) {
+    tolTest = 0;
+    iterNum += 1;
+    phi[nx - 1] = phi[nx - 2];
+    A[nx - 1] = A[nx - 2];
+    for (int j = 1; j < nx - 1; ++j) {
+      const double q = (n - g * A[j]) / (j * h);
+      const double F0 = (q * q + 0.5 * lambda * (phi[j] * phi[j] - eta * eta)) * phi[j] -
+                        (phi[j + 1] - phi[j - 1]) / (2 * j * h * h) -
+                        (phi[j + 1] - 2 * phi[j] + phi[j - 1]) / (h * h);
+      const double F1 = q * q + 0.5 * lambda * (3 * (phi[j] * phi[j]) - eta * eta) + 2 / (h * h);
+      const double G0 = -2 * g * (n - g * A[j]) * (phi[j] * phi[j]) + (A[j + 1] - A[j - 1]) / (2 * j * h * h) -
+                        (A[j + 1] - 2 * A[j] + A[j - 1]) / (h * h);
+      const double G1 = 2 * ((g * phi[j]) * (g * phi[j])) + 2 / (h * h);
+      phi[j] += -w * F0 / F1;
+      A[j] += -w * G0 / G1;
+      if (fabs(F0) > tolTest) tolTest = fabs(F0);
+      if (fabs(G0) > tolTest) tolTest = fabs(G0);
+    }
+  }
+  for (int i = 0; i < nx; ++i) { out[2 * i] = phi[i]; out[2 * i + 1] = A[i]; }
+  if (residual) *residual = tolTest;
+  return iterNum;
+}
+
+/* Straight string from a profile, straight_ic.cpp:59-146: the five
+ * ic.txt columns (Re phi, Im phi, dx g Ax, dy g Ay, dz g Az) per site in
+ * x-major order into st[5][nx*ny*nz].  The nearest profile point being the
+ * last one counts as beyond the profile (the reference reads past its table
+ * there).  Returns the number of sites beyond the profile. */
+long long orc_straight_string(int nx, int ny, int nz, double dx, double dy, double dz, const double *prof,
+                              int nprof, double dr, double n, double g, double *st) {
+  const size_t V = (size_t)nx * ny * nz;
+  const int x0 = (nx - 1) / 2, y0 = (ny - 1) / 2;
+  long long off = 0;
+  for (int i = 0; i < nx; ++i) {
+    const double x = (i - x0) * dx;
+    for (int j = 0; j < ny; ++j) {
+      const double y = (j - y0) * dy;
+      for (int k = 0; k < nz; ++k) {
+        const double distance = sqrt(x * x + y * y);
+        const int p = (int)round(distance / dr);
+        double phiMag, AMag;
+        if (p == 0) {
+          phiMag = (prof[2 * (p + 1)] * distance - prof[2 * p] * (distance - dr)) / dr;
+          AMag = (prof[2 * (p + 1) + 1] * distance - prof[2 * p + 1] * (distance - dr)) / dr;
+        } else if (p < nprof - 1) {
+          phiMag = (prof[2 * (p - 1)] * (distance - p * dr) * (distance - (p + 1) * dr) -
+                    2 * prof[2 * p] * (distance - (p - 1) * dr) * (distance - (p + 1) * dr) +
+                    prof[2 * (p + 1)] * (distance - (p - 1) * dr) * (distance - p * dr)) / (2 * dr * dr);
+          AMag = (prof[2 * (p - 1) + 1] * (distance - p * dr) * (distance - (p + 1) * dr) -
+                  2 * prof[2 * p + 1] * (distance - (p - 1) * dr) * (distance - (p + 1) * dr) +
+                  prof[2 * (p + 1) + 1] * (distance - (p - 1) * dr) * (distance - p * dr)) / (2 * dr * dr);
+        } else {
+          phiMag = 1;
+          AMag = g == 0 ? 0 : n / g;
+          ++off;
+        }
+        double f0, f1, a0, a1;
+        if (x == 0) { f0 = 0; a1 = 0; }
+        else { f0 = phiMag * x / distance; a1 = AMag * x / (distance * distance); }
+        if (y == 0) { f1 = 0; a0 = 0; }
+        else { f1 = phiMag * y / distance; a0 = -AMag * y / (distance * distance); }
+        const size_t s = ((size_t)i * ny + j) * nz + k;
+        st[s] = f0;
+        st[V + s] = f1;
+        st[2 * V + s] = dx * g * a0;
+        st[3 * V + s] = dy * g * a1;
+        st[4 * V + s] = dz * g * 0.0;
+      }
+    }
+  }
+  return off;
 }
 
 /* global model (AX_MODEL_GLOBAL): nsteps TimeSteps of leapfrog.cpp on st

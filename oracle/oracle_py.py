@@ -40,6 +40,12 @@ def lib():
         L.orc_crossings.argtypes = [P, vp, ctypes.c_int, vp, ctypes.c_size_t,
                                     ctypes.POINTER(ctypes.c_size_t)]
         L.orc_trig.argtypes = [ctypes.c_int, vp, vp, vp, ctypes.c_size_t]
+        L.orc_sor_profile.argtypes = [ctypes.c_int, ctypes.c_int, vp, vp, vp, vp]
+        L.orc_sor_profile.restype = ctypes.c_int
+        L.orc_straight_string.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                          ctypes.c_double, ctypes.c_double, vp, ctypes.c_int, ctypes.c_double,
+                                          ctypes.c_double, ctypes.c_double, vp]
+        L.orc_straight_string.restype = ctypes.c_longlong
         _LIB = L
     return _LIB
 
@@ -92,6 +98,27 @@ def string_length(xyz, nx, ny, nz, dx, dy, dz, pxy=True, pz=True):
     lib().orc_string_length(pts.ctypes.data if pts.size else None, pts.shape[0], nx, ny, nz, dx, dy, dz,
                             int(pxy), int(pz), out.ctypes.data)
     return float(out[0]), float(out[1]), float(out[2])
+
+
+def sor_profile(nx, h, w, tol, n, lam, eta, g, max_iter, init=None):
+    """singleSemiAnalytic.cpp's SOR, one thread: (profile [nx, 2], sweeps,
+    last residual)."""
+    par = np.array([h, w, tol, n, lam, eta, g], dtype=np.float64)
+    out = np.zeros((nx, 2))
+    res = np.zeros(1)
+    ini = None if init is None else np.ascontiguousarray(init, dtype=np.float64)
+    it = lib().orc_sor_profile(nx, max_iter, par.ctypes.data, None if ini is None else ini.ctypes.data,
+                               out.ctypes.data, res.ctypes.data)
+    return out, it, float(res[0])
+
+
+def straight_string(nx, ny, nz, dx, dy, dz, prof, dr, n, g):
+    """straight_ic.cpp's five ic.txt columns as [5, nx*ny*nz] and the number
+    of sites beyond the profile."""
+    pr = np.ascontiguousarray(prof, dtype=np.float64).reshape(-1, 2)
+    st = np.zeros((5, nx * ny * nz))
+    off = lib().orc_straight_string(nx, ny, nz, dx, dy, dz, pr.ctypes.data, pr.shape[0], dr, n, g, st.ctypes.data)
+    return st, int(off)
 
 
 def diag(p, st, n, trig_mode=0):

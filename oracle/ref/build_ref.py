@@ -69,6 +69,16 @@ CONFIGS = {
                                                  splitLength="true"), False),
     "icw_len_total": ("ic_w_evolution.cpp", dict(nx=12, ny=13, nz=11, nt=1, dx=0.5, dy=0.45, dz=0.4,
                                                  splitLength="false"), False),
+    # straight-string pipeline: SOR radial profile (singleSemiAnalytic.cpp)
+    # as shipped (global string, 30001 points), a gauged one, and one that
+    # ends at its sweep cap; then the lattice interpolation (straight_ic.cpp)
+    # as shipped and on an anisotropic lattice reaching past its profile
+    "sor_ref": ("singleSemiAnalytic.cpp", {}, False),
+    "sor_gauged": ("singleSemiAnalytic.cpp", dict(nx=1501, h=0.02, g=1, **{"lambda": 2}, w=1.6, tol=1e-4), False),
+    "sor_cap": ("singleSemiAnalytic.cpp", dict(nx=801, h=0.03, maxIter=300, g=0.5, n=2), False),
+    "straight_ref": ("straight_ic.cpp", {}, False),
+    "straight_gauged": ("straight_ic.cpp", dict(nx=40, ny=33, nz=3, dx=0.45, dy=0.55, dz=0.5, g=1, SORnx=403,
+                                                SORa=0.02), False),
 }
 
 # extra translation units compiled from their own substituted text

@@ -133,9 +133,84 @@ def run_icw(name):
     print(name, "length", length)
 
 
+# straight-string pipeline: name -> SOR constants (singleSemiAnalytic.cpp:19-32)
+SOR = {
+    "sor_ref": dict(nx=30001, h=0.01, w=1.5, tol=1e-3, n=1.0, lam=1.0, eta=1.0, g=0.0, max_iter=2000000),
+    "sor_gauged": dict(nx=1501, h=0.02, w=1.6, tol=1e-4, n=1.0, lam=2.0, eta=1.0, g=1.0, max_iter=2000000),
+    "sor_cap": dict(nx=801, h=0.03, w=1.5, tol=1e-3, n=2.0, lam=1.0, eta=1.0, g=0.5, max_iter=300),
+}
+# name -> straight_ic.cpp:20-31 constants and the SOR run whose output it reads
+STRAIGHT = {
+    "straight_ref": dict(lattice=[101, 101, 51], spacing=[0.5, 0.5, 0.5], n=1, g=0.5, SORnx=20001, SORa=0.01,
+                         profile="sor_ref"),
+    "straight_gauged": dict(lattice=[40, 33, 3], spacing=[0.45, 0.55, 0.5], n=1, g=1.0, SORnx=403, SORa=0.02,
+                            profile="sor_gauged"),
+}
+
+
+def sampled(a, k, seed=7):
+    """64 sampled rows of a [rows, cols] array: indices and values."""
+    rng = np.random.default_rng(seed)
+    idx = sorted(int(i) for i in rng.choice(a.shape[0], size=min(k, a.shape[0]), replace=False))
+    return idx, a[idx].tolist()
+
+
+def run_sor(name):
+    """singleSemiAnalytic.cpp, one thread, ic_type 0: the profile it writes
+    to Data/SOR_Fields.txt and the sweep count it prints."""
+    dest = os.path.join(REPO, "oracle", "_ref", name)
+    if not os.path.exists(os.path.join(dest, "prog")):
+        build_ref.build(name, "/root/reference")
+    env = dict(os.environ, OMP_NUM_THREADS="1")
+    r = subprocess.run(["./prog"], cwd=dest, text=True, env=env, check=True, capture_output=True)
+    iters = None
+    for part in r.stdout.replace("\r", "\n").splitlines():
+        if part.startswith("Number of iterations:"):
+            iters = int(part.split(":")[1])
+    prof = np.loadtxt(os.path.join(dest, "Data", "SOR_Fields.txt"), dtype=np.float64)
+    idx, rows = sampled(prof, 64)
+    fx = {"name": name, "program": "singleSemiAnalytic.cpp", "consts": SOR[name], "iterations": iters,
+          "profile_sha256": sha(prof), "sample_idx": idx, "sample": rows,
+          "tail": prof[-3:].tolist(), "head": prof[:3].tolist()}
+    with open(os.path.join(HERE, f"{name}.json"), "w") as f:
+        json.dump(fx, f)
+    print(name, "iterations", iters, "sha", fx["profile_sha256"][:12])
+
+
+def run_straight(name):
+    """straight_ic.cpp on the profile of STRAIGHT[name]["profile"] (read at
+    17 digits): its five ic.txt columns and the sites it reports beyond the
+    profile."""
+    spec = STRAIGHT[name]
+    dest = os.path.join(REPO, "oracle", "_ref", name)
+    if not os.path.exists(os.path.join(dest, "prog")):
+        build_ref.build(name, "/root/reference")
+    src = os.path.join(REPO, "oracle", "_ref", spec["profile"], "Data", "SOR_Fields.txt")
+    if not os.path.exists(src):
+        run_sor(spec["profile"])
+    with open(src) as fi, open(os.path.join(dest, "SOR_Fields.txt"), "w") as fo:
+        fo.write(fi.read())
+    r = subprocess.run(["./prog"], cwd=dest, text=True, check=True, capture_output=True)
+    off = sum(1 for line in r.stdout.splitlines() if line.startswith("Off straight string solution grid"))
+    ic = np.loadtxt(os.path.join(dest, "ic.txt"), dtype=np.float64)
+    cols = ic.T.copy()                                   # [5, nx*ny*nz]
+    idx, rows = sampled(ic, 64)
+    fx = {"name": name, "program": "straight_ic.cpp", **spec, "off_grid": off,
+          "ic_sha256": sha(cols), "sample_idx": idx, "sample": rows}
+    with open(os.path.join(HERE, f"{name}.json"), "w") as f:
+        json.dump(fx, f)
+    for fn in ("ic.txt", "SOR_Fields.txt"):
+        os.remove(os.path.join(dest, fn))
+    print(name, "off-grid", off, "sha", fx["ic_sha256"][:12])
+
+
 def run(name):
     if name in ICW:
         return run_icw(name)
+    if name in SOR:
+        return run_sor(name)
+    if name in STRAIGHT:
+        return run_straight(name)
     builder, nt, save, det0, trig = SPECS[name]
     p = builder()
     if p.model == 2:
@@ -195,7 +270,7 @@ def run(name):
 
 
 def main():
-    for n in (sys.argv[1:] or list(SPECS) + list(ICW)):
+    for n in (sys.argv[1:] or list(SPECS) + list(ICW) + list(SOR) + list(STRAIGHT)):
         run(n)
 
 

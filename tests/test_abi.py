@@ -63,3 +63,18 @@ def test_strerror():
     import axions_amd as ax
     assert ax.lib.ax_strerror(-3) == b"HBM budget exceeded"
     assert np.isfinite(ax.random_phases(1, 10)).all()
+
+
+def test_sor_params_and_validation_without_gpu():
+    """singleSemiAnalytic.cpp:19-32 defaults; bad sizes fail before any GPU work."""
+    import axions_amd as ax
+    p = ax.sor_params()
+    assert (p.nx, p.h, p.w, p.max_iter, p.tol, p.n, p.lambda_, p.eta, p.g) == \
+        (30001, 0.01, 1.5, 2000000, 1e-3, 1.0, 1.0, 1.0, 0.0)
+    for bad in (dict(nx=2), dict(nx=10, h=0.0), dict(nx=10, h=-1.0)):
+        try:
+            ax.sor_profile(**bad)
+        except ax.AxError as e:
+            assert e.code == -1
+        else:
+            raise AssertionError(bad)

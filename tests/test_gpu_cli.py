@@ -134,3 +134,42 @@ def test_mpi_evolution_random_ic(tmp_path):
             assert pts.shape[0] == co and np.array_equal(pts, po)
             frame += 1
         orc.step(p, st, t, 1, 0, 16)
+
+
+def test_straight_string_pipeline(tmp_path):
+    """sor -> straight_ic -> evolution, the reference's file chain
+    (Data/SOR_Fields.txt, copied to SOR_Fields.txt, -> ic.txt -> Data/ic.txt):
+    profile and ic.txt equal the compiled reference's (fixtures), and the
+    evolution program runs the string it reads."""
+    d = str(tmp_path)
+    sor = json.load(open(os.path.join(GOLDEN, "sor_gauged.json")))
+    c = sor["consts"]
+    run(["--program", "sor", f"nx={c['nx']}", f"h={c['h']!r}", f"w={c['w']!r}", f"tol={c['tol']!r}",
+         f"lambda={c['lam']!r}", f"g={c['g']!r}", f"n={c['n']!r}"], d)
+    prof = np.loadtxt(os.path.join(d, "Data", "SOR_Fields.txt"))
+    assert sha(prof) == sor["profile_sha256"]
+    # ic_type 1 continues from the file: tol already met -> one more sweep
+    run(["--program", "sor", f"nx={c['nx']}", f"h={c['h']!r}", f"w={c['w']!r}", f"tol={c['tol']!r}",
+         f"lambda={c['lam']!r}", f"g={c['g']!r}", "ic_type=1", "maxIter=1"], d)
+    more = np.loadtxt(os.path.join(d, "Data", "SOR_Fields.txt"))
+    o, it, _ = orc.sor_profile(c["nx"], c["h"], c["w"], c["tol"], c["n"], c["lam"], c["eta"], c["g"], 1, init=prof)
+    assert it == 1 and np.array_equal(more, o)
+    fx = json.load(open(os.path.join(GOLDEN, "straight_gauged.json")))
+    np.savetxt(os.path.join(d, "SOR_Fields.txt"), prof, fmt="%.17g")
+    nx, ny, nz = fx["lattice"]
+    dx, dy, dz = fx["spacing"]
+    run(["--program", "straight_ic", f"nx={nx}", f"ny={ny}", f"nz={nz}", f"dx={dx!r}", f"dy={dy!r}",
+         f"dz={dz!r}", f"g={fx['g']!r}", f"SORnx={fx['SORnx']}", f"SORa={fx['SORa']!r}"], d)
+    ic = np.loadtxt(os.path.join(d, "ic.txt"))
+    assert sha(ic.T.copy()) == fx["ic_sha256"]
+    os.replace(os.path.join(d, "ic.txt"), os.path.join(d, "Data", "ic.txt"))
+    run(["--program", "evolution", "--tag", "s", f"nx={nx}", f"ny={ny}", f"nz={nz}", f"dx={dx!r}", f"dy={dy!r}",
+         f"dz={dz!r}", "dt=0.05", "nt=6", f"g={fx['g']!r}", "lambda=2", "xyBC=fixed", "damped_nt=0",
+         "alpha=0", "saveFreq=5", "finalOut=true"], d)
+    fin = np.loadtxt(os.path.join(d, "Data", "finalField.txt"))
+    p = ah_params(nx, ny=ny, nz=nz, dx=dx, dy=dy, dz=dz, dt=0.05, lam=2.0, g=fx["g"], theta_damp_form=1,
+                  boundary=1)
+    st = np.repeat(ic.T[:, None, :], 2, axis=1).ravel().copy()
+    orc.step(p, st, 0, 6, 0, 16)
+    cur, _ = orc.levels(p, st, 6)
+    assert np.array_equal(fin.T, cur.reshape(5, -1))

@@ -21,8 +21,9 @@ from oracle import oracle_py as orc
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 FIXTURES = sorted(os.path.basename(p)[:-5] for p in glob.glob(os.path.join(GOLDEN, "*.json"))
-                  if not os.path.basename(p).startswith(("global_", "icw_")))
-# global_*: leapfrog.cpp (test_global_oracle.py); icw_*: string length (test_string_length.py)
+                  if not os.path.basename(p).startswith(("global_", "icw_", "sor_", "straight_")))
+# global_*: leapfrog.cpp (test_global_oracle.py); icw_*: string length (test_string_length.py);
+# sor_*, straight_*: straight-string pipeline (test_profile_oracle.py)
 FAST = [f for f in FIXTURES if not f.startswith("evo_c1")]
 
 
