@@ -21,7 +21,9 @@ AX_MODEL_MOORE = 1
 AX_MODEL_GLOBAL = 2
 AX_BC_PERIODIC = 0
 AX_BC_FIXED = 1
-AX_BC_NEUMANN_XY = 2
+AX_BC_NEUMANN = 2
+AX_BC_NEUMANN_XY = AX_BC_NEUMANN
+AX_BC_ABSORBING = 3
 
 
 class AxParams(ctypes.Structure):
@@ -41,6 +43,7 @@ class AxParams(ctypes.Structure):
         ("energy_form", ctypes.c_int32),
         ("boundary", ctypes.c_int32),
         ("precision", ctypes.c_int32),
+        ("boundary_z", ctypes.c_int32),
     ]
 
 
@@ -73,9 +76,11 @@ class AxDiag(ctypes.Structure):
 
 def ah_params(n=64, ny=None, nz=None, dx=0.5, dt=0.1, lam=2.0, eta=1.0, g=1.0, alpha=0.0,
               scaling=1.0, damped_nt=0, dampFac=0.5, theta_damp_form=0, energy_form=0,
-              precision=64, boundary=AX_BC_PERIODIC, dy=None, dz=None):
-    """Abelian-Higgs parameters; defaults are the plain BASELINE set
-    (dx=0.5, dt=0.1, lambda=2, g=eta=1, no friction)."""
+              precision=64, boundary=AX_BC_PERIODIC, dy=None, dz=None, boundary_z=None):
+    """Abelian-Higgs parameters; defaults are the plain set (dx=0.5, dt=0.1,
+    lambda=2, g=eta=1, no friction).  boundary is the reference's xyBC,
+    boundary_z its zBC (default: fixed when boundary is fixed, else
+    periodic)."""
     p = AxParams()
     p.model = AX_MODEL_AH
     p.nx, p.ny, p.nz = n, (ny or n), (nz or n)
@@ -92,13 +97,17 @@ def ah_params(n=64, ny=None, nz=None, dx=0.5, dt=0.1, lam=2.0, eta=1.0, g=1.0, a
     p.damped_nt, p.dampFac = damped_nt, dampFac
     p.theta_damp_form, p.energy_form = theta_damp_form, energy_form
     p.boundary, p.precision = boundary, precision
+    if boundary_z is None:
+        boundary_z = AX_BC_FIXED if boundary == AX_BC_FIXED else AX_BC_PERIODIC
+    p.boundary_z = boundary_z
     return p
 
 
 def moore_params(n=64, ny=None, nz=None, dx=0.7, dt=0.3, lambda1=2.0, lambda2=2.0, lambda12=0.0,
                  eta1=1.0, eta2=1.0, q1=2, q2=1, g=None, boundary=AX_BC_PERIODIC, precision=64):
     """Moore-model parameters at mooreEvol.cpp:23-36 defaults;
-    g = sqrt(0.5*lambda1*eta1^2/(q1^2+q2^2)) as mooreEvol.cpp:36."""
+    g = sqrt(0.5*lambda1*eta1^2/(q1^2+q2^2)) as mooreEvol.cpp:36.  z is
+    periodic with periodic x/y, else fixed (mooreEvol.cpp:187-189)."""
     p = AxParams()
     p.model = AX_MODEL_MOORE
     p.nx, p.ny, p.nz = n, (ny or n), (nz or n)
@@ -112,6 +121,7 @@ def moore_params(n=64, ny=None, nz=None, dx=0.7, dt=0.3, lambda1=2.0, lambda2=2.
     p.alpha, p.scaling, p.damped_nt, p.dampFac = 0.0, 1.0, 0, 0.0
     p.theta_damp_form, p.energy_form = 0, 0
     p.boundary, p.precision = boundary, precision
+    p.boundary_z = AX_BC_PERIODIC if boundary == AX_BC_PERIODIC else AX_BC_FIXED
     return p
 
 
@@ -128,6 +138,7 @@ def global_params(n=64, ny=None, nz=None, dx=0.25, dt=0.1, dy=None, dz=None, bou
     p.lambda_, p.eta, p.g = 2.0, 1.0, 0.0
     p.scaling = 1.0
     p.boundary, p.precision = boundary, precision
+    p.boundary_z = AX_BC_PERIODIC
     return p
 
 

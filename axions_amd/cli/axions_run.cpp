@@ -41,8 +41,8 @@
  * when absent), --digits the significant digits written (default 6, the
  * reference's ostream default; 17 round-trips doubles).
  *
- * Differences, all deliberate: periodic boundaries for the gauged programs
- * unless xyBC=fixed (mooreEvol.cpp's default); the string-length column(s)
+ * Differences, all deliberate: mpi_evolution is periodic (the MPI program
+ * has no other boundary); the string-length column(s)
  * of evolution.cpp (uninitialised there: its chaining, evolution.cpp:1229-1330,
  * is commented out) hold the chained length of ic_w_evolution.cpp:961-1087
  * (total, or inner/outer with splitLength) via ax_string_length; all gauged
@@ -97,7 +97,8 @@ std::map<std::string, std::string> defaults(const std::string &prog) {
             {"dz", "0.5"}, {"dt", "0.05"}, {"lambda", "1"}, {"eta", "1"}, {"g", "0"}, {"damped_nt", "200"},
             {"dampFac", "0.5"}, {"alpha", "3"}, {"scaling", "1"}, {"makeStringPosGif", "true"},
             {"saveFreq", "5"}, {"stationary_ic", "true"}, {"stringDetect", "true"}, {"detectBuffer", "true"},
-            {"finalOut", "false"}, {"xyBC", "periodic"}, {"splitLength", "false"}, {"precision", "64"}};
+            {"finalOut", "false"}, {"xyBC", "periodic"}, {"zBC", "periodic"}, {"splitLength", "false"},
+            {"precision", "64"}};
   if (prog == "mpi_evolution")   /* mpi_evolution.cpp:17-60 */
     return {{"nx", "1001"}, {"ny", "1001"}, {"nz", "1001"}, {"nt", "2501"}, {"dx", "0.25"}, {"dy", "0.25"},
             {"dz", "0.25"}, {"dt", "0.05"}, {"lambda", "1"}, {"eta", "1"}, {"g", "0"}, {"damped_nt", "200"},
@@ -257,7 +258,12 @@ struct Run {
     p.scaling = C.d("scaling");
     p.theta_damp_form = 1;   /* evolution.cpp:371-373 */
     p.energy_form = 0;       /* evolution.cpp:403-427 */
-    p.boundary = C.s("xyBC") == "periodic" ? AX_BC_PERIODIC : AX_BC_FIXED;
+    /* evolution.cpp:52-53: "periodic", "neumann", "absorbing" (x, y only);
+     * any other string is fixed */
+    const std::string xy = C.s("xyBC"), zb = C.s("zBC");
+    p.boundary = xy == "periodic" ? AX_BC_PERIODIC : xy == "neumann" ? AX_BC_NEUMANN
+                                                   : xy == "absorbing" ? AX_BC_ABSORBING : AX_BC_FIXED;
+    p.boundary_z = zb == "periodic" ? AX_BC_PERIODIC : zb == "neumann" ? AX_BC_NEUMANN : AX_BC_FIXED;
     const int nt = C.i("nt"), save = C.i("saveFreq");
     const bool detect = C.b("stringDetect"), buffer = C.b("detectBuffer"), gif = C.b("makeStringPosGif");
     const bool stationary = C.b("stationary_ic"), fin = C.b("finalOut"), split = C.b("splitLength");
@@ -281,7 +287,7 @@ struct Run {
         const std::vector<double> xyz = crossings();
         if (gif && n % save == 0) frame(D("GifData/gifStringPosData_" + std::to_string(frameNo++) + ".txt"), xyz);
         check(ax_string_length(xyz.data(), xyz.size() / 3, p.nx, p.ny, p.nz, p.dx, p.dy, p.dz,
-                               p.boundary == AX_BC_PERIODIC, p.boundary == AX_BC_PERIODIC, len), "ax_string_length");
+                               p.boundary == AX_BC_PERIODIC, p.boundary_z == AX_BC_PERIODIC, len), "ax_string_length");
       }
       vals.num(d.energy, false);
       vals.num(d.gauss_mean, !detect);
@@ -346,7 +352,12 @@ struct Run {
     const std::string gs = C.s("g");
     /* mooreEvol.cpp:36: g = sqrt(0.5*lambda1*pow(eta1,2)/(q1*q1 + q2*q2)) */
     p.g = gs.empty() ? sqrt(0.5 * p.lambda1 * (p.eta1 * p.eta1) / (p.q1 * p.q1 + p.q2 * p.q2)) : strtod(gs.c_str(), nullptr);
-    p.boundary = C.s("xyBC") == "periodic" ? AX_BC_PERIODIC : AX_BC_FIXED;
+    /* mooreEvol.cpp:42: "neumann" copies the x/y faces, z stays fixed;
+     * absorbing is not implemented there for this model */
+    const std::string xy = C.s("xyBC");
+    if (xy == "absorbing") die("moore: xyBC=absorbing is not defined for the Moore model (mooreEvol.cpp:42)");
+    p.boundary = xy == "periodic" ? AX_BC_PERIODIC : xy == "neumann" ? AX_BC_NEUMANN : AX_BC_FIXED;
+    p.boundary_z = xy == "periodic" ? AX_BC_PERIODIC : AX_BC_FIXED;
     const int nt = C.i("nt"), save = C.i("saveFreq");
     const bool gif = C.b("makeGif");
     check(ax_create(&p, &lat), "ax_create");
@@ -359,7 +370,7 @@ struct Run {
       check(ax_diagnostics(lat, 0, &d), "ax_diagnostics");
       vals.num(d.energy, false);
       /* mooreEvol.cpp:697 divides the per-site mean once more */
-      vals.num(p.boundary == AX_BC_FIXED ? d.gauss_mean / inner : d.gauss_mean, true);
+      vals.num(p.boundary != AX_BC_PERIODIC ? d.gauss_mean / inner : d.gauss_mean, true);
       check(ax_step(lat, 1), "ax_step");
       if (gif && n % save == 0) {
         const std::vector<double> f = newest(7);
@@ -412,6 +423,7 @@ struct Run {
     V = (size_t)p.nx * p.ny * p.nz;
     p.g = C.d("g");
     p.boundary = AX_BC_FIXED;
+    p.boundary_z = AX_BC_FIXED;
     const int sn = C.i("SORnx");
     const std::vector<double> prof = read_numbers(D("SOR_Fields.txt"), 2 * (size_t)sn);
     check(ax_create(&p, &lat), "ax_create");

@@ -53,9 +53,19 @@ int check_params(const ax_params *p) {
     return fail(AX_EINVAL, "lattice %dx%dx%d exceeds 2^30 sites", p->nx, p->ny, p->nz);
   if (!(p->dx > 0 && p->dy > 0 && p->dz > 0 && p->dt > 0)) return fail(AX_EINVAL, "spacings must be > 0");
   if (p->precision != 64 && p->precision != 32) return fail(AX_EINVAL, "precision %d (64 or 32)", p->precision);
-  if (p->model == AX_MODEL_GLOBAL ? (p->boundary != AX_BC_PERIODIC && p->boundary != AX_BC_NEUMANN_XY)
-                                   : (p->boundary != AX_BC_PERIODIC && p->boundary != AX_BC_FIXED))
-    return fail(AX_EINVAL, "boundary %d for model %d", p->boundary, p->model);
+  {
+    const int b = p->boundary, bz = p->boundary_z;
+    bool ok;
+    if (p->model == AX_MODEL_GLOBAL)
+      ok = (b == AX_BC_PERIODIC || b == AX_BC_NEUMANN) && bz == AX_BC_PERIODIC;
+    else if (p->model == AX_MODEL_MOORE)
+      ok = (b == AX_BC_PERIODIC && bz == AX_BC_PERIODIC) || ((b == AX_BC_FIXED || b == AX_BC_NEUMANN) && bz == AX_BC_FIXED);
+    else
+      ok = b >= AX_BC_PERIODIC && b <= AX_BC_ABSORBING && bz >= AX_BC_PERIODIC && bz <= AX_BC_NEUMANN;
+    if (!ok) return fail(AX_EINVAL, "boundary (%d, z %d) for model %d", b, bz, p->model);
+    if (p->model == AX_MODEL_AH && bz == AX_BC_NEUMANN && p->nz > p->nx)
+      return fail(AX_EINVAL, "Neumann z faces need nz <= nx (the reference's z-face loop, evolution.cpp:237)");
+  }
   if (p->theta_damp_form != 0 && p->theta_damp_form != 1) return fail(AX_EINVAL, "theta_damp_form");
   if (p->energy_form != 0 && p->energy_form != 1) return fail(AX_EINVAL, "energy_form");
   return AX_OK;
@@ -66,7 +76,8 @@ StepConsts step_consts(const ax_params *p, int n) { return make_step_consts(*p, 
 DiagConsts diag_consts(const ax_params *p) {
   DiagConsts k;
   k.nx = p->nx; k.ny = p->ny; k.nz = p->nz;
-  k.edge = p->boundary == AX_BC_FIXED;
+  k.ex = p->boundary != AX_BC_PERIODIC;
+  k.ez = p->boundary_z != AX_BC_PERIODIC;
   k.model = p->model;
   k.energy_form = p->energy_form;
   k.ncomp = p->model == AX_MODEL_AH ? 5 : 7;
@@ -147,7 +158,7 @@ void ax_default_params(ax_params *p) {
   p->q1 = 2; p->q2 = 1;
   p->alpha = 0; p->scaling = 1; p->damped_nt = 0; p->dampFac = 0.5;
   p->theta_damp_form = 0; p->energy_form = 0;
-  p->boundary = AX_BC_PERIODIC; p->precision = 64;
+  p->boundary = AX_BC_PERIODIC; p->precision = 64; p->boundary_z = AX_BC_PERIODIC;
 }
 
 const char *ax_strerror(int code) {
@@ -459,6 +470,8 @@ int ax_diagnostics(ax_lattice *L, int32_t want_winding, ax_diag *out) {
       e = hipMemsetAsync(L->dres, 0, 3 * sizeof(double), L->stream);
   } else {
     DiagConsts k = diag_consts(&p);
+    int rc = prepare_faces(L);
+    if (rc) return rc;
     if (L->esize == 8)
       e = launch_diag<double>(k, levels_of<double>(L, L->timestep), L->part, L->dres, L->stream);
     else
@@ -478,9 +491,10 @@ int ax_diagnostics(ax_lattice *L, int32_t want_winding, ax_diag *out) {
   out->timestep = L->timestep;
   out->energy = (glob && L->timestep == 0) ? NAN : L->hres[0];
   double dev = L->hres[1];
-  if (p.boundary == AX_BC_PERIODIC) {
-    dev = dev / (p.nx * p.ny);                             /* evolution.cpp:1404-1408 */
-    dev = dev / p.nz;
+  if (p.model != AX_MODEL_MOORE || p.boundary == AX_BC_PERIODIC) {
+    /* evolution.cpp:1404-1408 (Moore with fixed edges divides per site) */
+    dev = p.boundary == AX_BC_PERIODIC ? dev / (p.nx * p.ny) : dev / ((p.nx - 2) * (p.ny - 2));
+    dev = p.boundary_z == AX_BC_PERIODIC ? dev / p.nz : dev / (p.nz - 2);
   }
   out->gauss_mean = ((p.model == AX_MODEL_AH && p.energy_form == 1) || glob) ? 0.0 : dev;
   out->finite = L->hres[2] == 0 ? 1 : 0;
@@ -531,15 +545,19 @@ int ax_crossings(ax_lattice *L, double *xyz, size_t cap, size_t *count) {
     }
     L->cxyz_cap = cap;
   }
-  const int tN = p.model == AX_MODEL_GLOBAL ? L->timestep % 2 : (L->timestep + 1) % 2;
+  const bool glob = p.model == AX_MODEL_GLOBAL;
+  const int tN = glob ? L->timestep % 2 : (L->timestep + 1) % 2;
+  const int ex = !glob && p.boundary != AX_BC_PERIODIC, ez = !glob && p.boundary_z != AX_BC_PERIODIC;
+  int rc = prepare_faces(L);
+  if (rc) return rc;
   if (L->esize == 8) {
     const double *b = (const double *)L->state;
     e = launch_crossings<double>(p.nx, p.ny, p.nz, p.dx, p.dy, p.dz, b + (size_t)tN * L->pitch, b + (size_t)(2 + tN) * L->pitch,
-                                 L->bcount, L->boff, L->ctotal, L->cxyz, (long long)cap, L->stream, cap > 0);
+                                 ex, ez, L->bcount, L->boff, L->ctotal, L->cxyz, (long long)cap, L->stream, cap > 0);
   } else {
     const float *b = (const float *)L->state;
     e = launch_crossings<float>(p.nx, p.ny, p.nz, p.dx, p.dy, p.dz, b + (size_t)tN * L->pitch, b + (size_t)(2 + tN) * L->pitch,
-                                L->bcount, L->boff, L->ctotal, L->cxyz, (long long)cap, L->stream, cap > 0);
+                                ex, ez, L->bcount, L->boff, L->ctotal, L->cxyz, (long long)cap, L->stream, cap > 0);
   }
   if (e != hipSuccess) return hip_fail(e, "crossing launch");
   long long tot = 0;
@@ -559,6 +577,19 @@ int ax_crossings(ax_lattice *L, double *xyz, size_t cap, size_t *count) {
 }  // extern "C"
 
 namespace ax {
+
+/* Neumann faces of the current level, as the coming TimeStep sets them
+ * before its site loop (evolution.cpp:192-265), so that energy, crossings and
+ * windings see what the reference's loop sees.  Idempotent: the step's own
+ * copy rewrites the same values. */
+int prepare_faces(ax_lattice *L) {
+  const ax_params &p = L->p;
+  if (p.model == AX_MODEL_GLOBAL || (p.boundary != AX_BC_NEUMANN && p.boundary_z != AX_BC_NEUMANN)) return AX_OK;
+  const int tN = (L->timestep + 1) % 2;
+  hipError_t e = L->esize == 8 ? launch_neumann_faces<double>(p, L->state, L->pitch, tN, L->stream)
+                               : launch_neumann_faces<float>(p, L->state, L->pitch, tN, L->stream);
+  return e == hipSuccess ? AX_OK : hip_fail(e, "Neumann faces launch");
+}
 
 int ensure_winding_buffer(ax_lattice *L) {
   if (L->wind) return AX_OK;
@@ -580,14 +611,17 @@ int launch_winding_field(ax_lattice *L, int field, int8_t *out, double *part, do
   const double q = p.model == AX_MODEL_AH ? 1.0 : glob ? 0.0 : (double)(field == 0 ? p.q1 : p.q2);
   hipError_t e;
   auto P = [&](int c) { return glob && c >= 2 ? (size_t)4 * L->pitch : (size_t)(c * 2 + tN) * L->pitch; };
+  const int ex = !glob && p.boundary != AX_BC_PERIODIC, ez = !glob && p.boundary_z != AX_BC_PERIODIC;
+  int rc = prepare_faces(L);
+  if (rc) return rc;
   if (L->esize == 8) {
     const double *b = (const double *)L->state;
     e = launch_winding<double>(p.nx, p.ny, p.nz, b + P(c0), b + P(c0 + 1), b + P(t0), b + P(t0 + 1), b + P(t0 + 2), q,
-                               out, part, res3, L->stream);
+                               ex, ez, out, part, res3, L->stream);
   } else {
     const float *b = (const float *)L->state;
     e = launch_winding<float>(p.nx, p.ny, p.nz, b + P(c0), b + P(c0 + 1), b + P(t0), b + P(t0 + 1), b + P(t0 + 2), q,
-                              out, part, res3, L->stream);
+                              ex, ez, out, part, res3, L->stream);
   }
   return e == hipSuccess ? AX_OK : hip_fail(e, "winding launch");
 }
@@ -601,6 +635,17 @@ int launch_step(ax_lattice *L, int n) {
   }
   StepConsts k = step_consts(&L->p, n);
   const bool tiled = use_tiled_ah(L->p), moore = L->p.model == AX_MODEL_MOORE;
+  const int tN = (n + 1) % 2, tP = n % 2;
+  if (L->p.boundary == AX_BC_NEUMANN || L->p.boundary_z == AX_BC_NEUMANN) {
+    e = L->esize == 8 ? launch_neumann_faces<double>(L->p, L->state, L->pitch, tN, L->stream)
+                      : launch_neumann_faces<float>(L->p, L->state, L->pitch, tN, L->stream);
+    if (e != hipSuccess) return hip_fail(e, "Neumann faces launch");
+  }
+  if (L->p.boundary == AX_BC_ABSORBING) {
+    e = L->esize == 8 ? launch_absorbing<double>(k, L->p, L->state, L->pitch, tN, tP, L->stream)
+                      : launch_absorbing<float>(k, L->p, L->state, L->pitch, tN, tP, L->stream);
+    if (e != hipSuccess) return hip_fail(e, "absorbing boundary launch");
+  }
   if (L->esize == 8) {
     Levels<double> lv = levels_of<double>(L, n);
     e = !tiled ? launch_step_site<double>(k, lv, L->p.model, L->stream)

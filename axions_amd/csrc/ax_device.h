@@ -24,7 +24,7 @@ namespace ax {
  * device expression using them rounds exactly like the reference's. */
 struct StepConsts {
   int nx, ny, nz;
-  int edge;              /* 0 periodic, 1 fixed edges (sites on the boundary frozen) */
+  int ex, ez;            /* 1: x,y (z) not periodic -- the bulk update skips those faces */
   double dx2, dy2, dz2;  /* dx*dx ... */
   double dt, dt2;        /* dt, dt*dt */
   double hl, eta2;       /* 0.5*lambda, eta*eta */
@@ -42,7 +42,8 @@ struct StepConsts {
 __host__ __device__ inline StepConsts make_step_consts(const ax_params &p, int n) {
   StepConsts k;
   k.nx = p.nx; k.ny = p.ny; k.nz = p.nz;
-  k.edge = p.boundary == AX_BC_FIXED;
+  k.ex = p.boundary != AX_BC_PERIODIC;
+  k.ez = p.boundary_z != AX_BC_PERIODIC;
   k.dx2 = p.dx * p.dx; k.dy2 = p.dy * p.dy; k.dz2 = p.dz * p.dz;
   k.dt = p.dt; k.dt2 = p.dt * p.dt;
   k.hl = 0.5 * p.lambda; k.eta2 = p.eta * p.eta;

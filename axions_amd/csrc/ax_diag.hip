@@ -164,7 +164,7 @@ __device__ __forceinline__ void site_energy_moore(const DiagConsts &k, const Lev
              gy[1][0] * gy[1][0] + gy[1][1] * gy[1][1] + gz[1][0] * gz[1][0] + gz[1][1] * gz[1][1] + tdc +
              fcont + 0.25 * k.lambda1 * (w1 * w1) + 0.25 * k.lambda2 * (w2 * w2) + 0.5 * k.lambda12 * w1 * w2;
   rho = dx * dy * dz * r;
-  absG = k.edge ? fabs(G) / k.inner : fabs(G);
+  absG = k.ex ? fabs(G) / k.inner : fabs(G);
 }
 
 /* Sites per thread of the winding pass (each thread takes sites 256 apart,
@@ -183,7 +183,8 @@ __global__ void __launch_bounds__(256) diag_kernel(DiagConsts k, Levels<T> L, in
     const int s = blockIdx.x * kDiagSitesPerBlock + q * 256 + threadIdx.x;
     if (s >= V) break;
     const int kk = s % k.nz, r = s / k.nz, j = r % k.ny, i = r / k.ny;
-    const bool inner = !k.edge || (i > 0 && i < k.nx - 1 && j > 0 && j < k.ny - 1 && kk > 0 && kk < k.nz - 1);
+    const bool inner = !((k.ex && (i == 0 || i == k.nx - 1 || j == 0 || j == k.ny - 1)) ||
+                         (k.ez && (kk == 0 || kk == k.nz - 1)));
     if (inner) {
       double r1 = 0, g1 = 0;
       if (MODEL == AX_MODEL_AH)
@@ -232,7 +233,7 @@ __global__ void __launch_bounds__(1024) sum3_kernel(const double *part, int nb, 
 template <typename T>
 __global__ void __launch_bounds__(256) winding_kernel(int nx, int ny, int nz, const T *fr, const T *fi,
                                                       const T *tx, const T *ty, const T *tz, double q,
-                                                      int8_t *out, double *part) {
+                                                      int ex, int ez, int8_t *out, double *part) {
   __shared__ double sh[4];
   const int V = nx * ny * nz;
   double tot[3] = {0, 0, 0};
@@ -242,6 +243,11 @@ __global__ void __launch_bounds__(256) winding_kernel(int nx, int ny, int nz, co
     const int kk = s % nz, r = s / nz, j = r % ny, i = r / ny;
     int w[3];
     site_winding<T>(nx, ny, nz, fr, fi, tx, ty, tz, q, i, j, kk, w);
+    /* plaquettes that would wrap across a non-periodic axis do not exist */
+    const bool xe = ex && i == nx - 1, ye = ex && j == ny - 1, ze = ez && kk == nz - 1;
+    if (xe || ye) w[0] = 0;
+    if (xe || ze) w[1] = 0;
+    if (ye || ze) w[2] = 0;
 #pragma unroll
     for (int pl = 0; pl < 3; ++pl) {
       out[(size_t)pl * V + s] = (int8_t)w[pl];
@@ -304,8 +310,10 @@ __device__ __forceinline__ int face_roots(const double v00[2], const double v10[
 /* All crossings of site s, in the reference's order; returns count (<= 6). */
 template <typename T>
 __device__ int site_crossings(int nx, int ny, int nz, double dx, double dy, double dz, const T *fr, const T *fi,
-                              int s, double pts[18]) {
+                              int ex, int ez, int s, double pts[18]) {
   const int kk = s % nz, r = s / nz, j = r % ny, i = r / ny;
+  /* only the sites the bulk update covers (evolution.cpp:731-743) */
+  if ((ex && (i == 0 || i == nx - 1 || j == 0 || j == ny - 1)) || (ez && (kk == 0 || kk == nz - 1))) return 0;
   const int ip = wrapp(i, nx), jp = wrapp(j, ny), kp = wrapp(kk, nz);
   auto id = [&](int a, int b, int c) { return (a * ny + b) * nz + c; };
   const int sx = id(ip, j, kk), sy = id(i, jp, kk), sz = id(i, j, kp);
@@ -330,14 +338,14 @@ __device__ int site_crossings(int nx, int ny, int nz, double dx, double dy, doub
 /* pass 1: per-block crossing counts */
 template <typename T>
 __global__ void __launch_bounds__(256) cross_count_kernel(int nx, int ny, int nz, double dx, double dy, double dz,
-                                                          const T *fr, const T *fi, int *bcount) {
+                                                          const T *fr, const T *fi, int ex, int ez, int *bcount) {
   __shared__ int sh[4];
   const int V = nx * ny * nz;
   const int s = blockIdx.x * 256 + threadIdx.x;
   int n = 0;
   if (s < V) {
     double pts[18];
-    n = site_crossings<T>(nx, ny, nz, dx, dy, dz, fr, fi, s, pts);
+    n = site_crossings<T>(nx, ny, nz, dx, dy, dz, fr, fi, ex, ez, s, pts);
   }
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) n += __shfl_xor(n, off, 64);
@@ -373,14 +381,14 @@ __global__ void __launch_bounds__(1024) scan_kernel(const int *in, long long *ou
 /* pass 3: recompute and write points at their global offsets. */
 template <typename T>
 __global__ void __launch_bounds__(256) cross_write_kernel(int nx, int ny, int nz, double dx, double dy, double dz,
-                                                          const T *fr, const T *fi, const long long *boff,
-                                                          double *xyz, long long cap) {
+                                                          const T *fr, const T *fi, int ex, int ez,
+                                                          const long long *boff, double *xyz, long long cap) {
   __shared__ int sh[256];
   const int V = nx * ny * nz;
   const int s = blockIdx.x * 256 + threadIdx.x;
   double pts[18];
   int n = 0;
-  if (s < V) n = site_crossings<T>(nx, ny, nz, dx, dy, dz, fr, fi, s, pts);
+  if (s < V) n = site_crossings<T>(nx, ny, nz, dx, dy, dz, fr, fi, ex, ez, s, pts);
   sh[threadIdx.x] = n;
   __syncthreads();
   for (int off = 1; off < 256; off <<= 1) {
@@ -414,36 +422,38 @@ hipError_t launch_diag(const DiagConsts &k, const Levels<T> &L, double *part, do
 
 template <typename T>
 hipError_t launch_winding(int nx, int ny, int nz, const T *fr, const T *fi, const T *tx, const T *ty,
-                          const T *tz, double q, int8_t *out, double *part, double *out1, hipStream_t st) {
+                          const T *tz, double q, int ex, int ez, int8_t *out, double *part, double *out1,
+                          hipStream_t st) {
   const int V = nx * ny * nz;
   const int nb = (V + kSitesPerBlock - 1) / kSitesPerBlock;
-  hipLaunchKernelGGL((winding_kernel<T>), dim3(nb), dim3(256), 0, st, nx, ny, nz, fr, fi, tx, ty, tz, q, out,
-                     part);
+  hipLaunchKernelGGL((winding_kernel<T>), dim3(nb), dim3(256), 0, st, nx, ny, nz, fr, fi, tx, ty, tz, q, ex, ez,
+                     out, part);
   hipLaunchKernelGGL(sum3_kernel, dim3(1), dim3(1024), 0, st, part, nb, out1);
   return hipGetLastError();
 }
 
 template <typename T>
 hipError_t launch_crossings(int nx, int ny, int nz, double dx, double dy, double dz, const T *fr, const T *fi,
-                            int *bcount, long long *boff, long long *total, double *xyz, long long cap,
-                            hipStream_t st, bool write) {
+                            int ex, int ez, int *bcount, long long *boff, long long *total, double *xyz,
+                            long long cap, hipStream_t st, bool write) {
   const int V = nx * ny * nz;
   const int nb = (V + 255) / 256;
-  hipLaunchKernelGGL((cross_count_kernel<T>), dim3(nb), dim3(256), 0, st, nx, ny, nz, dx, dy, dz, fr, fi,
+  hipLaunchKernelGGL((cross_count_kernel<T>), dim3(nb), dim3(256), 0, st, nx, ny, nz, dx, dy, dz, fr, fi, ex, ez,
                      bcount);
   hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(1024), 0, st, bcount, boff, nb, total);
   if (write)
     hipLaunchKernelGGL((cross_write_kernel<T>), dim3(nb), dim3(256), 0, st, nx, ny, nz, dx, dy, dz, fr, fi,
-                       boff, xyz, cap);
+                       ex, ez, boff, xyz, cap);
   return hipGetLastError();
 }
 
 #define AX_INST(T)                                                                                          \
   template hipError_t launch_diag<T>(const DiagConsts &, const Levels<T> &, double *, double *, hipStream_t); \
   template hipError_t launch_winding<T>(int, int, int, const T *, const T *, const T *, const T *, const T *,  \
-                                        double, int8_t *, double *, double *, hipStream_t);                  \
-  template hipError_t launch_crossings<T>(int, int, int, double, double, double, const T *, const T *, int *, \
-                                          long long *, long long *, double *, long long, hipStream_t, bool);
+                                        double, int, int, int8_t *, double *, double *, hipStream_t);        \
+  template hipError_t launch_crossings<T>(int, int, int, double, double, double, const T *, const T *, int,   \
+                                          int, int *, long long *, long long *, double *, long long,          \
+                                          hipStream_t, bool);
 AX_INST(double)
 AX_INST(float)
 #undef AX_INST

@@ -13,7 +13,7 @@ struct ax_lattice;
 namespace ax {
 
 struct DiagConsts {
-  int nx, ny, nz, edge, model, energy_form, ncomp;
+  int nx, ny, nz, ex, ez, model, energy_form, ncomp;
   double dx, dy, dz, dt, g, lambda, eta;
   double lambda1, lambda2, lambda12, eta1, eta2, q1, q2;
   double inner; /* (nx-2)(ny-2)(nz-2) as a double */
@@ -36,6 +36,7 @@ void lattice_reset_timestep(ax_lattice *L);
 
 int launch_step(ax_lattice *L, int n);
 int ensure_winding_buffer(ax_lattice *L);
+int prepare_faces(ax_lattice *L);
 int launch_winding_field(ax_lattice *L, int field, int8_t *out, double *part, double *res3);
 
 template <typename R>
@@ -57,11 +58,22 @@ template <typename T>
 hipError_t launch_diag(const DiagConsts &k, const Levels<T> &L, double *part, double *out3, hipStream_t st);
 template <typename T>
 hipError_t launch_winding(int nx, int ny, int nz, const T *fr, const T *fi, const T *tx, const T *ty, const T *tz,
-                          double q, int8_t *out, double *part, double *out3, hipStream_t st);
+                          double q, int ex, int ez, int8_t *out, double *part, double *out3, hipStream_t st);
 template <typename T>
 hipError_t launch_crossings(int nx, int ny, int nz, double dx, double dy, double dz, const T *fr, const T *fi,
-                            int *bcount, long long *boff, long long *total, double *xyz, long long cap,
-                            hipStream_t st, bool write);
+                            int ex, int ez, int *bcount, long long *boff, long long *total, double *xyz,
+                            long long cap, hipStream_t st, bool write);
+
+/* Boundary work of one TimeStep on a lattice that is not periodic
+ * (ax_bound.hip).  launch_neumann_faces sets the faces of level tN from
+ * their inner neighbours; launch_absorbing steps the x/y faces and corners
+ * and must run before the bulk update, which overwrites the level-tP values
+ * they read. */
+template <typename R>
+hipError_t launch_neumann_faces(const ax_params &p, void *state, size_t pitch, int tN, hipStream_t st);
+template <typename R>
+hipError_t launch_absorbing(const StepConsts &k, const ax_params &p, void *state, size_t pitch, int tN, int tP,
+                            hipStream_t st);
 
 }  // namespace ax
 

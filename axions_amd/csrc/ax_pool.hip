@@ -734,7 +734,7 @@ void ax_pool_default_config(ax_pool_config *c) {
 int ax_pool_create(const ax_params *tmpl, const ax_pool_config *cfg_in, ax_pool **out) {
   if (!tmpl || !out) return fail(AX_EINVAL, "NULL argument");
   *out = nullptr;
-  if (tmpl->model != AX_MODEL_AH || tmpl->boundary != AX_BC_PERIODIC)
+  if (tmpl->model != AX_MODEL_AH || tmpl->boundary != AX_BC_PERIODIC || tmpl->boundary_z != AX_BC_PERIODIC)
     return fail(AX_EINVAL, "the pool runs periodic Abelian-Higgs lattices");
   if (tmpl->precision != 64 && tmpl->precision != 32) return fail(AX_EINVAL, "precision %d", tmpl->precision);
   if (!(tmpl->dx > 0 && tmpl->dy > 0 && tmpl->dz > 0 && tmpl->dt > 0)) return fail(AX_EINVAL, "spacings must be > 0");

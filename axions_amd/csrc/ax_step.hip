@@ -46,7 +46,7 @@ __device__ __forceinline__ bool site_of(const StepConsts &k, int s, Site &o, int
   int r = s / nz;
   j = r % ny;
   i = r / ny;
-  if (k.edge && (i == 0 || i == nx - 1 || j == 0 || j == ny - 1 || kk == 0 || kk == nz - 1))
+  if ((k.ex && (i == 0 || i == nx - 1 || j == 0 || j == ny - 1)) || (k.ez && (kk == 0 || kk == nz - 1)))
     return false;
   const int ip = wrapp(i, nx), im = wrapm(i, nx), jp = wrapp(j, ny), jm = wrapm(j, ny);
   const int kp = wrapp(kk, nz), km = wrapm(kk, nz);

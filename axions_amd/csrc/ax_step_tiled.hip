@@ -67,6 +67,7 @@ template hipError_t launch_step_tiled_moore<float>(const StepConsts &, const Lev
 bool use_tiled_ah(const ax_params &p) {
   const int tail = p.nz % 64;
   return (p.model == AX_MODEL_AH || p.model == AX_MODEL_MOORE) && p.boundary == AX_BC_PERIODIC &&
+         p.boundary_z == AX_BC_PERIODIC &&
          (tail == 0 || tail >= 32) &&
          p.nx >= 3 && p.ny >= 3 &&
          (double)p.nx * p.ny * p.nz * (p.precision == 64 ? 8.0 : 4.0) < 4294967296.0 &&

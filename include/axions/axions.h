@@ -47,14 +47,37 @@ extern "C" {
 #define AX_MODEL_GLOBAL 2  /* one complex scalar, no gauge field, momentum (pi)
                               form of the leapfrog (leapfrog.cpp:200-347) */
 
-#define AX_BC_PERIODIC 0   /* periodic in x, y, z (mpi_evolution.cpp:385-405) */
-#define AX_BC_FIXED 1      /* edge sites never updated (mooreEvol.cpp:187-189) */
-#define AX_BC_NEUMANN_XY 2 /* global model: x and y faces copied from their inner
-                              neighbours each step, z periodic (leapfrog.cpp:168-193) */
+/* Boundary conditions, per axis group as the reference's xyBC and zBC
+ * (evolution.cpp:52-53): ax_params.boundary for x and y, boundary_z for z.
+ * Only periodic axes wrap; on the others the faces are not stepped by the
+ * bulk update, and the energy, Gauss sum and crossings cover the sites the
+ * bulk update covers (evolution.cpp:161-186).
+ *   AX_BC_PERIODIC   periodic (mpi_evolution.cpp:385-405)
+ *   AX_BC_FIXED      face sites never updated (evolution.cpp "fixed",
+ *                    mooreEvol.cpp:187-189)
+ *   AX_BC_NEUMANN    before each step the faces of the current level are
+ *                    set from their inner neighbours, parallel-transported
+ *                    along the link: y faces, then x faces (all y), then z
+ *                    faces (evolution.cpp:192-265; Moore with q-scaled links,
+ *                    mooreEvol.cpp:114-168).  Global model: plain copies of
+ *                    phi and pi, z periodic (leapfrog.cpp:168-193).
+ *   AX_BC_ABSORBING  x and y only (AH): the faces and corners step with the
+ *                    one-way wave equations of evolution.cpp:449-689.
+ * Supported pairs (boundary, boundary_z): AH any x/y mode with z periodic,
+ * fixed or Neumann (z Neumann needs nz <= nx: the reference's z-face loop
+ * runs its x index up to nz, evolution.cpp:237); Moore (periodic, periodic),
+ * (fixed, fixed), (Neumann, fixed); global (periodic, periodic),
+ * (Neumann, periodic). */
+#define AX_BC_PERIODIC 0
+#define AX_BC_FIXED 1
+#define AX_BC_NEUMANN 2
+#define AX_BC_NEUMANN_XY AX_BC_NEUMANN
+#define AX_BC_ABSORBING 3
 
 /* Run parameters.  Names and meanings follow the reference's file-scope
  * consts (evolution.cpp:23-59, mpi_evolution.cpp:17-60, mooreEvol.cpp:19-42).
- * ax_default_params() fills the BASELINE "plain" Abelian-Higgs set. */
+ * ax_default_params() fills the plain Abelian-Higgs set (dx = 0.5, dt = 0.1,
+ * lambda = 2, g = eta = 1, no friction, periodic). */
 typedef struct ax_params {
   int32_t model;            /* AX_MODEL_AH or AX_MODEL_MOORE */
   int32_t nx, ny, nz;       /* lattice sites per axis, each >= 3 */
@@ -77,9 +100,9 @@ typedef struct ax_params {
   /* 0: energy as evolution.cpp:403-427 (E_i = dtheta/(g dt dx_i), zero gauge
    *    terms when g == 0);  1: as mpi_evolution.cpp:513-530. */
   int32_t energy_form;
-  int32_t boundary;         /* AX_BC_PERIODIC, AX_BC_FIXED (AH, Moore) or
-                               AX_BC_NEUMANN_XY (global) */
+  int32_t boundary;         /* x and y: AX_BC_* (xyBC) */
   int32_t precision;        /* 64: FP64 drop-in; 32: FP32 storage+arithmetic */
+  int32_t boundary_z;       /* z: AX_BC_PERIODIC, AX_BC_FIXED or AX_BC_NEUMANN (zBC) */
 } ax_params;
 
 /* Diagnostics of the current state (level tNow of the next TimeStep),

@@ -272,21 +272,267 @@ void moore_site(const ax_params *p, const Lat &L, double *st, int tN, int tP, co
   tzP[s] = 2 * tz[s] - tzP[s] + dt * dt * atz;
 }
 
+/* ----------------------- non-periodic boundaries ------------------------
+ * R: storage/arithmetic type; T: sin/cos of R.  The reference's loops and
+ * expressions: Neumann faces evolution.cpp:192-265 (Moore: q-scaled links,
+ * mooreEvol.cpp:114-168); absorbing x/y faces and corners
+ * evolution.cpp:449-689. */
+template <class R, class T>
+void bc_neumann(const ax_params *p, const Lat &L, R *st, int tN) {
+  const bool moore = p->model == AX_MODEL_MOORE;
+  const int t0 = moore ? 4 : 2, nsc = moore ? 2 : 1;
+  const int c[2] = {1, -1};
+  auto TH = [&](int a, int i, int j, int k) -> R & { return st[((size_t)(t0 + a) * 2 + tN) * L.V + L.id(i, j, k)]; };
+  auto PH = [&](int sc, int comp, int i, int j, int k) -> R & {
+    return st[((size_t)(2 * sc + comp) * 2 + tN) * L.V + L.id(i, j, k)];
+  };
+  auto ang = [&](int sc, R th) -> R { return moore ? (R)(sc == 0 ? p->q1 : p->q2) * th : th; };
+  const bool zper = p->boundary_z == AX_BC_PERIODIC;
+  const int zEdge1 = zper ? 0 : 1, zEdge2 = zper ? L.nz : L.nz - 1;
+  if (p->boundary == AX_BC_NEUMANN) {
+    for (int i = 1; i < L.nx - 1; ++i)
+      for (int k = zEdge1; k < zEdge2; ++k) {
+        for (int comp = 0; comp < 3; ++comp) {
+          TH(comp, i, 0, k) = TH(comp, i, 1, k);
+          TH(comp, i, L.ny - 1, k) = TH(comp, i, L.ny - 2, k);
+        }
+        for (int comp = 0; comp < 2; ++comp)
+          for (int sc = 0; sc < nsc; ++sc) {
+            PH(sc, comp, i, 0, k) = T::cs(ang(sc, TH(1, i, 0, k))) * PH(sc, comp, i, 1, k) +
+                                    (R)c[comp] * T::sn(ang(sc, TH(1, i, 0, k))) * PH(sc, comp + c[comp], i, 1, k);
+            PH(sc, comp, i, L.ny - 1, k) =
+                T::cs(ang(sc, TH(1, i, L.ny - 2, k))) * PH(sc, comp, i, L.ny - 2, k) -
+                (R)c[comp] * T::sn(ang(sc, TH(1, i, L.ny - 2, k))) * PH(sc, comp + c[comp], i, L.ny - 2, k);
+          }
+      }
+    for (int j = 0; j < L.ny; ++j)
+      for (int k = zEdge1; k < zEdge2; ++k) {
+        for (int comp = 0; comp < 3; ++comp) {
+          TH(comp, 0, j, k) = TH(comp, 1, j, k);
+          TH(comp, L.nx - 1, j, k) = TH(comp, L.nx - 2, j, k);
+        }
+        for (int comp = 0; comp < 2; ++comp)
+          for (int sc = 0; sc < nsc; ++sc) {
+            PH(sc, comp, 0, j, k) = T::cs(ang(sc, TH(0, 0, j, k))) * PH(sc, comp, 1, j, k) +
+                                    (R)c[comp] * T::sn(ang(sc, TH(0, 0, j, k))) * PH(sc, comp + c[comp], 1, j, k);
+            PH(sc, comp, L.nx - 1, j, k) =
+                T::cs(ang(sc, TH(0, L.nx - 2, j, k))) * PH(sc, comp, L.nx - 2, j, k) -
+                (R)c[comp] * T::sn(ang(sc, TH(0, L.nx - 2, j, k))) * PH(sc, comp + c[comp], L.nx - 2, j, k);
+          }
+      }
+  }
+  if (!moore && p->boundary_z == AX_BC_NEUMANN) {
+    for (int i = 0; i < L.nz; ++i)          /* sic: x index bounded by nz (evolution.cpp:237) */
+      for (int j = 0; j < L.ny; ++j) {
+        for (int comp = 0; comp < 3; ++comp) {
+          TH(comp, i, j, 0) = TH(comp, i, j, 1);
+          TH(comp, i, j, L.nz - 1) = TH(comp, i, j, L.nz - 2);
+        }
+        for (int comp = 0; comp < 2; ++comp) {
+          PH(0, comp, i, j, 0) = T::cs(TH(2, i, j, 0)) * PH(0, comp, i, j, 1) +
+                                 (R)c[comp] * T::sn(TH(2, i, j, 0)) * PH(0, comp + c[comp], i, j, 1);
+          PH(0, comp, i, j, L.nz - 1) = T::cs(TH(2, i, j, L.nz - 2)) * PH(0, comp, i, j, L.nz - 2) -
+                                        (R)c[comp] * T::sn(TH(2, i, j, L.nz - 2)) * PH(0, comp + c[comp], i, j, L.nz - 2);
+        }
+      }
+  }
+}
+
+/* Absorbing x/y boundary (AH): second time derivatives of every face and
+ * corner site from the pre-update state, then their update
+ * prv = 2 cur - prv + dt^2 tt.  Must run before the bulk update. */
+template <class R, class T>
+void bc_absorbing(const ax_params *p, const Lat &L, R *st, int tN, int tP) {
+  const int c[2] = {1, -1};
+  const int nx = L.nx, ny = L.ny, nz = L.nz;
+  auto PHI = [&](int comp, int lev, int i, int j, int k) -> R & {
+    return st[((size_t)comp * 2 + lev) * L.V + L.id(i, j, k)];
+  };
+  auto TH = [&](int comp, int lev, int i, int j, int k) -> R & {
+    return st[((size_t)(2 + comp) * 2 + lev) * L.V + L.id(i, j, k)];
+  };
+  auto cs = [&](int a, int lev, int i, int j, int k) { return T::cs(TH(a, lev, i, j, k)); };
+  auto sn = [&](int a, int lev, int i, int j, int k) { return T::sn(TH(a, lev, i, j, k)); };
+  const R dx2 = (R)(p->dx * p->dx), dy2 = (R)(p->dy * p->dy), dz2 = (R)(p->dz * p->dz);
+  const R dtdx = (R)(p->dt * p->dx), dtdy = (R)(p->dt * p->dy), dt2 = (R)(p->dt * p->dt);
+  const R half = (R)0.5;
+  const bool zper = p->boundary_z == AX_BC_PERIODIC;
+  const int zEdge1 = zper ? 0 : 1, zEdge2 = zper ? nz : nz - 1;
+  const int tNow = tN, tPast = tP;
+  std::vector<R> phitt(2 * L.V, 0), thetatt(3 * L.V, 0);
+  auto PT = [&](int comp, int i, int j, int k) -> R & { return phitt[(size_t)comp * L.V + L.id(i, j, k)]; };
+  auto TT = [&](int comp, int i, int j, int k) -> R & { return thetatt[(size_t)comp * L.V + L.id(i, j, k)]; };
+  /* covariant second difference along axis a (0 x, 1 y, 2 z) of comp at (i,j,k) */
+  auto lap = [&](int comp, int a, int i, int j, int k, int ip, int jp, int kp, int im, int jm, int km, R d2) {
+    return (cs(a, tNow, i, j, k) * PHI(comp, tNow, ip, jp, kp) +
+            (R)c[comp] * sn(a, tNow, i, j, k) * PHI(comp + c[comp], tNow, ip, jp, kp) - 2 * PHI(comp, tNow, i, j, k) +
+            cs(a, tNow, im, jm, km) * PHI(comp, tNow, im, jm, km) -
+            (R)c[comp] * sn(a, tNow, im, jm, km) * PHI(comp + c[comp], tNow, im, jm, km)) / d2;
+  };
+  for (int k = zEdge1; k < zEdge2; ++k) {
+    const int kp = (k + 1) % nz, km = (k - 1 + nz) % nz;
+    /* x faces, evolution.cpp:453-517 */
+    for (int j = 1; j < ny - 1; ++j) {
+      for (int comp = 0; comp < 2; ++comp) {
+        const int o = comp + c[comp];
+        R phiyy = lap(comp, 1, 0, j, k, 0, j + 1, k, 0, j - 1, k, dy2);
+        R phizz = lap(comp, 2, 0, j, k, 0, j, kp, 0, j, km, dz2);
+        R phitx = (cs(0, tNow, 0, j, k) * PHI(comp, tNow, 1, j, k) + (R)c[comp] * sn(0, tNow, 0, j, k) * PHI(o, tNow, 1, j, k) -
+                   PHI(comp, tNow, 0, j, k) - cs(0, tPast, 0, j, k) * PHI(comp, tPast, 1, j, k) -
+                   (R)c[comp] * sn(0, tPast, 0, j, k) * PHI(o, tPast, 1, j, k) + PHI(comp, tPast, 0, j, k)) / dtdx;
+        PT(comp, 0, j, k) = phitx + half * (phiyy + phizz);
+        phiyy = lap(comp, 1, nx - 1, j, k, nx - 1, j + 1, k, nx - 1, j - 1, k, dy2);
+        phizz = lap(comp, 2, nx - 1, j, k, nx - 1, j, kp, nx - 1, j, km, dz2);
+        phitx = (PHI(comp, tNow, nx - 1, j, k) - cs(0, tNow, nx - 2, j, k) * PHI(comp, tNow, nx - 2, j, k) +
+                 (R)c[comp] * sn(0, tNow, nx - 2, j, k) * PHI(o, tNow, nx - 2, j, k) - PHI(comp, tPast, nx - 1, j, k) +
+                 cs(0, tPast, nx - 2, j, k) * PHI(comp, tPast, nx - 2, j, k) -
+                 (R)c[comp] * sn(0, tPast, nx - 2, j, k) * PHI(o, tPast, nx - 2, j, k)) / dtdx;
+        PT(comp, nx - 1, j, k) = -phitx + half * (phiyy + phizz);
+      }
+      for (int comp = 0; comp < 3; ++comp) {
+        R thetayy = (TH(comp, tNow, 0, j + 1, k) - 2 * TH(comp, tNow, 0, j, k) + TH(comp, tNow, 0, j - 1, k)) / dy2;
+        R thetazz = (TH(comp, tNow, 0, j, kp) - 2 * TH(comp, tNow, 0, j, k) + TH(comp, tNow, 0, j, km)) / dz2;
+        R thetatx = (TH(comp, tNow, 1, j, k) - TH(comp, tNow, 0, j, k) - TH(comp, tPast, 1, j, k) + TH(comp, tPast, 0, j, k)) / dtdx;
+        TT(comp, 0, j, k) = thetatx + half * (thetayy + thetazz);
+        thetayy = (TH(comp, tNow, nx - 1, j + 1, k) - 2 * TH(comp, tNow, nx - 1, j, k) + TH(comp, tNow, nx - 1, j - 1, k)) / dy2;
+        thetazz = (TH(comp, tNow, nx - 1, j, kp) - 2 * TH(comp, tNow, nx - 1, j, k) + TH(comp, tNow, nx - 1, j, km)) / dz2;
+        thetatx = (TH(comp, tNow, nx - 1, j, k) - TH(comp, tNow, nx - 2, j, k) - TH(comp, tPast, nx - 1, j, k) +
+                   TH(comp, tPast, nx - 2, j, k)) / dtdx;
+        TT(comp, nx - 1, j, k) = -thetatx + half * (thetayy + thetazz);
+      }
+    }
+    /* y faces, evolution.cpp:521-604 */
+    for (int i = 1; i < nx - 1; ++i) {
+      for (int comp = 0; comp < 2; ++comp) {
+        const int o = comp + c[comp];
+        R phixx = lap(comp, 0, i, 0, k, i + 1, 0, k, i - 1, 0, k, dx2);
+        R phizz = lap(comp, 2, i, 0, k, i, 0, kp, i, 0, km, dz2);
+        R phity = (cs(1, tNow, i, 0, k) * PHI(comp, tNow, i, 1, k) + (R)c[comp] * sn(1, tNow, i, 0, k) * PHI(o, tNow, i, 1, k) -
+                   PHI(comp, tNow, i, 0, k) - cs(1, tPast, i, 0, k) * PHI(comp, tPast, i, 1, k) -
+                   (R)c[comp] * sn(1, tPast, i, 0, k) * PHI(o, tPast, i, 1, k) + PHI(comp, tPast, i, 0, k)) / dtdy;
+        PT(comp, i, 0, k) = phity + half * (phixx + phizz);
+        phixx = lap(comp, 0, i, ny - 1, k, i + 1, ny - 1, k, i - 1, ny - 1, k, dx2);
+        phizz = lap(comp, 2, i, ny - 1, k, i, ny - 1, kp, i, ny - 1, km, dz2);
+        phity = (PHI(comp, tNow, i, ny - 1, k) - cs(1, tNow, i, ny - 2, k) * PHI(comp, tNow, i, ny - 2, k) +
+                 (R)c[comp] * sn(1, tNow, i, ny - 2, k) * PHI(o, tNow, i, ny - 2, k) - PHI(comp, tPast, i, ny - 1, k) +
+                 cs(1, tPast, i, ny - 2, k) * PHI(comp, tPast, i, ny - 2, k) -
+                 (R)c[comp] * sn(1, tPast, i, ny - 2, k) * PHI(o, tPast, i, ny - 2, k)) / dtdy;
+        PT(comp, i, ny - 1, k) = -phity + half * (phixx + phizz);
+      }
+      for (int comp = 0; comp < 3; ++comp) {
+        R thetaxx = (TH(comp, tNow, i + 1, 0, k) - 2 * TH(comp, tNow, i, 0, k) + TH(comp, tNow, i - 1, 0, k)) / dx2;
+        R thetazz = (TH(comp, tNow, i, 0, kp) - 2 * TH(comp, tNow, i, 0, k) + TH(comp, tNow, i, 0, km)) / dz2;
+        R thetaty = (TH(comp, tNow, i, 1, k) - TH(comp, tNow, i, 0, k) - TH(comp, tPast, i, 1, k) + TH(comp, tPast, i, 0, k)) / dtdy;
+        TT(comp, i, 0, k) = thetaty + half * (thetaxx + thetazz);
+        thetaxx = (TH(comp, tNow, i + 1, ny - 1, k) - 2 * TH(comp, tNow, i, ny - 1, k) + TH(comp, tNow, i - 1, ny - 1, k)) / dx2;
+        thetazz = (TH(comp, tNow, i, ny - 1, kp) - 2 * TH(comp, tNow, i, ny - 1, k) + TH(comp, tNow, i, ny - 1, km)) / dz2;
+        thetaty = (TH(comp, tNow, i, ny - 1, k) - TH(comp, tNow, i, ny - 2, k) - TH(comp, tPast, i, ny - 1, k) +
+                   TH(comp, tPast, i, ny - 2, k)) / dtdy;
+        TT(comp, i, ny - 1, k) = -thetaty + half * (thetaxx + thetazz);
+      }
+    }
+    /* corners, evolution.cpp:608-683 */
+    for (int comp = 0; comp < 2; ++comp) {
+      const int o = comp + c[comp];
+      /* x,y < 0 */
+      R phizz = lap(comp, 2, 0, 0, k, 0, 0, kp, 0, 0, km, dz2);
+      R phitx = (cs(0, tNow, 0, 0, k) * PHI(comp, tNow, 1, 0, k) + (R)c[comp] * sn(0, tNow, 0, 0, k) * PHI(o, tNow, 1, 0, k) -
+                 PHI(comp, tNow, 0, 0, k) - cs(0, tPast, 0, 0, k) * PHI(comp, tPast, 1, 0, k) -
+                 (R)c[comp] * sn(0, tPast, 0, 0, k) * PHI(o, tPast, 1, 0, k) + PHI(comp, tPast, 0, 0, k)) / dtdx;
+      R phity = (cs(1, tNow, 0, 0, k) * PHI(comp, tNow, 0, 1, k) + (R)c[comp] * sn(1, tNow, 0, 0, k) * PHI(o, tNow, 0, 1, k) -
+                 PHI(comp, tNow, 0, 0, k) - cs(1, tPast, 0, 0, k) * PHI(comp, tPast, 0, 1, k) -
+                 (R)c[comp] * sn(1, tPast, 0, 0, k) * PHI(o, tPast, 0, 1, k) + PHI(comp, tPast, 0, 0, k)) / dtdy;
+      PT(comp, 0, 0, k) = (phizz + 2 * (phitx + phity)) / 3;
+      /* x < 0, y > 0 */
+      phizz = lap(comp, 2, 0, ny - 1, k, 0, ny - 1, kp, 0, ny - 1, km, dz2);
+      phitx = (cs(0, tNow, 0, ny - 1, k) * PHI(comp, tNow, 1, ny - 1, k) +
+               (R)c[comp] * sn(0, tNow, 0, ny - 1, k) * PHI(o, tNow, 1, ny - 1, k) - PHI(comp, tNow, 0, ny - 1, k) -
+               cs(0, tPast, 0, ny - 1, k) * PHI(comp, tPast, 1, ny - 1, k) -
+               (R)c[comp] * sn(0, tPast, 0, ny - 1, k) * PHI(o, tPast, 1, ny - 1, k) + PHI(comp, tPast, 0, ny - 1, k)) / dtdx;
+      phity = (PHI(comp, tNow, 0, ny - 1, k) - cs(1, tNow, 0, ny - 2, k) * PHI(comp, tNow, 0, ny - 2, k) +
+               (R)c[comp] * sn(1, tNow, 0, ny - 2, k) * PHI(o, tNow, 0, ny - 2, k) - PHI(comp, tPast, 0, ny - 1, k) +
+               cs(1, tPast, 0, ny - 2, k) * PHI(comp, tPast, 0, ny - 2, k) -
+               (R)c[comp] * sn(1, tPast, 0, ny - 2, k) * PHI(o, tPast, 0, ny - 2, k)) / dtdy;
+      PT(comp, 0, ny - 1, k) = (phizz + 2 * (phitx - phity)) / 3;
+      /* x > 0, y < 0 */
+      phizz = lap(comp, 2, nx - 1, 0, k, nx - 1, 0, kp, nx - 1, 0, km, dz2);
+      phitx = (PHI(comp, tNow, nx - 1, 0, k) - cs(0, tNow, nx - 2, 0, k) * PHI(comp, tNow, nx - 2, 0, k) +
+               (R)c[comp] * sn(0, tNow, nx - 2, 0, k) * PHI(o, tNow, nx - 2, 0, k) - PHI(comp, tPast, nx - 1, 0, k) +
+               cs(0, tPast, nx - 2, 0, k) * PHI(comp, tPast, nx - 2, 0, k) -
+               (R)c[comp] * sn(0, tPast, nx - 2, 0, k) * PHI(o, tPast, nx - 2, 0, k)) / dtdx;
+      phity = (cs(1, tNow, nx - 1, 0, k) * PHI(comp, tNow, nx - 1, 1, k) +
+               (R)c[comp] * sn(1, tNow, nx - 1, 0, k) * PHI(o, tNow, nx - 1, 1, k) - PHI(comp, tNow, nx - 1, 0, k) -
+               cs(1, tPast, nx - 1, 0, k) * PHI(comp, tPast, nx - 1, 1, k) -
+               (R)c[comp] * sn(1, tPast, nx - 1, 0, k) * PHI(o, tPast, nx - 1, 1, k) + PHI(comp, tPast, nx - 1, 0, k)) / dtdy;
+      PT(comp, nx - 1, 0, k) = (phizz - 2 * (phitx - phity)) / 3;
+      /* x, y > 0; the level-tPast cosine is taken at the corner (evolution.cpp:642) */
+      phizz = lap(comp, 2, nx - 1, ny - 1, k, nx - 1, ny - 1, kp, nx - 1, ny - 1, km, dz2);
+      phitx = (PHI(comp, tNow, nx - 1, ny - 1, k) - cs(0, tNow, nx - 2, ny - 1, k) * PHI(comp, tNow, nx - 2, ny - 1, k) +
+               (R)c[comp] * sn(0, tNow, nx - 2, ny - 1, k) * PHI(o, tNow, nx - 2, ny - 1, k) -
+               PHI(comp, tPast, nx - 1, ny - 1, k) + cs(0, tPast, nx - 1, ny - 1, k) * PHI(comp, tPast, nx - 2, ny - 1, k) -
+               (R)c[comp] * sn(0, tPast, nx - 2, ny - 1, k) * PHI(o, tPast, nx - 2, ny - 1, k)) / dtdx;
+      phity = (PHI(comp, tNow, nx - 1, ny - 1, k) - cs(1, tNow, nx - 1, ny - 2, k) * PHI(comp, tNow, nx - 1, ny - 2, k) +
+               (R)c[comp] * sn(1, tNow, nx - 1, ny - 2, k) * PHI(o, tNow, nx - 1, ny - 2, k) -
+               PHI(comp, tPast, nx - 1, ny - 1, k) + cs(1, tPast, nx - 1, ny - 2, k) * PHI(comp, tPast, nx - 1, ny - 2, k) -
+               (R)c[comp] * sn(1, tPast, nx - 1, ny - 2, k) * PHI(o, tPast, nx - 1, ny - 2, k)) / dtdy;
+      PT(comp, nx - 1, ny - 1, k) = (phizz - 2 * (phitx + phity)) / 3;
+    }
+    for (int comp = 0; comp < 3; ++comp) {
+      R thetazz = (TH(comp, tNow, 0, 0, kp) - 2 * TH(comp, tNow, 0, 0, k) + TH(comp, tNow, 0, 0, km)) / dz2;
+      R thetatx = (TH(comp, tNow, 1, 0, k) - TH(comp, tNow, 0, 0, k) - TH(comp, tPast, 1, 0, k) + TH(comp, tPast, 0, 0, k)) / dtdx;
+      R thetaty = (TH(comp, tNow, 0, 1, k) - TH(comp, tNow, 0, 0, k) - TH(comp, tPast, 0, 1, k) + TH(comp, tPast, 0, 0, k)) / dtdy;
+      TT(comp, 0, 0, k) = (thetazz + 2 * (thetatx + thetaty)) / 3;
+      thetazz = (TH(comp, tNow, 0, ny - 1, kp) - 2 * TH(comp, tNow, 0, ny - 1, k) + TH(comp, tNow, 0, ny - 1, km)) / dz2;
+      thetatx = (TH(comp, tNow, 1, ny - 1, k) - TH(comp, tNow, 0, ny - 1, k) - TH(comp, tPast, 1, ny - 1, k) +
+                 TH(comp, tPast, 0, ny - 1, k)) / dtdx;
+      thetaty = (TH(comp, tNow, 0, ny - 1, k) - TH(comp, tNow, 0, ny - 2, k) - TH(comp, tPast, 0, ny - 1, k) +
+                 TH(comp, tPast, 0, ny - 2, k)) / dtdy;
+      TT(comp, 0, ny - 1, k) = (thetazz + 2 * (thetatx - thetaty)) / 3;
+      thetazz = (TH(comp, tNow, nx - 1, 0, kp) - 2 * TH(comp, tNow, nx - 1, 0, k) + TH(comp, tNow, nx - 1, 0, km)) / dz2;
+      thetatx = (TH(comp, tNow, nx - 1, 0, k) - TH(comp, tNow, nx - 2, 0, k) - TH(comp, tPast, nx - 1, 0, k) +
+                 TH(comp, tPast, nx - 2, 0, k)) / dtdx;
+      thetaty = (TH(comp, tNow, nx - 1, 1, k) - TH(comp, tNow, nx - 1, 0, k) - TH(comp, tPast, nx - 1, 1, k) +
+                 TH(comp, tPast, nx - 1, 0, k)) / dtdy;
+      TT(comp, nx - 1, 0, k) = (thetazz - 2 * (thetatx - thetaty)) / 3;
+      thetazz = (TH(comp, tNow, nx - 1, ny - 1, kp) - 2 * TH(comp, tNow, nx - 1, ny - 1, k) +
+                 TH(comp, tNow, nx - 1, ny - 1, km)) / dz2;
+      thetatx = (TH(comp, tNow, nx - 1, ny - 1, k) - TH(comp, tNow, nx - 2, ny - 1, k) - TH(comp, tPast, nx - 1, ny - 1, k) +
+                 TH(comp, tPast, nx - 2, ny - 1, k)) / dtdx;
+      thetaty = (TH(comp, tNow, nx - 1, ny - 1, k) - TH(comp, tNow, nx - 1, ny - 2, k) - TH(comp, tPast, nx - 1, ny - 1, k) +
+                 TH(comp, tPast, nx - 1, ny - 2, k)) / dtdy;
+      TT(comp, nx - 1, ny - 1, k) = (thetazz - 2 * (thetatx + thetaty)) / 3;
+    }
+  }
+  /* update of the boundary sites (the bulk update covers the interior) */
+  for (int i = 0; i < nx; ++i)
+    for (int j = 0; j < ny; ++j) {
+      if (i > 0 && i < nx - 1 && j > 0 && j < ny - 1) continue;
+      for (int k = zEdge1; k < zEdge2; ++k) {
+        for (int comp = 0; comp < 2; ++comp)
+          PHI(comp, tPast, i, j, k) = 2 * PHI(comp, tNow, i, j, k) - PHI(comp, tPast, i, j, k) + dt2 * PT(comp, i, j, k);
+        for (int comp = 0; comp < 3; ++comp)
+          TH(comp, tPast, i, j, k) = 2 * TH(comp, tNow, i, j, k) - TH(comp, tPast, i, j, k) + dt2 * TT(comp, i, j, k);
+      }
+    }
+}
+
 template <class T>
 void run_steps(const ax_params *p, double *st, int ts0, int nsteps, int nthreads) {
   Lat L(p);
-  const int e = (p->boundary == AX_BC_FIXED) ? 1 : 0;
+  const int e = (p->boundary != AX_BC_PERIODIC) ? 1 : 0, ez = (p->boundary_z != AX_BC_PERIODIC) ? 1 : 0;
   for (int n = ts0; n < ts0 + nsteps; ++n) {
     const int tN = (n + 1) % 2, tP = n % 2;
     const double time = 1 + n * p->dt;
     const double damp = (n < p->damped_nt) ? p->dampFac : 0;
     const double fric = damp + p->alpha * p->scaling / time;
+    if (p->boundary == AX_BC_NEUMANN || p->boundary_z == AX_BC_NEUMANN) bc_neumann<double, T>(p, L, st, tN);
+    if (p->boundary == AX_BC_ABSORBING) bc_absorbing<double, T>(p, L, st, tN, tP);
 #ifdef _OPENMP
 #pragma omp parallel for schedule(static) num_threads(nthreads > 0 ? nthreads : 1)
 #endif
     for (int i = e; i < L.nx - e; ++i)
       for (int j = e; j < L.ny - e; ++j)
-        for (int k = e; k < L.nz - e; ++k) {
+        for (int k = ez; k < L.nz - ez; ++k) {
           Nb nb(L, i, j, k);
           if (p->model == AX_MODEL_AH)
             ah_site<T>(p, L, st, tN, tP, nb, damp, fric);
@@ -311,9 +557,11 @@ void diag_ah(const ax_params *p, const double *st, int n, double *out) {
   const double dx = p->dx, dy = p->dy, dz = p->dz, dt = p->dt, g = p->g, lam = p->lambda, eta = p->eta;
   const double sg[2] = {1.0, -1.0};
   double energy = 0, dev = 0;
-  for (int i = 0; i < L.nx; ++i)
-    for (int j = 0; j < L.ny; ++j)
-      for (int k = 0; k < L.nz; ++k) {
+  /* the bulk-update region (evolution.cpp:161-186) */
+  const int e = p->boundary != AX_BC_PERIODIC, ez = p->boundary_z != AX_BC_PERIODIC;
+  for (int i = e; i < L.nx - e; ++i)
+    for (int j = e; j < L.ny - e; ++j)
+      for (int k = ez; k < L.nz - ez; ++k) {
         Nb n_(L, i, j, k);
         const size_t s = L.id(i, j, k);
         const size_t sxp = L.id(n_.ip, j, k), sxm = L.id(n_.im, j, k);
@@ -372,8 +620,11 @@ void diag_ah(const ax_params *p, const double *st, int n, double *out) {
                      0.25 * lam * (vv * vv)) * dx * dy * dz;
         }
       }
-  dev = dev / (L.nx * L.ny);
-  dev = dev / L.nz;
+  /* evolution.cpp:1400-1408 */
+  if (p->boundary == AX_BC_PERIODIC) dev = dev / (L.nx * L.ny);
+  else dev = dev / ((L.nx - 2) * (L.ny - 2));
+  if (p->boundary_z == AX_BC_PERIODIC) dev = dev / L.nz;
+  else dev = dev / (L.nz - 2);
   out[0] = energy;
   out[1] = (p->energy_form == 0) ? dev : 0.0;
 }
@@ -391,7 +642,7 @@ void diag_moore(const ax_params *p, const double *st, int n, double *out) {
   const double dx = p->dx, dy = p->dy, dz = p->dz, dt = p->dt, g = p->g;
   const int q1 = p->q1, q2 = p->q2;
   const double sg[2] = {1.0, -1.0};
-  const bool fixed = p->boundary == AX_BC_FIXED;
+  const bool fixed = p->boundary != AX_BC_PERIODIC;
   const int e = fixed ? 1 : 0;
   const int inner = (L.nx - 2) * (L.ny - 2) * (L.nz - 2);
   double energy = 0, dev = 0;
@@ -486,6 +737,7 @@ void winding_impl(const ax_params *p, const double *st, int n, int8_t *out, int 
   const double q = (p->model == AX_MODEL_AH) ? 1.0 : (double)(field == 0 ? p->q1 : p->q2);
   const double *fr = fld(st, L, c0, tN), *fi = fld(st, L, c0 + 1, tN);
   const double *th[3] = {fld(st, L, t0, tN), fld(st, L, t0 + 1, tN), fld(st, L, t0 + 2, tN)};
+  const bool ex = p->boundary != AX_BC_PERIODIC, ez = p->boundary_z != AX_BC_PERIODIC;
   for (int i = 0; i < L.nx; ++i)
     for (int j = 0; j < L.ny; ++j)
       for (int k = 0; k < L.nz; ++k) {
@@ -507,7 +759,10 @@ void winding_impl(const ax_params *p, const double *st, int n, int8_t *out, int 
           double da2 = link_phase(fr[s_b], fi[s_b], fr[s_ab], fi[s_ab], q * th[a][s_b]);
           double db3 = link_phase(fr[s], fi[s], fr[s_b], fi[s_b], q * th[b][s]);
           double P = q * th[a][s] + q * th[b][s_a] - q * th[a][s_b] - q * th[b][s];
-          out[(size_t)pl * L.V + s] = (int8_t)plaq_wind(da0, db1, da2, db3, P);
+          /* a plaquette wrapping across a non-periodic axis does not exist */
+          const bool wx = (a == 0 || b == 0) && ex && i == L.nx - 1, wy = (a == 1 || b == 1) && ex && j == L.ny - 1,
+                     wz = (a == 2 || b == 2) && ez && k == L.nz - 1;
+          out[(size_t)pl * L.V + s] = (wx || wy || wz) ? 0 : (int8_t)plaq_wind(da0, db1, da2, db3, P);
         }
       }
 }
@@ -573,9 +828,11 @@ void crossings_impl(const ax_params *p, const double *st, int n, Pts &P) {
   const double *f[2] = {fld(st, L, 0, tN), fld(st, L, 1, tN)};
   const int x0 = int(0.5 * (L.nx - 1)), y0 = int(0.5 * (L.ny - 1)), z0 = int(0.5 * (L.nz - 1));
   const double dx = p->dx, dy = p->dy, dz = p->dz;
-  for (int i = 0; i < L.nx; ++i)
-    for (int j = 0; j < L.ny; ++j)
-      for (int k = 0; k < L.nz; ++k) {
+  /* the bulk-update region (evolution.cpp:731-743) */
+  const int e = p->boundary != AX_BC_PERIODIC, ez = p->boundary_z != AX_BC_PERIODIC;
+  for (int i = e; i < L.nx - e; ++i)
+    for (int j = e; j < L.ny - e; ++j)
+      for (int k = ez; k < L.nz - ez; ++k) {
         Nb n_(L, i, j, k);
         size_t s = L.id(i, j, k), sx = L.id(n_.ip, j, k), sy = L.id(i, n_.jp, k), sz = L.id(i, j, n_.kp);
         size_t sxy = L.id(n_.ip, n_.jp, k), sxz = L.id(n_.ip, j, n_.kp), syz = L.id(i, n_.jp, n_.kp);
@@ -687,12 +944,14 @@ void ah_site_r(const Lat &L, R *st, int tN, int tP, const Nb &n, const RConsts<R
 template <class R>
 void run_steps_r(const ax_params *p, R *st, int ts0, int nsteps, int nthreads) {
   Lat L(p);
-  const int e = (p->boundary == AX_BC_FIXED) ? 1 : 0;
+  const int e = (p->boundary != AX_BC_PERIODIC) ? 1 : 0, ez = (p->boundary_z != AX_BC_PERIODIC) ? 1 : 0;
   for (int n = ts0; n < ts0 + nsteps; ++n) {
     const int tN = (n + 1) % 2, tP = n % 2;
     const double time = 1 + n * p->dt;
     const double damp = (n < p->damped_nt) ? p->dampFac : 0;
     const double fric = damp + p->alpha * p->scaling / time;
+    if (p->boundary == AX_BC_NEUMANN || p->boundary_z == AX_BC_NEUMANN) bc_neumann<R, TrigOf<R>>(p, L, st, tN);
+    if (p->boundary == AX_BC_ABSORBING) bc_absorbing<R, TrigOf<R>>(p, L, st, tN, tP);
     RConsts<R> k;
     k.dx2 = (R)(p->dx * p->dx); k.dy2 = (R)(p->dy * p->dy); k.dz2 = (R)(p->dz * p->dz);
     k.dt = (R)p->dt; k.dt2 = (R)(p->dt * p->dt);
@@ -705,7 +964,7 @@ void run_steps_r(const ax_params *p, R *st, int ts0, int nsteps, int nthreads) {
 #endif
     for (int i = e; i < L.nx - e; ++i)
       for (int j = e; j < L.ny - e; ++j)
-        for (int kk = e; kk < L.nz - e; ++kk) ah_site_r<R>(L, st, tN, tP, Nb(L, i, j, kk), k);
+        for (int kk = ez; kk < L.nz - ez; ++kk) ah_site_r<R>(L, st, tN, tP, Nb(L, i, j, kk), k);
   }
 }
 
@@ -794,6 +1053,19 @@ void global_steps(const ax_params *p, R *st, int nsteps, double *energy) {
 }  // namespace
 
 /* ================================ C ABI =================================== */
+/* With Neumann faces the reference's diagnostics see the faces its step
+ * has just set (evolution.cpp:192-265 precede the site loop): a copy of the
+ * state with that copy applied. */
+template <class T>
+const double *with_faces(const ax_params *p, const double *st, int n, std::vector<double> &buf) {
+  if (p->boundary != AX_BC_NEUMANN && p->boundary_z != AX_BC_NEUMANN) return st;
+  Lat L(p);
+  const int nc = (p->model == AX_MODEL_AH) ? 5 : 7;
+  buf.assign(st, st + (size_t)nc * 2 * L.V);
+  bc_neumann<double, T>(p, L, buf.data(), (n + 1) % 2);
+  return buf.data();
+}
+
 extern "C" {
 
 int orc_version(void) { return 1; }
@@ -1039,8 +1311,10 @@ int orc_global_step(const ax_params *p, void *st, int nsteps, double *energy) {
   return 0;
 }
 
-int orc_diag(const ax_params *p, const double *st, int n, int trig_mode, double *out) {
+int orc_diag(const ax_params *p, const double *st0, int n, int trig_mode, double *out) {
   if (p->model == AX_MODEL_GLOBAL) return -1;
+  std::vector<double> buf;
+  const double *st = trig_mode == 1 ? with_faces<TrigLibm>(p, st0, n, buf) : with_faces<TrigAx>(p, st0, n, buf);
   if (p->model == AX_MODEL_AH) {
     if (trig_mode == 1) diag_ah<TrigLibm>(p, st, n, out);
     else diag_ah<TrigAx>(p, st, n, out);
@@ -1060,14 +1334,16 @@ int orc_diag(const ax_params *p, const double *st, int n, int trig_mode, double 
 /* field: 0 = phi (AH) / phi1 (Moore), 1 = phi2 (Moore) */
 int orc_winding(const ax_params *p, const double *st, int n, int field, int8_t *out) {
   if (p->model == AX_MODEL_GLOBAL) return -1;
-  winding_impl(p, st, n, out, field);
+  std::vector<double> buf;
+  winding_impl(p, with_faces<TrigAx>(p, st, n, buf), n, out, field);
   return 0;
 }
 
 int orc_crossings(const ax_params *p, const double *st, int n, double *xyz, size_t cap, size_t *count) {
   if (p->model == AX_MODEL_GLOBAL) return -1;
   Pts P{xyz, cap, 0};
-  crossings_impl(p, st, n, P);
+  std::vector<double> buf;
+  crossings_impl(p, with_faces<TrigAx>(p, st, n, buf), n, P);
   *count = P.n;
   return 0;
 }

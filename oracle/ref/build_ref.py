@@ -69,6 +69,27 @@ CONFIGS = {
                                                  splitLength="true"), False),
     "icw_len_total": ("ic_w_evolution.cpp", dict(nx=12, ny=13, nz=11, nt=1, dx=0.5, dy=0.45, dz=0.4,
                                                  splitLength="false"), False),
+    # non-periodic boundaries of evolution.cpp (xyBC x zBC) and mooreEvol.cpp
+    "evo_bc_neumann": ("evolution.cpp", dict(_AH_SMALL, nx=14, ny=12, nz=10, nt=40, dx=0.5, dy=0.5, dz=0.5, dt=0.1,
+                                             **{"lambda": 2}, eta=1, g=1, damped_nt=10, dampFac=0.5, alpha=2,
+                                             scaling=1, xyBC='"neumann"', zBC='"periodic"'), True),
+    "evo_bc_neumann_z": ("evolution.cpp", dict(_AH_SMALL, nx=13, ny=11, nz=9, nt=40, dx=0.6, dy=0.5, dz=0.55,
+                                               dt=0.1, **{"lambda": 1.5}, eta=1, g=0.8, damped_nt=8, dampFac=0.4,
+                                               alpha=2, scaling=1, xyBC='"neumann"', zBC='"neumann"'), True),
+    "evo_bc_absorbing": ("evolution.cpp", dict(_AH_SMALL, nx=14, ny=13, nz=10, nt=40, dx=0.5, dy=0.5, dz=0.5,
+                                               dt=0.1, **{"lambda": 2}, eta=1, g=1, damped_nt=10, dampFac=0.5,
+                                               alpha=2, scaling=1, xyBC='"absorbing"', zBC='"periodic"'), True),
+    "evo_bc_absorbing_zfixed": ("evolution.cpp", dict(_AH_SMALL, nx=12, ny=14, nz=11, nt=40, dx=0.6, dy=0.55,
+                                                      dz=0.5, dt=0.1, **{"lambda": 1.5}, eta=0.9, g=0.8,
+                                                      damped_nt=8, dampFac=0.4, alpha=2, scaling=1,
+                                                      xyBC='"absorbing"', zBC='"fixed"'), True),
+    "evo_bc_fixed_zper": ("evolution.cpp", dict(_AH_SMALL, nx=13, ny=12, nz=10, nt=40, dt=0.1, **{"lambda": 2},
+                                                eta=1, g=1, damped_nt=10, dampFac=0.5, alpha=2, scaling=1,
+                                                xyBC='"fixed"', zBC='"periodic"'), True),
+    "evo_bc_fixed_zneu": ("evolution.cpp", dict(_AH_SMALL, nx=12, ny=11, nz=12, nt=40, dt=0.1, **{"lambda": 2},
+                                                eta=1, g=1, damped_nt=10, dampFac=0.5, alpha=2, scaling=1,
+                                                xyBC='"fixed"', zBC='"neumann"'), True),
+    "moore_neumann": ("mooreEvol.cpp", dict(nx=12, ny=11, nz=10, nt=30, makeGif="false", xyBC='"neumann"'), True),
     # straight-string pipeline: SOR radial profile (singleSemiAnalytic.cpp)
     # as shipped (global string, 30001 points), a gauged one, and one that
     # ends at its sweep cap; then the lattice interpolation (straight_ic.cpp)

@@ -27,7 +27,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 REPO = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, REPO)
 
-from axions_amd.abi import ah_params, global_params, moore_params, AX_BC_FIXED  # noqa: E402
+from axions_amd.abi import (ah_params, global_params, moore_params, AX_BC_ABSORBING, AX_BC_FIXED,  # noqa: E402
+                             AX_BC_NEUMANN, AX_BC_PERIODIC)
 from oracle import oracle_py as orc  # noqa: E402
 sys.path.insert(0, os.path.join(REPO, "oracle", "ref"))
 import build_ref  # noqa: E402
@@ -47,6 +48,26 @@ SPECS = {
                                        theta_damp_form=1), 40, 5, 10, "shared"),
     "evo_c1_shared": (lambda: ah_params(64, dx=0.5, dt=0.1, lam=2.0, eta=1.0, g=1.0, alpha=0.0,
                                         damped_nt=0, dampFac=0.5, theta_damp_form=1), 200, 10, 0, "shared"),
+    "evo_bc_neumann": (lambda: ah_params(14, ny=12, nz=10, dx=0.5, dt=0.1, lam=2.0, g=1.0, alpha=2.0, damped_nt=10,
+                                         dampFac=0.5, theta_damp_form=1, boundary=AX_BC_NEUMANN,
+                                         boundary_z=AX_BC_PERIODIC), 40, 5, 10, "shared"),
+    "evo_bc_neumann_z": (lambda: ah_params(13, ny=11, nz=9, dx=0.6, dy=0.5, dz=0.55, dt=0.1, lam=1.5, g=0.8,
+                                           alpha=2.0, damped_nt=8, dampFac=0.4, theta_damp_form=1,
+                                           boundary=AX_BC_NEUMANN, boundary_z=AX_BC_NEUMANN), 40, 5, 8, "shared"),
+    "evo_bc_absorbing": (lambda: ah_params(14, ny=13, nz=10, dx=0.5, dt=0.1, lam=2.0, g=1.0, alpha=2.0,
+                                           damped_nt=10, dampFac=0.5, theta_damp_form=1, boundary=AX_BC_ABSORBING,
+                                           boundary_z=AX_BC_PERIODIC), 40, 5, 10, "shared"),
+    "evo_bc_absorbing_zfixed": (lambda: ah_params(12, ny=14, nz=11, dx=0.6, dy=0.55, dz=0.5, dt=0.1, lam=1.5,
+                                                  eta=0.9, g=0.8, alpha=2.0, damped_nt=8, dampFac=0.4,
+                                                  theta_damp_form=1, boundary=AX_BC_ABSORBING,
+                                                  boundary_z=AX_BC_FIXED), 40, 5, 8, "shared"),
+    "evo_bc_fixed_zper": (lambda: ah_params(13, ny=12, nz=10, dx=0.5, dt=0.1, lam=2.0, g=1.0, alpha=2.0,
+                                            damped_nt=10, dampFac=0.5, theta_damp_form=1, boundary=AX_BC_FIXED,
+                                            boundary_z=AX_BC_PERIODIC), 40, 5, 10, "shared"),
+    "evo_bc_fixed_zneu": (lambda: ah_params(12, ny=11, nz=12, dx=0.5, dt=0.1, lam=2.0, g=1.0, alpha=2.0,
+                                            damped_nt=10, dampFac=0.5, theta_damp_form=1, boundary=AX_BC_FIXED,
+                                            boundary_z=AX_BC_NEUMANN), 40, 5, 10, "shared"),
+    "moore_neumann": (lambda: moore_params(12, ny=11, nz=10, boundary=AX_BC_NEUMANN), 30, None, None, "shared"),
     "moore_shared": (lambda: moore_params(12, boundary=AX_BC_FIXED), 30, None, None, "shared"),
     "moore_libm": (lambda: moore_params(12, boundary=AX_BC_FIXED), 30, None, None, "libm"),
     "global_neumann": (lambda: global_params(12, ny=10, nz=14), 40, None, None, "none"),

@@ -78,3 +78,24 @@ def test_sor_params_and_validation_without_gpu():
             assert e.code == -1
         else:
             raise AssertionError(bad)
+
+
+def test_boundary_combinations_validated_without_gpu():
+    """Supported (xyBC, zBC) pairs per model; everything else is refused."""
+    import axions_amd as ax
+    from axions_amd.abi import AX_BC_ABSORBING, AX_BC_FIXED, AX_BC_NEUMANN, AX_BC_PERIODIC
+    ok = [ax.ah_params(8, boundary=b, boundary_z=bz) for b in range(4) for bz in range(3)]
+    ok += [ax.moore_params(8, boundary=b) for b in (AX_BC_PERIODIC, AX_BC_FIXED, AX_BC_NEUMANN)]
+    ok += [ax.global_params(8, boundary=b) for b in (AX_BC_PERIODIC, AX_BC_NEUMANN)]
+    n = ctypes.c_size_t()
+    for p in ok:
+        assert ax.lib.ax_state_size(ctypes.byref(p), None, ctypes.byref(n)) == 0, (p.model, p.boundary, p.boundary_z)
+    bad = [ax.ah_params(8, boundary_z=AX_BC_ABSORBING), ax.ah_params(8, boundary=4),
+           ax.ah_params(8, nz=9, boundary_z=AX_BC_NEUMANN),          # z Neumann needs nz <= nx
+           ax.moore_params(8, boundary=AX_BC_ABSORBING), ax.global_params(8, boundary=AX_BC_FIXED)]
+    m = ax.moore_params(8, boundary=AX_BC_NEUMANN)
+    m.boundary_z = AX_BC_PERIODIC
+    g = ax.global_params(8)
+    g.boundary_z = AX_BC_FIXED
+    for p in bad + [m, g]:
+        assert ax.lib.ax_state_size(ctypes.byref(p), None, ctypes.byref(n)) == -1, (p.model, p.boundary, p.boundary_z)

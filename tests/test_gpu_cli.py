@@ -61,14 +61,19 @@ def consts_ah(p, fx, extra):
             f"scaling={p.scaling!r}", *extra]
 
 
-def test_evolution_program(tmp_path):
-    fx, p = load("evo_ah_g1_shared")
+BC_NAME = {0: "periodic", 1: "fixed", 2: "neumann", 3: "absorbing"}
+
+
+@pytest.mark.parametrize("name", ["evo_ah_g1_shared", "evo_bc_absorbing_zfixed", "evo_bc_neumann_z"])
+def test_evolution_program(tmp_path, name):
+    fx, p = load(name)
     d = str(tmp_path)
     os.makedirs(os.path.join(d, "Data"), exist_ok=True)
     st = orc.init_random_phase(p, fx["seed"])
     write_ic(os.path.join(d, "Data", "ic.txt"), st, p, 5)
     run(["--program", "evolution", "--tag", "golden",
-         *consts_ah(p, fx, ["saveFreq=5", "stationary_ic=true", "finalOut=true"])], d)
+         *consts_ah(p, fx, ["saveFreq=5", "stationary_ic=true", "finalOut=true", f"xyBC={BC_NAME[p.boundary]}",
+                            f"zBC={BC_NAME[p.boundary_z]}"])], d)
     vals = np.loadtxt(os.path.join(d, "Data", "valsPerLoop_golden.txt"))
     assert vals.shape == (fx["nt"], 3)
     for n in range(fx["nt"]):
@@ -82,13 +87,15 @@ def test_evolution_program(tmp_path):
     assert sha(fin) == fx["final_sha256"]
 
 
-def test_moore_program(tmp_path):
-    fx, p = load("moore_shared")
+@pytest.mark.parametrize("name", ["moore_shared", "moore_neumann"])
+def test_moore_program(tmp_path, name):
+    fx, p = load(name)
     d = str(tmp_path)
     os.makedirs(os.path.join(d, "Data"), exist_ok=True)
     st = orc.init_random_phase(p, fx["seed"])
     write_ic(os.path.join(d, "Data", "moore_ic.txt"), st, p, 7)
-    run(["--program", "moore", f"nx={p.nx}", f"ny={p.ny}", f"nz={p.nz}", f"nt={fx['nt']}", "makeGif=false"], d)
+    run(["--program", "moore", f"nx={p.nx}", f"ny={p.ny}", f"nz={p.nz}", f"nt={fx['nt']}", "makeGif=false",
+         f"xyBC={BC_NAME[p.boundary]}"], d)
     vals = np.loadtxt(os.path.join(d, "Data", "valsPerLoop.txt"))
     for n in range(fx["nt"]):
         assert rel(vals[n, 0], fx["energy"][n]) < 1e-12, n
@@ -165,7 +172,7 @@ def test_straight_string_pipeline(tmp_path):
     os.replace(os.path.join(d, "ic.txt"), os.path.join(d, "Data", "ic.txt"))
     run(["--program", "evolution", "--tag", "s", f"nx={nx}", f"ny={ny}", f"nz={nz}", f"dx={dx!r}", f"dy={dy!r}",
          f"dz={dz!r}", "dt=0.05", "nt=6", f"g={fx['g']!r}", "lambda=2", "xyBC=fixed", "damped_nt=0",
-         "alpha=0", "saveFreq=5", "finalOut=true"], d)
+         "alpha=0", "saveFreq=5", "finalOut=true", "zBC=fixed"], d)
     fin = np.loadtxt(os.path.join(d, "Data", "finalField.txt"))
     p = ah_params(nx, ny=ny, nz=nz, dx=dx, dy=dy, dz=dz, dt=0.05, lam=2.0, g=fx["g"], theta_damp_form=1,
                   boundary=1)

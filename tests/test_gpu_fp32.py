@@ -11,7 +11,7 @@ import numpy as np
 import pytest
 
 import axions_amd as ax
-from axions_amd.abi import AX_BC_FIXED
+from axions_amd.abi import AX_BC_ABSORBING, AX_BC_FIXED, AX_BC_NEUMANN
 from oracle import oracle_py as orc
 
 pytestmark = pytest.mark.gpu
@@ -39,6 +39,14 @@ F32 = {
     # site kernel: short rows and fixed edges
     "f32_site_odd": (lambda: ax.ah_params(11, ny=13, nz=17, precision=32), 16, False),
     "f32_site_fixed": (lambda: ax.ah_params(10, boundary=AX_BC_FIXED, precision=32), 16, False),
+    # non-periodic boundaries (site kernel + boundary kernels)
+    "f32_neumann_zneu": (lambda: ax.ah_params(11, ny=9, nz=8, dx=0.6, dz=0.55, g=0.8, alpha=2.0, damped_nt=5,
+                                              dampFac=0.4, boundary=AX_BC_NEUMANN, boundary_z=AX_BC_NEUMANN,
+                                              precision=32), 16, False),
+    "f32_absorbing_zper": (lambda: ax.ah_params(10, ny=12, nz=9, g=1.0, alpha=2.0, damped_nt=5,
+                                                boundary=AX_BC_ABSORBING, precision=32), 16, False),
+    "f32_absorbing_zfixed": (lambda: ax.ah_params(9, ny=8, nz=10, dy=0.45, g=0.7, boundary=AX_BC_ABSORBING,
+                                                  boundary_z=AX_BC_FIXED, precision=32), 16, False),
 }
 
 

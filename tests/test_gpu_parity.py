@@ -14,7 +14,7 @@ import numpy as np
 import pytest
 
 import axions_amd as ax
-from axions_amd.abi import AX_BC_FIXED, AxParams
+from axions_amd.abi import AX_BC_FIXED, AX_BC_PERIODIC, AX_MODEL_MOORE, AxParams
 from oracle import oracle_py as orc
 
 pytestmark = pytest.mark.gpu
@@ -175,7 +175,9 @@ def test_nonfinite_is_flagged():
         assert lat.diagnostics(winding=False)["finite"] == 1
 
 
-GOLDEN_SHARED = [n for n in ("evo_ah_g0", "evo_ah_g1_shared", "evo_ah_aniso", "moore_shared", "evo_c1_shared")
+GOLDEN_SHARED = [n for n in ("evo_ah_g0", "evo_ah_g1_shared", "evo_ah_aniso", "moore_shared", "evo_c1_shared",
+                              "evo_bc_neumann", "evo_bc_neumann_z", "evo_bc_absorbing", "evo_bc_absorbing_zfixed",
+                              "evo_bc_fixed_zper", "evo_bc_fixed_zneu", "moore_neumann")
                  if os.path.exists(os.path.join(GOLDEN, n + ".json"))]
 
 
@@ -196,7 +198,7 @@ def test_gpu_matches_reference_fixture(name):
             d = lat.diagnostics(winding=False)
             assert rel(d["energy"], fx["energy"][n]) < 1e-12, (n, d["energy"], fx["energy"][n])
             G = d["gauss_mean"]
-            if p.boundary == AX_BC_FIXED:
+            if p.model == AX_MODEL_MOORE and p.boundary != AX_BC_PERIODIC:
                 G = G / ((p.nx - 2) * (p.ny - 2) * (p.nz - 2))
             if fx["gauss"][n] != 0:
                 assert rel(G, fx["gauss"][n]) < 1e-12, (n, G, fx["gauss"][n])
