@@ -104,10 +104,11 @@ struct ax_lattice {
   int device = 0;
   hipStream_t stream = nullptr;
   void *state = nullptr;
-  double *part = nullptr;   /* 3 doubles per 256-site block */
+  double *part = nullptr;   /* 3 doubles per reduction block (diag_part_blocks) */
   double *dres = nullptr;   /* small device results */
   double *hres = nullptr;   /* pinned host copy */
   int8_t *wind = nullptr;
+  double *phase = nullptr;  /* forward edge phases [3][V] of the marching diagnostics */
   int *bcount = nullptr;
   long long *boff = nullptr;
   long long *ctotal = nullptr;
@@ -218,7 +219,7 @@ int ax_create(const ax_params *p, ax_lattice **out) {
     return hip_fail(e, "hipStreamCreate");
   }
   const size_t bytes = (size_t)L->ncomp * 2 * L->pitch * L->esize;
-  const size_t nb = (L->V + 255) / 256;
+  const size_t nb = diag_part_blocks(*p);
   if (hipMalloc(&L->state, bytes) != hipSuccess || hipMalloc((void **)&L->part, 3 * nb * sizeof(double)) != hipSuccess ||
       hipMalloc((void **)&L->dres, 16 * sizeof(double)) != hipSuccess ||
       hipHostMalloc((void **)&L->hres, 16 * sizeof(double), hipHostMallocDefault) != hipSuccess) {
@@ -247,6 +248,7 @@ void ax_destroy(ax_lattice *L) {
   if (L->dres) hipFree(L->dres);
   if (L->hres) hipHostFree(L->hres);
   if (L->wind) hipFree(L->wind);
+  if (L->phase) hipFree(L->phase);
   if (L->bcount) hipFree(L->bcount);
   if (L->boff) hipFree(L->boff);
   if (L->ctotal) hipFree(L->ctotal);
@@ -460,6 +462,7 @@ int ax_diagnostics(ax_lattice *L, int32_t want_winding, ax_diag *out) {
   const ax_params &p = L->p;
   const bool glob = p.model == AX_MODEL_GLOBAL;
   hipError_t e;
+  bool march = false;
   if (glob) {
     /* the energy leapfrog.cpp prints for the last TimeStep (its pre-step
      * field is still in the other buffer); none before the first step */
@@ -472,7 +475,11 @@ int ax_diagnostics(ax_lattice *L, int32_t want_winding, ax_diag *out) {
     DiagConsts k = diag_consts(&p);
     int rc = prepare_faces(L);
     if (rc) return rc;
-    if (L->esize == 8)
+    march = use_diag_march(p);
+    if (march && want_winding && ensure_phase_buffer(L) != AX_OK) march = false;
+    if (march)
+      e = launch_march_diag(L, want_winding != 0);
+    else if (L->esize == 8)
       e = launch_diag<double>(k, levels_of<double>(L, L->timestep), L->part, L->dres, L->stream);
     else
       e = launch_diag<float>(k, levels_of<float>(L, L->timestep), L->part, L->dres, L->stream);
@@ -481,8 +488,13 @@ int ax_diagnostics(ax_lattice *L, int32_t want_winding, ax_diag *out) {
   if (want_winding) {
     int rc = ensure_winding_buffer(L);
     if (rc) return rc;
-    rc = launch_winding_field(L, 0, L->wind, L->part, L->dres + 4);
-    if (rc) return rc;
+    if (march) {
+      e = launch_phase_winding(L, L->wind, L->dres + 4);
+      if (e != hipSuccess) return hip_fail(e, "winding launch");
+    } else {
+      rc = launch_winding_field(L, 0, L->wind, L->part, L->dres + 4);
+      if (rc) return rc;
+    }
   }
   e = hipMemcpyAsync(L->hres, L->dres, 8 * sizeof(double), hipMemcpyDeviceToHost, L->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(L->stream);
@@ -515,8 +527,14 @@ int ax_winding_field(ax_lattice *L, int32_t field, int8_t *out, size_t n) {
   if (field < 0 || field > (L->p.model == AX_MODEL_MOORE ? 1 : 0)) return fail(AX_EINVAL, "field %d", field);
   int rc = ensure_winding_buffer(L);
   if (rc) return rc;
-  rc = launch_winding_field(L, field, L->wind, L->part, L->dres + 4);
-  if (rc) return rc;
+  if (use_diag_march(L->p) && ensure_phase_buffer(L) == AX_OK) {
+    hipError_t e = launch_march_diag(L, true);
+    if (e == hipSuccess) e = launch_phase_winding(L, L->wind, L->dres + 4);
+    if (e != hipSuccess) return hip_fail(e, "winding launch");
+  } else {
+    rc = launch_winding_field(L, field, L->wind, L->part, L->dres + 4);
+    if (rc) return rc;
+  }
   hipError_t e = hipMemcpyAsync(out, L->wind, n, hipMemcpyDeviceToHost, L->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(L->stream);
   return e == hipSuccess ? AX_OK : hip_fail(e, "winding readback");
@@ -589,6 +607,54 @@ int prepare_faces(ax_lattice *L) {
   hipError_t e = L->esize == 8 ? launch_neumann_faces<double>(p, L->state, L->pitch, tN, L->stream)
                                : launch_neumann_faces<float>(p, L->state, L->pitch, tN, L->stream);
   return e == hipSuccess ? AX_OK : hip_fail(e, "Neumann faces launch");
+}
+
+bool use_diag_march(const ax_params &p) {
+  return p.model == AX_MODEL_AH && p.boundary == AX_BC_PERIODIC && p.boundary_z == AX_BC_PERIODIC &&
+         (double)p.nx * p.ny * p.nz * (p.precision == 64 ? 8.0 : 4.0) < 4294967296.0 &&
+         getenv("AX_DIAG_SITE_KERNEL") == nullptr;
+}
+
+/* partial-sum blocks any diagnostic pass of this lattice writes */
+size_t diag_part_blocks(const ax_params &p) {
+  const size_t V = (size_t)p.nx * p.ny * p.nz;
+  const size_t site = (V + 255) / 256;
+  const size_t march = (size_t)((p.nz + 63) / 64) * p.ny * ((p.nx + kDiagXseg - 1) / kDiagXseg);
+  return site > march ? site : march;
+}
+
+int ensure_phase_buffer(ax_lattice *L) {
+  if (L->phase) return AX_OK;
+  if (hipMalloc((void **)&L->phase, 3 * L->V * sizeof(double)) != hipSuccess) {
+    hipGetLastError();
+    L->phase = nullptr;
+    return fail(AX_ENOMEM, "edge-phase buffer of %zu bytes", 3 * L->V * sizeof(double));
+  }
+  return AX_OK;
+}
+
+/* energy/Gauss (and, with phases, the edge phases) of the current level */
+hipError_t launch_march_diag(ax_lattice *L, bool phases) {
+  const DiagConsts k = diag_consts(&L->p);
+  double *ph = phases ? L->phase : nullptr;
+  return L->esize == 8
+             ? launch_diag_march<double>(k, levels_of<double>(L, L->timestep), kDiagXseg, ph, L->part, L->dres, L->stream)
+             : launch_diag_march<float>(k, levels_of<float>(L, L->timestep), kDiagXseg, ph, L->part, L->dres, L->stream);
+}
+
+/* windings of the current level from the phases launch_march_diag stored */
+hipError_t launch_phase_winding(ax_lattice *L, int8_t *out, double *res3) {
+  const ax_params &p = L->p;
+  const int tN = (L->timestep + 1) % 2;
+  auto P = [&](int c) { return (size_t)(c * 2 + tN) * L->pitch; };
+  if (L->esize == 8) {
+    const double *b = (const double *)L->state;
+    return launch_wind_phase<double>(p.nx, p.ny, p.nz, b + P(2), b + P(3), b + P(4), L->phase, out, L->part, res3,
+                                     L->stream);
+  }
+  const float *b = (const float *)L->state;
+  return launch_wind_phase<float>(p.nx, p.ny, p.nz, b + P(2), b + P(3), b + P(4), L->phase, out, L->part, res3,
+                                  L->stream);
 }
 
 int ensure_winding_buffer(ax_lattice *L) {

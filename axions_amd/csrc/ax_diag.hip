@@ -18,6 +18,7 @@
  * update of the same TimeStep.
  */
 #include "ax_internal.h"
+#include "ax_march.h"
 #include "ax_wind.h"
 
 namespace ax {
@@ -407,6 +408,204 @@ __global__ void __launch_bounds__(256) cross_write_kernel(int nx, int ny, int nz
   }
 }
 
+/* --------------- marching diagnostics (periodic Abelian-Higgs) -------------
+ * One 64-lane wave per (64-site z chunk, row, x segment) marching along x,
+ * as the step kernel.  Per site it evaluates the energy density and Gauss
+ * residual with the expressions of site_energy_ah (the same per-site bits),
+ * carrying the x-backward field, link and link trig from the previous
+ * plane, and, with PH, the three forward edge phases
+ *   delta_a(s) = arg( conj(phi(s)) exp(-i theta_a(s)) phi(s+a) )
+ * from the same link sincos, stored for wind_phase_kernel.  The winding of
+ * a plaquette needs four edge phases of three sites; computing each edge
+ * once here instead of nine per site in winding_kernel is the saving. */
+
+/* link_phase with the link's sin/cos already at hand (the same arithmetic) */
+__device__ __forceinline__ double link_phase_sc(double cr, double ci, double nr, double ni, double sn, double c) {
+  double wr = c * nr + sn * ni;
+  double wi = c * ni - sn * nr;
+  double re = cr * wr + ci * wi;
+  double im = cr * wi - ci * wr;
+  return ax_atan2(im, re);
+}
+
+template <typename T, bool PH>
+__global__ void __launch_bounds__(64) diag_march_kernel(DiagConsts k, Levels<T> L, TileGrid g, double *phase,
+                                                        double *part) {
+  const int w = (int)blockIdx.x;
+  const int zc = w % g.nw, r = w / g.nw;
+  const int y = r % k.ny, seg = r / k.ny;
+  const int x0 = seg * g.xseg, x1 = min(k.nx, x0 + g.xseg);
+  const int NX = k.nx, NY = k.ny, NZ = k.nz;
+  const int lane = (int)(threadIdx.x & 63);
+  const int zr = zc * 64 + lane;
+  const bool act = zr < NZ;
+  const int z = act ? zr : NZ - 1;
+  const int ym = y == 0 ? NY - 1 : y - 1, yp = y == NY - 1 ? 0 : y + 1;
+  const uint32_t E = (uint32_t)sizeof(T);
+  const uint32_t oz = (uint32_t)z * E;
+  const uint32_t ozm = (uint32_t)(z == 0 ? NZ - 1 : z - 1) * E;
+  const uint32_t ozp = (uint32_t)(z == NZ - 1 ? 0 : z + 1) * E;
+  const uint32_t PLB = (uint32_t)NY * (uint32_t)NZ * E;
+  const uint32_t R0 = (uint32_t)y * NZ * E, RM = (uint32_t)ym * NZ * E, RP = (uint32_t)yp * NZ * E;
+  const T *PR = L.cur[0], *PI = L.cur[1], *TX = L.cur[2], *TY = L.cur[3], *TZ = L.cur[4];
+  const T *OPR = L.prv[0], *OPI = L.prv[1], *OTX = L.prv[2], *OTY = L.prv[3], *OTZ = L.prv[4];
+  auto ld_ = [](const T *b, uint32_t row, uint32_t off) { return (double)ldo(b, row, off); };
+  const double dx = k.dx, dy = k.dy, dz = k.dz, dt = k.dt, gg = k.g;
+  const size_t V = (size_t)NX * NY * NZ;
+
+  /* carried from plane x-1: phi, theta_x (both levels), sin/cos theta_x */
+  const int xm0 = x0 == 0 ? NX - 1 : x0 - 1;
+  const uint32_t bm = (uint32_t)xm0 * PLB;
+  double PRm = ld_(PR, bm + R0, oz), PIm = ld_(PI, bm + R0, oz);
+  double TXm = ld_(TX, bm + R0, oz), oXm = ld_(OTX, bm + R0, oz);
+  double sxm, cxm;
+  ax_sincos(TXm, &sxm, &cxm);
+  const uint32_t b0 = (uint32_t)x0 * PLB;
+  double PRc = ld_(PR, b0 + R0, oz), PIc = ld_(PI, b0 + R0, oz);
+  double X = ld_(TX, b0 + R0, oz), Y = ld_(TY, b0 + R0, oz), Z = ld_(TZ, b0 + R0, oz);
+  double rho_sum = 0, g_sum = 0, bad = 0;
+
+  for (int x = x0; x < x1; ++x) {
+    const int xp = x == NX - 1 ? 0 : x + 1;
+    const uint32_t b = (uint32_t)x * PLB, bn = (uint32_t)xp * PLB;
+    const double PRn = ld_(PR, bn + R0, oz), PIn = ld_(PI, bn + R0, oz);
+    const double TXn = ld_(TX, bn + R0, oz), TYn = ld_(TY, bn + R0, oz), TZn = ld_(TZ, bn + R0, oz);
+    const double oPR = ld_(OPR, b + R0, oz), oPI = ld_(OPI, b + R0, oz);
+    const double oX = ld_(OTX, b + R0, oz), oY = ld_(OTY, b + R0, oz), oZ = ld_(OTZ, b + R0, oz);
+    const double PRyp = ld_(PR, b + RP, oz), PIyp = ld_(PI, b + RP, oz);
+    const double TXyp = ld_(TX, b + RP, oz), TZyp = ld_(TZ, b + RP, oz);
+    const double PRzp = ld_(PR, b + R0, ozp), PIzp = ld_(PI, b + R0, ozp);
+    const double TXzp = ld_(TX, b + R0, ozp), TYzp = ld_(TY, b + R0, ozp);
+    const double PRym = ld_(PR, b + RM, oz), PIym = ld_(PI, b + RM, oz);
+    const double TYym = ld_(TY, b + RM, oz), oYym = ld_(OTY, b + RM, oz);
+    const double PRzm = ld_(PR, b + R0, ozm), PIzm = ld_(PI, b + R0, ozm);
+    const double TZzm = ld_(TZ, b + R0, ozm), oZzm = ld_(OTZ, b + R0, ozm);
+
+    /* site_energy_ah at s, operands by name */
+    const double pr = PRc, pi = PIc;
+    const double mag2 = pr * pr + pi * pi;
+    double sym_, cym_, szm_, czm_;
+    ax_sincos(TYym, &sym_, &cym_);
+    ax_sincos(TZzm, &szm_, &czm_);
+    double vel[2], gx[2], gy[2], gz[2];
+    vel[0] = (pr - oPR) / dt;
+    gx[0] = (pr - cxm * PRm + sxm * PIm) / dx;
+    gy[0] = (pr - cym_ * PRym + sym_ * PIym) / dy;
+    gz[0] = (pr - czm_ * PRzm + szm_ * PIzm) / dz;
+    vel[1] = (pi - oPI) / dt;
+    gx[1] = (pi - cxm * PIm + (-sxm) * PRm) / dx;
+    gy[1] = (pi - cym_ * PIym + (-sym_) * PRym) / dy;
+    gz[1] = (pi - czm_ * PIzm + (-szm_) * PRzm) / dz;
+    const double pxy = X + TYn - TXyp - Y;
+    const double pxz = X + TZn - TXzp - Z;
+    const double pyz = Y + TZyp - TYzp - Z;
+    double tdc, fcont, G = 0, rho;
+    const double vv = mag2 - k.eta * k.eta;
+    if (k.energy_form == 0) {
+      if (gg == 0) {
+        tdc = 0;
+        fcont = 0;
+      } else {
+        double divN = (X - TXm) / (dx * dx) + (Y - TYym) / (dy * dy) + (Z - TZzm) / (dz * dz);
+        double divP = (oX - oXm) / (dx * dx) + (oY - oYym) / (dy * dy) + (oZ - oZzm) / (dz * dz);
+        double divT = (divN - divP) / dt;
+        G = divT / gg - 2 * gg * (pi * vel[0] - pr * vel[1]);
+        double ex = (X - oX) / (gg * dt * dx), ey = (Y - oY) / (gg * dt * dy), ez = (Z - oZ) / (gg * dt * dz);
+        tdc = 0.5 * (ex * ex + ey * ey + ez * ez);
+        double bxy = (1 - ax_cos(pxy)) / ((gg * dx * dy) * (gg * dx * dy));
+        double bxz = (1 - ax_cos(pxz)) / ((gg * dx * dz) * (gg * dx * dz));
+        double byz = (1 - ax_cos(pyz)) / ((gg * dy * dz) * (gg * dy * dz));
+        fcont = bxy + bxz + byz;
+      }
+      double r_ = vel[0] * vel[0] + vel[1] * vel[1] + gx[0] * gx[0] + gx[1] * gx[1] + gy[0] * gy[0] +
+                  gy[1] * gy[1] + gz[0] * gz[0] + gz[1] * gz[1] + tdc + fcont + 0.25 * k.lambda * (vv * vv);
+      rho = dx * dy * dz * r_;
+    } else {
+      double ex = (X - oX) / dt, ey = (Y - oY) / dt, ez = (Z - oZ) / dt;
+      tdc = 0.5 * (ex * ex + ey * ey + ez * ez);
+      double bxy = (1 - ax_cos(pxy)) / ((gg * dx * dy) * (gg * dx * dy));
+      double bxz = (1 - ax_cos(pxz)) / ((gg * dx * dz) * (gg * dx * dz));
+      double byz = (1 - ax_cos(pyz)) / ((gg * dy * dz) * (gg * dy * dz));
+      fcont = bxy + bxz + byz;
+      rho = (vel[0] * vel[0] + vel[1] * vel[1] + gx[0] * gx[0] + gx[1] * gx[1] + gy[0] * gy[0] +
+             gy[1] * gy[1] + gz[0] * gz[0] + gz[1] * gz[1] + tdc + fcont + 0.25 * k.lambda * (vv * vv)) *
+            dx * dy * dz;
+    }
+    double sx, cx;
+    ax_sincos(X, &sx, &cx);
+    if (PH) {
+      double sy, cy, sz, cz;
+      ax_sincos(Y, &sy, &cy);
+      ax_sincos(Z, &sz, &cz);
+      const double dX = link_phase_sc(pr, pi, PRn, PIn, sx, cx);
+      const double dY = link_phase_sc(pr, pi, PRyp, PIyp, sy, cy);
+      const double dZ = link_phase_sc(pr, pi, PRzp, PIzp, sz, cz);
+      if (act) {
+        const size_t s = ((size_t)x * NY + y) * NZ + z;
+        phase[s] = dX;
+        phase[V + s] = dY;
+        phase[2 * V + s] = dZ;
+      }
+    }
+    if (act) {
+      rho_sum += rho;
+      g_sum += fabs(G);
+      const double vals[10] = {PRc, PIc, X, Y, Z, oPR, oPI, oX, oY, oZ};
+#pragma unroll
+      for (int q = 0; q < 10; ++q) bad += isfinite(vals[q]) ? 0.0 : 1.0;
+    }
+    PRm = PRc; PIm = PIc; TXm = X; oXm = oX; sxm = sx; cxm = cx;
+    PRc = PRn; PIc = PIn; X = TXn; Y = TYn; Z = TZn;
+  }
+  const double a = wave_sum(rho_sum), bsum = wave_sum(g_sum), c = wave_sum(bad);
+  if (lane == 0) {
+    part[3 * w] = a;
+    part[3 * w + 1] = bsum;
+    part[3 * w + 2] = c;
+  }
+}
+
+/* Windings of the xy, xz, yz plaquettes at s from stored edge phases: the
+ * arithmetic of site_winding with q = 1 (phases and angles in its order). */
+template <typename T>
+__global__ void __launch_bounds__(256) wind_phase_kernel(int nx, int ny, int nz, const T *tx, const T *ty,
+                                                         const T *tz, const double *phase, int8_t *out,
+                                                         double *part) {
+  __shared__ double sh[4];
+  const int V = nx * ny * nz;
+  double tot[3] = {0, 0, 0};
+  for (int qq = 0; qq < kSitesPerThread; ++qq) {
+    const int s = blockIdx.x * kSitesPerBlock + qq * 256 + threadIdx.x;
+    if (s >= V) break;
+    const int kk = s % nz, r = s / nz, j = r % ny, i = r / ny;
+    const int ip = wrapp(i, nx), jp = wrapp(j, ny), kp = wrapp(kk, nz);
+    auto id = [&](int a, int b, int c) { return (a * ny + b) * nz + c; };
+    const int sa[3] = {id(ip, j, kk), id(i, jp, kk), id(i, j, kp)};
+    const T *th[3] = {tx, ty, tz};
+    const double *D[3] = {phase, phase + V, phase + 2 * (size_t)V};
+    const int pa[3] = {0, 0, 1}, pb[3] = {1, 2, 2};
+#pragma unroll
+    for (int pl = 0; pl < 3; ++pl) {
+      const int a = pa[pl], b = pb[pl];
+      const double th0a = 1.0 * ld(th[a], s), th0b = 1.0 * ld(th[b], s);
+      const double th1ab = 1.0 * ld(th[b], sa[a]), th1ba = 1.0 * ld(th[a], sa[b]);
+      const double P = th0a + th1ab - th1ba - th0b;
+      const double sum = D[a][s] + D[b][sa[a]] - D[a][sa[b]] - D[b][s] + P;
+      const int wv = (int)rint(sum * 0.15915494309189535);
+      out[(size_t)pl * V + s] = (int8_t)wv;
+      tot[pl] += (double)(wv < 0 ? -wv : wv);
+    }
+  }
+  double a = block_sum256(tot[0], sh);
+  double b = block_sum256(tot[1], sh);
+  double c = block_sum256(tot[2], sh);
+  if (threadIdx.x == 0) {
+    part[3 * blockIdx.x] = a;
+    part[3 * blockIdx.x + 1] = b;
+    part[3 * blockIdx.x + 2] = c;
+  }
+}
+
 /* ------------------------------ launchers ------------------------------- */
 template <typename T>
 hipError_t launch_diag(const DiagConsts &k, const Levels<T> &L, double *part, double *out3, hipStream_t st) {
@@ -447,13 +646,42 @@ hipError_t launch_crossings(int nx, int ny, int nz, double dx, double dy, double
   return hipGetLastError();
 }
 
+template <typename T>
+hipError_t launch_diag_march(const DiagConsts &k, const Levels<T> &L, int xseg, double *phase, double *part,
+                             double *out3, hipStream_t st) {
+  TileGrid g;
+  g.xseg = (xseg <= 0 || xseg > k.nx) ? k.nx : xseg;
+  g.nw = (k.nz + 63) / 64;
+  g.work = g.nw * k.ny * ((k.nx + g.xseg - 1) / g.xseg);
+  if (phase)
+    hipLaunchKernelGGL((diag_march_kernel<T, true>), dim3(g.work), dim3(64), 0, st, k, L, g, phase, part);
+  else
+    hipLaunchKernelGGL((diag_march_kernel<T, false>), dim3(g.work), dim3(64), 0, st, k, L, g, phase, part);
+  hipLaunchKernelGGL(sum3_kernel, dim3(1), dim3(1024), 0, st, part, g.work, out3);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_wind_phase(int nx, int ny, int nz, const T *tx, const T *ty, const T *tz, const double *phase,
+                             int8_t *out, double *part, double *out3, hipStream_t st) {
+  const int V = nx * ny * nz;
+  const int nb = (V + kSitesPerBlock - 1) / kSitesPerBlock;
+  hipLaunchKernelGGL((wind_phase_kernel<T>), dim3(nb), dim3(256), 0, st, nx, ny, nz, tx, ty, tz, phase, out, part);
+  hipLaunchKernelGGL(sum3_kernel, dim3(1), dim3(1024), 0, st, part, nb, out3);
+  return hipGetLastError();
+}
+
 #define AX_INST(T)                                                                                          \
   template hipError_t launch_diag<T>(const DiagConsts &, const Levels<T> &, double *, double *, hipStream_t); \
   template hipError_t launch_winding<T>(int, int, int, const T *, const T *, const T *, const T *, const T *,  \
                                         double, int, int, int8_t *, double *, double *, hipStream_t);        \
   template hipError_t launch_crossings<T>(int, int, int, double, double, double, const T *, const T *, int,   \
                                           int, int *, long long *, long long *, double *, long long,          \
-                                          hipStream_t, bool);
+                                          hipStream_t, bool);                                               \
+  template hipError_t launch_diag_march<T>(const DiagConsts &, const Levels<T> &, int, double *, double *,    \
+                                           double *, hipStream_t);                                          \
+  template hipError_t launch_wind_phase<T>(int, int, int, const T *, const T *, const T *, const double *,    \
+                                           int8_t *, double *, double *, hipStream_t);
 AX_INST(double)
 AX_INST(float)
 #undef AX_INST

@@ -12,6 +12,14 @@ struct ax_lattice;
 
 namespace ax {
 
+/* Work items of a marching kernel: one 64-lane wave per (64-site z chunk,
+ * row, x segment of xseg planes), z chunk fastest. */
+struct TileGrid {
+  int nw;      /* z chunks per row */
+  int xseg;    /* x planes marched by one wave */
+  int work;    /* nw * ny * nseg */
+};
+
 struct DiagConsts {
   int nx, ny, nz, ex, ez, model, energy_form, ncomp;
   double dx, dy, dz, dt, g, lambda, eta;
@@ -36,6 +44,9 @@ void lattice_reset_timestep(ax_lattice *L);
 
 int launch_step(ax_lattice *L, int n);
 int ensure_winding_buffer(ax_lattice *L);
+int ensure_phase_buffer(ax_lattice *L);
+hipError_t launch_march_diag(ax_lattice *L, bool phases);
+hipError_t launch_phase_winding(ax_lattice *L, int8_t *out, double *res3);
 int prepare_faces(ax_lattice *L);
 int launch_winding_field(ax_lattice *L, int field, int8_t *out, double *part, double *res3);
 
@@ -56,6 +67,17 @@ hipError_t launch_global_energy(const ax_params &p, const void *state, size_t pi
 
 template <typename T>
 hipError_t launch_diag(const DiagConsts &k, const Levels<T> &L, double *part, double *out3, hipStream_t st);
+/* periodic Abelian-Higgs: energy/Gauss march, with phase != NULL also the
+ * forward edge phases [3][V]; launch_wind_phase turns them into windings */
+constexpr int kDiagXseg = 32;
+bool use_diag_march(const ax_params &p);
+size_t diag_part_blocks(const ax_params &p);
+template <typename T>
+hipError_t launch_diag_march(const DiagConsts &k, const Levels<T> &L, int xseg, double *phase, double *part,
+                             double *out3, hipStream_t st);
+template <typename T>
+hipError_t launch_wind_phase(int nx, int ny, int nz, const T *tx, const T *ty, const T *tz, const double *phase,
+                             int8_t *out, double *part, double *out3, hipStream_t st);
 template <typename T>
 hipError_t launch_winding(int nx, int ny, int nz, const T *fr, const T *fi, const T *tx, const T *ty, const T *tz,
                           double q, int ex, int ez, int8_t *out, double *part, double *out3, hipStream_t st);

@@ -16,12 +16,6 @@
 
 namespace ax {
 
-struct TileGrid {
-  int nw;      /* z chunks per row */
-  int xseg;    /* x planes marched by one wave */
-  int work;    /* nw * ny * nseg */
-};
-
 template <typename R, bool POW2, bool PLAIN>
 __global__ void __launch_bounds__(64) ah_tiled_kernel(MarchConsts<R> k, Levels<R> L, int nx, int ny, int nz,
                                                       TileGrid g) {

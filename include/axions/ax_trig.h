@@ -165,6 +165,15 @@ AX_HD double ax_sin_small(double x) {
   return (x == 0.0) ? x : s;
 }
 
+/* Both members of ax_sincos(x) for ax_sin_is_small(x): with k = 0 the
+ * reduction returns r_hi = x, r_lo = +0 (x - x, -0 - -0 and fma(+0, c, +0)
+ * are +0) and quadrant 0, so sin is ax_sin_small(x) and cos is the cos
+ * kernel at (x, +0), bit for bit. */
+AX_HD void ax_sincos_small(double x, double *s, double *c) {
+  *s = ax_sin_small(x);
+  *c = ax_cos_kernel(x, 0.0);
+}
+
 /* atan kernel tail A(z), z = u^2, |u| <= tan(pi/8) */
 #define AX_A0 -0.3333333333333333
 #define AX_A1 0.19999999999995513
@@ -267,6 +276,16 @@ AX_HD float ax_sinf_small(float x) {
   ps = fmaf(z, ps, -1.6666667e-01f);
   float sk = fmaf(z * x, ps, x);
   return (x == 0.0f) ? x : sk;
+}
+
+/* Both members of ax_sincosf(x) for ax_sinf_is_small(x) (r = x, quadrant 0). */
+AX_HD void ax_sincosf_small(float x, float *s, float *c) {
+  const float z = x * x;
+  *s = ax_sinf_small(x);
+  float pc = fmaf(z, -2.7557319e-07f, 2.4801587e-05f);
+  pc = fmaf(z, pc, -1.3888889e-03f);
+  pc = fmaf(z, pc, 4.1666668e-02f);
+  *c = fmaf(z * z, pc, fmaf(-0.5f, z, 1.0f));
 }
 
 #endif /* AXIONS_AX_TRIG_H */

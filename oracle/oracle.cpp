@@ -1350,7 +1350,8 @@ int orc_crossings(const ax_params *p, const double *st, int n, double *xyz, size
 
 /* fn: 0 sin, 1 cos, 2 atan2(x, y), 3 sinf, 4 cosf, 5 ax_sin_small (NaN where
  * ax_sin_is_small(x) is false), 6 ax_sinf_small (NaN where ax_sinf_is_small
- * is false) */
+ * is false), 7 / 8 the cos member of ax_sincos_small / ax_sincosf_small
+ * (NaN outside their domains) */
 void orc_trig(int fn, const double *x, const double *y, double *out, size_t n) {
   for (size_t i = 0; i < n; ++i) {
     switch (fn) {
@@ -1359,6 +1360,18 @@ void orc_trig(int fn, const double *x, const double *y, double *out, size_t n) {
       case 2: out[i] = ax_atan2(x[i], y[i]); break;
       case 5: out[i] = ax_sin_is_small(x[i]) ? ax_sin_small(x[i]) : NAN; break;
       case 6: out[i] = ax_sinf_is_small((float)x[i]) ? (double)ax_sinf_small((float)x[i]) : NAN; break;
+      case 7: {
+        double sv, cv;
+        ax_sincos_small(x[i], &sv, &cv);
+        out[i] = ax_sin_is_small(x[i]) ? cv : NAN;
+        break;
+      }
+      case 8: {
+        float sv, cv;
+        ax_sincosf_small((float)x[i], &sv, &cv);
+        out[i] = ax_sinf_is_small((float)x[i]) ? (double)cv : NAN;
+        break;
+      }
       case 3: { float s, c; ax_sincosf((float)x[i], &s, &c); out[i] = s; break; }
       default: { float s, c; ax_sincosf((float)x[i], &s, &c); out[i] = c; break; }
     }

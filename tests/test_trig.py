@@ -123,3 +123,36 @@ def test_sinf_small_form_is_bit_identical():
     assert np.array_equal(small[ok].view(np.uint64), full.view(np.uint64))
     kpick = np.rint(x.astype(np.float32) * np.float32(2 / math.pi)) + np.float32(0.0)
     assert np.array_equal(ok, kpick == 0)
+
+
+def _small_domain_points(lim, rng, n=400000):
+    x = np.concatenate([rng.uniform(-lim, lim, n), rng.standard_normal(20000) * 1e-3,
+                        10.0 ** rng.uniform(-320, 0, 20000) * rng.choice([-1, 1], 20000),
+                        np.array([0.0, -0.0, 5e-324, -5e-324, 1e-200])])
+    edge = [lim]
+    lo = hi = lim
+    for _ in range(200):
+        lo, hi = np.nextafter(lo, 0), np.nextafter(hi, 2)
+        edge += [lo, hi]
+    edge = np.array(edge)
+    return np.concatenate([x, edge, -edge])
+
+
+def test_cos_small_form_is_bit_identical():
+    """The cos member of the k = 0 sincos (the link fast path of the marching
+    kernel) equals ax_cos bit for bit wherever ax_sin_is_small holds."""
+    x = _small_domain_points(0.5 * math.pi / 2, np.random.default_rng(23))
+    small = orc.trig(7, x)
+    ok = ~np.isnan(small)
+    assert ok.sum() > 400000
+    assert np.array_equal(small[ok].view(np.uint64), orc.trig(1, x[ok]).view(np.uint64))
+    assert np.array_equal(ok, (np.rint(x * (2 / math.pi)) + 0.0) == 0)
+
+
+def test_cosf_small_form_is_bit_identical():
+    """Same for binary32: ax_sincosf_small's cos equals ax_sincosf's cos."""
+    x = _small_domain_points(0.5 * math.pi / 2, np.random.default_rng(29)).astype(np.float32).astype(np.float64)
+    small = orc.trig(8, x)
+    ok = ~np.isnan(small)
+    assert ok.sum() > 400000
+    assert np.array_equal(small[ok], orc.trig(4, x[ok]))
