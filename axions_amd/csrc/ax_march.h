@@ -169,6 +169,27 @@ struct RegOps {
   }
 };
 
+/* The value of lane-1 (lane 0: `edge`) and of lane+1 (lane 63: `edge`),
+ * by DPP wave shifts; every lane of the wave must be active. */
+__device__ __forceinline__ double lane_prev(double v, double edge) {
+  const long long b = __double_as_longlong(v), e = __double_as_longlong(edge);
+  const int lo = __builtin_amdgcn_update_dpp((int)e, (int)b, 0x138, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp((int)(e >> 32), (int)(b >> 32), 0x138, 0xF, 0xF, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+__device__ __forceinline__ double lane_next(double v, double edge) {
+  const long long b = __double_as_longlong(v), e = __double_as_longlong(edge);
+  const int lo = __builtin_amdgcn_update_dpp((int)e, (int)b, 0x130, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp((int)(e >> 32), (int)(b >> 32), 0x130, 0xF, 0xF, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+__device__ __forceinline__ float lane_prev(float v, float edge) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(edge), __float_as_int(v), 0x138, 0xF, 0xF, false));
+}
+__device__ __forceinline__ float lane_next(float v, float edge) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(edge), __float_as_int(v), 0x130, 0xF, 0xF, false));
+}
+
 /* Keeps the compiler from moving instructions across phase boundaries, so
  * operands fetched on demand (LDS) are not all hoisted to the top. */
 template <bool PHASED>
@@ -322,7 +343,16 @@ __device__ __forceinline__ void ah_march(const MarchConsts<R> &k, const Levels<R
   R *OPR = L.prv[0], *OPI = L.prv[1], *OTX = L.prv[2], *OTY = L.prv[3], *OTZ = L.prv[4];
   const bool plain = PLAIN == 1 || (PLAIN == 2 && k.plain);
 
-  auto load_plane = [&](int x, PlaneOps<R> &d) {
+  /* FP32: z+-1 operands come from the neighbouring lanes (DPP) instead of
+   * reloading shifted rows; only the two chunk-edge lanes load: lane 0 the
+   * site before the chunk, lane `last` (the last active lane) the site after
+   * it, both in the same instructions.  Requires last >= 1 (z tails of 0 or
+   * >= 32 sites, as the callers guarantee). */
+  const int z0 = zc * 64;
+  const int last = min(63, NZ - 1 - z0);
+  const bool e0 = lane == 0, eL = lane == last;
+  const uint32_t oe = (uint32_t)(e0 ? (z0 == 0 ? NZ - 1 : z0 - 1) : (z0 + last + 1 == NZ ? 0 : z0 + last + 1)) * E;
+  auto load_plane = [&](int x, PlaneOps<R> &d, R PRc_, R PIc_, R TXc_, R TYc_, R TZc_) {
     const int xp = x == NX - 1 ? 0 : x + 1;
     const uint32_t b = (uint32_t)x * PLB, bn = (uint32_t)xp * PLB;
     d.PRn = ldo(PR, bn + R0, oz); d.PIn = ldo(PI, bn + R0, oz);
@@ -331,14 +361,39 @@ __device__ __forceinline__ void ah_march(const MarchConsts<R> &k, const Levels<R
     d.oX = ldo(OTX, b + R0, oz); d.oY = ldo(OTY, b + R0, oz); d.oZ = ldo(OTZ, b + R0, oz);
     d.PRyp = ldo(PR, b + RP, oz); d.PIyp = ldo(PI, b + RP, oz);
     d.PRym = ldo(PR, b + RM, oz); d.PIym = ldo(PI, b + RM, oz);
-    d.PRzp = ldo(PR, b + R0, ozp); d.PIzp = ldo(PI, b + R0, ozp);
-    d.PRzm = ldo(PR, b + R0, ozm); d.PIzm = ldo(PI, b + R0, ozm);
     d.TXyp = ldo(TX, b + RP, oz); d.TZyp = ldo(TZ, b + RP, oz);
-    d.TXzp = ldo(TX, b + R0, ozp); d.TYzp = ldo(TY, b + R0, ozp);
     d.TXym = ldo(TX, b + RM, oz); d.TYym = ldo(TY, b + RM, oz); d.TZym = ldo(TZ, b + RM, oz);
-    d.TYxpym = ldo(TY, bn + RM, oz); d.TYymzp = ldo(TY, b + RM, ozp);
-    d.TXzm = ldo(TX, b + R0, ozm); d.TYzm = ldo(TY, b + R0, ozm); d.TZzm = ldo(TZ, b + R0, ozm);
-    d.TZxpzm = ldo(TZ, bn + R0, ozm); d.TZypzm = ldo(TZ, b + RP, ozm);
+    d.TYxpym = ldo(TY, bn + RM, oz);
+    if (sizeof(R) == 8) {
+      /* FP64 is issue-bound: the shifts' extra VALU costs more than the
+       * reloads (L1 hits) they save; FP32 gains from the fewer requests */
+      d.PRzp = ldo(PR, b + R0, ozp); d.PIzp = ldo(PI, b + R0, ozp);
+      d.PRzm = ldo(PR, b + R0, ozm); d.PIzm = ldo(PI, b + R0, ozm);
+      d.TXzp = ldo(TX, b + R0, ozp); d.TYzp = ldo(TY, b + R0, ozp);
+      d.TYymzp = ldo(TY, b + RM, ozp);
+      d.TXzm = ldo(TX, b + R0, ozm); d.TYzm = ldo(TY, b + R0, ozm); d.TZzm = ldo(TZ, b + R0, ozm);
+      d.TZxpzm = ldo(TZ, bn + R0, ozm); d.TZypzm = ldo(TZ, b + RP, ozm);
+      return;
+    }
+    R ePR = PRc_, ePI = PIc_, eTX = TXc_, eTY = TYc_, eTZ = TZc_, eTZxp = TZc_, eTZyp = TZc_, eTYym = TYc_;
+    if (e0 || eL) {
+      ePR = ldo(PR, b + R0, oe); ePI = ldo(PI, b + R0, oe);
+      eTX = ldo(TX, b + R0, oe); eTY = ldo(TY, b + R0, oe);
+    }
+    if (e0) {
+      eTZ = ldo(TZ, b + R0, oe); eTZxp = ldo(TZ, bn + R0, oe); eTZyp = ldo(TZ, b + RP, oe);
+    }
+    if (eL) eTYym = ldo(TY, b + RM, oe);
+    d.PRzm = lane_prev(PRc_, ePR); d.PIzm = lane_prev(PIc_, ePI);
+    d.TXzm = lane_prev(TXc_, eTX); d.TYzm = lane_prev(TYc_, eTY); d.TZzm = lane_prev(TZc_, eTZ);
+    d.TZxpzm = lane_prev(d.TZn, eTZxp); d.TZypzm = lane_prev(d.TZyp, eTZyp);
+    /* the shifts run with every lane active (a DPP read of an inactive lane
+     * returns the old operand), then the edge lane takes its own load */
+    const R nPR = lane_next(PRc_, ePR), nPI = lane_next(PIc_, ePI), nTX = lane_next(TXc_, eTX);
+    const R nTY = lane_next(TYc_, eTY), nTYym = lane_next(d.TYym, eTYym);
+    d.PRzp = eL ? ePR : nPR; d.PIzp = eL ? ePI : nPI;
+    d.TXzp = eL ? eTX : nTX; d.TYzp = eL ? eTY : nTY;
+    d.TYymzp = eL ? eTYym : nTYym;
   };
 
   /* ---- prologue: plane x0-1 quantities needed by plane x0 ---- */
@@ -354,7 +409,7 @@ __device__ __forceinline__ void ah_march(const MarchConsts<R> &k, const Levels<R
 
   for (int x = x0; x < x1; ++x) {
     PlaneOps<R> D;
-    load_plane(x, D);
+    load_plane(x, D, PRc, PIc, TXc, TYc, TZc);
     const uint32_t so = (uint32_t)x * PLB + R0;
     SiteOut<R> o;
     ah_site_update<R, POW2, PHASED>(k, plain, RegOps<R>{D, PRc, PIc, TXc, TYc, TZc}, m, o);
