@@ -327,11 +327,13 @@ __device__ __forceinline__ void march_prologue(Get G, int xm0, int x0, MarchCarr
  * compile time, 2 decided at run time from k.plain (wave-uniform). */
 template <typename R, bool POW2, int PLAIN, bool PHASED = false>
 __device__ __forceinline__ void ah_march(const MarchConsts<R> &k, const Levels<R> &L, int NX, int NY, int NZ,
-                                         int y, int zc, int x0, int x1) {
+                                         int y, int zc, int x0, int x1, bool zfix = false) {
   const int lane = (int)(threadIdx.x & 63);
   const int zr = zc * 64 + lane;
-  const bool act = zr < NZ;
-  const int z = act ? zr : NZ - 1;
+  const bool inz = zr < NZ;
+  const int z = inz ? zr : NZ - 1;
+  /* zfix: z not periodic, the z = 0 and NZ-1 sites are not stepped */
+  const bool act = inz && !(zfix && (zr == 0 || zr == NZ - 1));
   const int ym = y == 0 ? NY - 1 : y - 1, yp = y == NY - 1 ? 0 : y + 1;
   const uint32_t E = (uint32_t)sizeof(R);
   const uint32_t oz = (uint32_t)z * E;

@@ -32,12 +32,13 @@ inline void moore_consts_from(const StepConsts &k, MooreConsts<R> &m) {
 
 template <typename R, bool POW2>
 __device__ __forceinline__ void moore_march(const MooreConsts<R> &k, const Levels<R> &L, int NX, int NY, int NZ, int y,
-                                            int zc, int x0, int x1) {
+                                            int zc, int x0, int x1, bool zfix = false) {
   typedef TrigR<R> T;
   const int lane = (int)(threadIdx.x & 63);
   const int zr = zc * 64 + lane;
-  const bool act = zr < NZ;
-  const int z = act ? zr : NZ - 1;
+  const bool inz = zr < NZ;
+  const int z = inz ? zr : NZ - 1;
+  const bool act = inz && !(zfix && (zr == 0 || zr == NZ - 1));
   const int ym = y == 0 ? NY - 1 : y - 1, yp = y == NY - 1 ? 0 : y + 1;
   const uint32_t E = (uint32_t)sizeof(R);
   const uint32_t oz = (uint32_t)z * E;

@@ -16,28 +16,43 @@
 
 namespace ax {
 
+/* The planes [x0, x1) and row y a work item steps; with non-periodic x/y
+ * (ex) the boundary planes and rows are left alone (their neighbours would
+ * wrap), evolution.cpp:161-186.  Returns false for an empty item. */
+__device__ __forceinline__ bool tile_range(const TileGrid &g, int nx, int ny, int ex, int w, int &zc, int &y,
+                                           int &x0, int &x1) {
+  zc = w % g.nw;
+  const int r = w / g.nw;
+  y = r % ny;
+  const int seg = r / ny;
+  x0 = seg * g.xseg;
+  x1 = min(nx, x0 + g.xseg);
+  if (ex) {
+    x0 = max(x0, 1);
+    x1 = min(x1, nx - 1);
+    if (y == 0 || y == ny - 1) return false;
+  }
+  return x0 < x1;
+}
+
 template <typename R, bool POW2, bool PLAIN>
 __global__ void __launch_bounds__(64) ah_tiled_kernel(MarchConsts<R> k, Levels<R> L, int nx, int ny, int nz,
-                                                      TileGrid g) {
+                                                      TileGrid g, int ex, int ez) {
   const int w = (int)blockIdx.x;
   if (w >= g.work) return;
-  const int zc = w % g.nw, r = w / g.nw;
-  const int y = r % ny, seg = r / ny;
-  const int x0 = seg * g.xseg;
-  const int x1 = min(nx, x0 + g.xseg);
-  ah_march<R, POW2, PLAIN ? 1 : 0>(k, L, nx, ny, nz, y, zc, x0, x1);
+  int zc, y, x0, x1;
+  if (!tile_range(g, nx, ny, ex, w, zc, y, x0, x1)) return;
+  ah_march<R, POW2, PLAIN ? 1 : 0>(k, L, nx, ny, nz, y, zc, x0, x1, ez != 0);
 }
 
 template <typename R, bool POW2>
 __global__ void __launch_bounds__(64) moore_tiled_kernel(MooreConsts<R> k, Levels<R> L, int nx, int ny, int nz,
-                                                         TileGrid g) {
+                                                         TileGrid g, int ex, int ez) {
   const int w = (int)blockIdx.x;
   if (w >= g.work) return;
-  const int zc = w % g.nw, r = w / g.nw;
-  const int y = r % ny, seg = r / ny;
-  const int x0 = seg * g.xseg;
-  const int x1 = min(nx, x0 + g.xseg);
-  moore_march<R, POW2>(k, L, nx, ny, nz, y, zc, x0, x1);
+  int zc, y, x0, x1;
+  if (!tile_range(g, nx, ny, ex, w, zc, y, x0, x1)) return;
+  moore_march<R, POW2>(k, L, nx, ny, nz, y, zc, x0, x1, ez != 0);
 }
 
 template <typename R>
@@ -50,8 +65,8 @@ hipError_t launch_step_tiled_moore(const StepConsts &k, const Levels<R> &L, int 
   moore_consts_from<R>(k, m);
   const bool pow2 = is_pow2(k.dx2) && is_pow2(k.dy2) && is_pow2(k.dz2);
   dim3 grid(g.work), block(64);
-  if (pow2) hipLaunchKernelGGL((moore_tiled_kernel<R, true>), grid, block, 0, st, m, L, k.nx, k.ny, k.nz, g);
-  else hipLaunchKernelGGL((moore_tiled_kernel<R, false>), grid, block, 0, st, m, L, k.nx, k.ny, k.nz, g);
+  if (pow2) hipLaunchKernelGGL((moore_tiled_kernel<R, true>), grid, block, 0, st, m, L, k.nx, k.ny, k.nz, g, k.ex, k.ez);
+  else hipLaunchKernelGGL((moore_tiled_kernel<R, false>), grid, block, 0, st, m, L, k.nx, k.ny, k.nz, g, k.ex, k.ez);
   return hipGetLastError();
 }
 
@@ -60,8 +75,7 @@ template hipError_t launch_step_tiled_moore<float>(const StepConsts &, const Lev
 
 bool use_tiled_ah(const ax_params &p) {
   const int tail = p.nz % 64;
-  return (p.model == AX_MODEL_AH || p.model == AX_MODEL_MOORE) && p.boundary == AX_BC_PERIODIC &&
-         p.boundary_z == AX_BC_PERIODIC &&
+  return (p.model == AX_MODEL_AH || p.model == AX_MODEL_MOORE) &&
          (tail == 0 || tail >= 32) &&
          p.nx >= 3 && p.ny >= 3 &&
          (double)p.nx * p.ny * p.nz * (p.precision == 64 ? 8.0 : 4.0) < 4294967296.0 &&
@@ -87,10 +101,13 @@ hipError_t launch_step_tiled_ah(const StepConsts &k, const Levels<R> &L, int xse
   march_consts_from<R>(k, rate_limit<R>((R)k.dt), m);
   const bool pow2 = is_pow2(k.dx2) && is_pow2(k.dy2) && is_pow2(k.dz2);
   dim3 grid(g.work), block(64);
-  if (pow2 && m.plain) hipLaunchKernelGGL((ah_tiled_kernel<R, true, true>), grid, block, 0, st, m, L, k.nx, k.ny, k.nz, g);
-  else if (pow2) hipLaunchKernelGGL((ah_tiled_kernel<R, true, false>), grid, block, 0, st, m, L, k.nx, k.ny, k.nz, g);
-  else if (m.plain) hipLaunchKernelGGL((ah_tiled_kernel<R, false, true>), grid, block, 0, st, m, L, k.nx, k.ny, k.nz, g);
-  else hipLaunchKernelGGL((ah_tiled_kernel<R, false, false>), grid, block, 0, st, m, L, k.nx, k.ny, k.nz, g);
+#define AX_AHL(P2, PL) \
+  hipLaunchKernelGGL((ah_tiled_kernel<R, P2, PL>), grid, block, 0, st, m, L, k.nx, k.ny, k.nz, g, k.ex, k.ez)
+  if (pow2 && m.plain) AX_AHL(true, true);
+  else if (pow2) AX_AHL(true, false);
+  else if (m.plain) AX_AHL(false, true);
+  else AX_AHL(false, false);
+#undef AX_AHL
   return hipGetLastError();
 }
 

@@ -11,7 +11,7 @@ import numpy as np
 import pytest
 
 import axions_amd as ax
-from axions_amd.abi import AX_BC_ABSORBING, AX_BC_FIXED, AX_BC_NEUMANN
+from axions_amd.abi import AX_BC_ABSORBING, AX_BC_FIXED, AX_BC_NEUMANN, AX_BC_PERIODIC
 from oracle import oracle_py as orc
 
 pytestmark = pytest.mark.gpu
@@ -47,6 +47,14 @@ F32 = {
                                                 boundary=AX_BC_ABSORBING, precision=32), 16, False),
     "f32_absorbing_zfixed": (lambda: ax.ah_params(9, ny=8, nz=10, dy=0.45, g=0.7, boundary=AX_BC_ABSORBING,
                                                   boundary_z=AX_BC_FIXED, precision=32), 16, False),
+    # the same boundaries on the marching kernel (DPP z shifts, edge masks)
+    "f32_march_neumann_zneu": (lambda: ax.ah_params(66, ny=9, nz=64, g=0.8, alpha=2.0, damped_nt=5,
+                                                    boundary=AX_BC_NEUMANN, boundary_z=AX_BC_NEUMANN,
+                                                    precision=32), 12, True),
+    "f32_march_absorbing_zfixed": (lambda: ax.ah_params(10, ny=9, nz=96, g=0.7, boundary=AX_BC_ABSORBING,
+                                                        boundary_z=AX_BC_FIXED, precision=32), 12, True),
+    "f32_march_fixed_zper": (lambda: ax.ah_params(12, ny=8, nz=100, boundary=AX_BC_FIXED,
+                                                  boundary_z=AX_BC_PERIODIC, precision=32), 12, True),
 }
 
 

@@ -14,7 +14,8 @@ import numpy as np
 import pytest
 
 import axions_amd as ax
-from axions_amd.abi import AX_BC_FIXED, AX_BC_PERIODIC, AX_MODEL_MOORE, AxParams
+from axions_amd.abi import (AX_BC_ABSORBING, AX_BC_FIXED, AX_BC_NEUMANN, AX_BC_PERIODIC, AX_MODEL_MOORE,
+                             AxParams)
 from oracle import oracle_py as orc
 
 pytestmark = pytest.mark.gpu
@@ -61,6 +62,19 @@ CASES = {
     "moore_tiled_64": (lambda: ax.moore_params(20, ny=12, nz=64, dt=0.2), 12),
     "moore_tiled_96_aniso": (lambda: ax.moore_params(9, ny=8, nz=96, dx=0.6, dt=0.17), 10),
     "moore_fixed_12": (lambda: ax.moore_params(12, boundary=AX_BC_FIXED), 25),
+    # non-periodic boundaries on the marching kernel (nz tail 0 or >= 32):
+    # interior planes/rows/z sites stepped, faces by the boundary kernels
+    "march_fixed_zper": (lambda: ax.ah_params(14, ny=11, nz=64, alpha=2.0, damped_nt=5, boundary=AX_BC_FIXED,
+                                              boundary_z=AX_BC_PERIODIC), 10),
+    "march_neumann_zneu": (lambda: ax.ah_params(70, ny=9, nz=64, g=0.8, alpha=2.0, damped_nt=5, dampFac=0.4,
+                                                theta_damp_form=1, boundary=AX_BC_NEUMANN,
+                                                boundary_z=AX_BC_NEUMANN), 10),
+    "march_absorbing_zfixed": (lambda: ax.ah_params(13, ny=12, nz=96, dy=0.45, g=1.0, alpha=2.0, damped_nt=5,
+                                                    boundary=AX_BC_ABSORBING, boundary_z=AX_BC_FIXED), 10),
+    "march_absorbing_zper": (lambda: ax.ah_params(11, ny=10, nz=128, boundary=AX_BC_ABSORBING,
+                                                  boundary_z=AX_BC_PERIODIC), 10),
+    "moore_march_fixed": (lambda: ax.moore_params(12, ny=10, nz=64, dt=0.2, boundary=AX_BC_FIXED), 10),
+    "moore_march_neumann": (lambda: ax.moore_params(11, ny=12, nz=96, dt=0.2, boundary=AX_BC_NEUMANN), 10),
 }
 
 
