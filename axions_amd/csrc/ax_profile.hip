@@ -37,7 +37,8 @@
 
 namespace ax {
 
-constexpr int SOR_WAVES = 4;                 /* one wave per SIMD of a CU */
+constexpr int SOR_WAVES = 16;                /* four waves per SIMD hide the divide latency
+                                                (30001 points: 1.37 s with 4 waves, 1.04 s with 16) */
 constexpr int SOR_LANES = 64 * SOR_WAVES;    /* sweeps in flight per launch */
 constexpr int SOR_PT = 64;                   /* time steps per phase */
 constexpr int SOR_RING = 3 * SOR_PT;         /* slots: the phase written, and the two read */

@@ -159,6 +159,74 @@ def run_lattice(ax, n, precision, steps, warmup, diag_every, seed, log, moore=Fa
     }
 
 
+def boundary_leg(ax, n, steps, seed, log):
+    """evolution.cpp's own schedule at scale: global string (g = 0), lambda = 1,
+    dt = 0.05, PRS friction alpha = 3 with damping 0.5 for the first 200
+    steps, absorbing x/y faces, periodic z (evolution.cpp:23-53)."""
+    import time as _t
+    from axions_amd.abi import AX_BC_ABSORBING, AX_BC_PERIODIC
+    p = ax.ah_params(n, dx=0.5, dt=0.05, lam=1.0, eta=1.0, g=0.0, alpha=3.0, scaling=1.0, damped_nt=200,
+                     dampFac=0.5, theta_damp_form=1, boundary=AX_BC_ABSORBING, boundary_z=AX_BC_PERIODIC)
+    log(f"boundary leg: {n}^3 absorbing x/y, PRS alpha=3, damping to step 200, {steps} steps")
+    with ax.Lattice(p) as lat:
+        lat.init_random_phase(seed)
+        lat.step(2)
+        lat.sync()
+        lat.kernel_timer(True)
+        lat.kernel_time()
+        t0 = _t.perf_counter()
+        done = 0
+        while done < steps:
+            if (lat.timestep % 20) == 0:
+                d = lat.diagnostics(winding=False)
+                if not d["finite"]:
+                    raise RuntimeError(f"boundary leg: non-finite state at step {lat.timestep}")
+            k = min(20 - lat.timestep % 20, steps - done)
+            lat.step(k)
+            done += k
+        lat.sync()
+        dt = _t.perf_counter() - t0
+        kms, ksteps = lat.kernel_time()
+        lat.kernel_timer(False)
+        d = lat.diagnostics(winding=False)
+    kernel_s = kms / 1e3 / max(ksteps, 1)
+    achieved = 120 * n ** 3 / kernel_s / 1e9
+    return {"workload": f"evolution.cpp schedule {n}^3 FP64: g=0, lambda=1, dt=0.05, alpha=3 (PRS), "
+                        f"damped_nt=200, absorbing x/y, periodic z, {steps} steps from step 2, energy every 20",
+            "value": n ** 3 * steps / dt, "unit": "site-updates/s", "ms_per_step": dt / steps * 1e3,
+            "kernel_ms": kernel_s * 1e3,
+            "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": achieved / HBM_PEAK_GBS, "bytes_per_site": 120},
+            "final_energy": d["energy"], "finite": bool(d["finite"])}
+
+
+def sor_leg(ax, cpu_sweeps, log):
+    """singleSemiAnalytic.cpp's straight-string profile at its own constants
+    (30001 points, h = 0.01, w = 1.5, tol = 1e-3) on the GPU, and the oracle's
+    one-thread sweeps on a bounded sample of the same problem."""
+    import time as _t
+    log("sor leg: singleSemiAnalytic.cpp constants")
+    p = ax.sor_params()
+    ax.sor_profile(ax.sor_params(nx=301, max_iter=10))           # load / warm up
+    t0 = _t.perf_counter()
+    prof, it, res = ax.sor_profile(p)
+    dt = _t.perf_counter() - t0
+    out = {"workload": "straight-string SOR profile, singleSemiAnalytic.cpp constants (nx=30001, h=0.01, w=1.5, "
+                       "tol=1e-3, n=1, lambda=1, eta=1, g=0)",
+           "sweeps": it, "last_residual": res, "gpu_seconds": dt,
+           "value": p.nx * it / dt, "unit": "point-updates/s"}
+    if cpu_sweeps:
+        from oracle import oracle_py as orc
+        t0 = _t.perf_counter()
+        orc.sor_profile(p.nx, p.h, p.w, 0.0, p.n, p.lambda_, p.eta, p.g, cpu_sweeps)
+        cdt = _t.perf_counter() - t0
+        out["cpu_baseline"] = {"value": p.nx * cpu_sweeps / cdt, "unit": "point-updates/s", "cores": 1,
+                               "kind": "port",
+                               "sample": f"oracle one-thread SOR (singleSemiAnalytic.cpp:82-134), first "
+                                         f"{cpu_sweeps} sweeps of the same problem, {cdt:.2f} s"}
+    return out
+
+
 def compare_series(a, b):
     """FP32 series b against FP64 series a at the same timesteps."""
     import numpy as np
@@ -319,6 +387,12 @@ def main():
     ap.add_argument("--ensemble-budget-gb", type=float, default=96.0)
     ap.add_argument("--ensemble-chunk", type=int, default=10)
     ap.add_argument("--ensemble-fp64", choices=("on", "off"), default="on")
+    ap.add_argument("--boundary-leg", choices=("on", "off"), default="on",
+                    help="evolution.cpp schedule with absorbing x/y faces at --boundary-n")
+    ap.add_argument("--boundary-n", type=int, default=512)
+    ap.add_argument("--boundary-steps", type=int, default=300)
+    ap.add_argument("--sor", choices=("on", "off"), default="on", help="straight-string SOR profile")
+    ap.add_argument("--sor-cpu-sweeps", type=int, default=1500)
     ap.add_argument("--quiet", action="store_true")
     a = ap.parse_args()
 
@@ -352,6 +426,8 @@ def main():
     if a.ensemble == "on":
         ens = ensemble(ax, a.ensemble_count, a.ensemble_rate, a.ensemble_seed, a.ensemble_budget_gb,
                        a.ensemble_chunk, a.ensemble_fp64 == "on", log)
+    bnd = boundary_leg(ax, a.boundary_n, a.boundary_steps, a.seed, log) if a.boundary_leg == "on" else None
+    sor = sor_leg(ax, a.sor_cpu_sweeps if a.cpu_baseline == "on" else 0, log) if a.sor == "on" else None
     cpu = None
     if a.cpu_baseline == "on":
         cpu = cpu_baseline(a.cpu_n or a.n, a.cpu_steps, min(16, physical_cores()), log)
@@ -378,6 +454,8 @@ def main():
         "fp32": fp32,
         "moore": moore,
         "ensemble": ens,
+        "boundaries": bnd,
+        "sor_profile": sor,
         "final_diag": {"timestep": last[0], "energy": last[1], "gauss_mean": last[2], "wound_plaquettes": last[3],
                        "xi": last[4]},
     }
