@@ -428,7 +428,10 @@ __device__ __forceinline__ double link_phase_sc(double cr, double ci, double nr,
   return ax_atan2(im, re);
 }
 
-template <typename T, bool PH>
+/* P2: dx, dy, dz and g (when g != 0) are powers of two, so every division by
+ * dx, dx*dx, g or (g*dx*dy)^2 is the multiplication by its exact reciprocal
+ * (the same correctly rounded value).  Divisions by dt stay divisions. */
+template <typename T, bool PH, bool P2>
 __global__ void __launch_bounds__(64) diag_march_kernel(DiagConsts k, Levels<T> L, TileGrid g, double *phase,
                                                         double *part) {
   const int w = (int)blockIdx.x;
@@ -452,6 +455,12 @@ __global__ void __launch_bounds__(64) diag_march_kernel(DiagConsts k, Levels<T> 
   auto ld_ = [](const T *b, uint32_t row, uint32_t off) { return (double)ldo(b, row, off); };
   const double dx = k.dx, dy = k.dy, dz = k.dz, dt = k.dt, gg = k.g;
   const size_t V = (size_t)NX * NY * NZ;
+  const double dxx = dx * dx, dyy = dy * dy, dzz = dz * dz;
+  const double bxyd = (gg * dx * dy) * (gg * dx * dy), bxzd = (gg * dx * dz) * (gg * dx * dz),
+               byzd = (gg * dy * dz) * (gg * dy * dz);
+  const double idx = 1 / dx, idy = 1 / dy, idz = 1 / dz, idxx = 1 / dxx, idyy = 1 / dyy, idzz = 1 / dzz;
+  const double igg = 1 / gg, ibxy = 1 / bxyd, ibxz = 1 / bxzd, ibyz = 1 / byzd;
+  auto dv = [](double a, double d, double inv) { return P2 ? a * inv : a / d; };
 
   /* carried from plane x-1: phi, theta_x (both levels), sin/cos theta_x */
   const int xm0 = x0 == 0 ? NX - 1 : x0 - 1;
@@ -489,16 +498,25 @@ __global__ void __launch_bounds__(64) diag_march_kernel(DiagConsts k, Levels<T> 
     ax_sincos(TZzm, &szm_, &czm_);
     double vel[2], gx[2], gy[2], gz[2];
     vel[0] = (pr - oPR) / dt;
-    gx[0] = (pr - cxm * PRm + sxm * PIm) / dx;
-    gy[0] = (pr - cym_ * PRym + sym_ * PIym) / dy;
-    gz[0] = (pr - czm_ * PRzm + szm_ * PIzm) / dz;
+    gx[0] = dv(pr - cxm * PRm + sxm * PIm, dx, idx);
+    gy[0] = dv(pr - cym_ * PRym + sym_ * PIym, dy, idy);
+    gz[0] = dv(pr - czm_ * PRzm + szm_ * PIzm, dz, idz);
     vel[1] = (pi - oPI) / dt;
-    gx[1] = (pi - cxm * PIm + (-sxm) * PRm) / dx;
-    gy[1] = (pi - cym_ * PIym + (-sym_) * PRym) / dy;
-    gz[1] = (pi - czm_ * PIzm + (-szm_) * PRzm) / dz;
+    gx[1] = dv(pi - cxm * PIm + (-sxm) * PRm, dx, idx);
+    gy[1] = dv(pi - cym_ * PIym + (-sym_) * PRym, dy, idy);
+    gz[1] = dv(pi - czm_ * PIzm + (-szm_) * PRzm, dz, idz);
     const double pxy = X + TYn - TXyp - Y;
     const double pxz = X + TZn - TXzp - Z;
     const double pyz = Y + TZyp - TYzp - Z;
+    /* cosines of the three plaquette angles; the short k = 0 form when every
+     * lane's angles qualify (bit-identical to ax_cos there) */
+    auto plaq_cos = [&](double &cxy, double &cxz, double &cyz) {
+      if (__all(ax_sin_is_small(pxy) && ax_sin_is_small(pxz) && ax_sin_is_small(pyz))) {
+        cxy = ax_cos_kernel(pxy, 0.0); cxz = ax_cos_kernel(pxz, 0.0); cyz = ax_cos_kernel(pyz, 0.0);
+      } else {
+        cxy = ax_cos(pxy); cxz = ax_cos(pxz); cyz = ax_cos(pyz);
+      }
+    };
     double tdc, fcont, G = 0, rho;
     const double vv = mag2 - k.eta * k.eta;
     if (k.energy_form == 0) {
@@ -506,15 +524,17 @@ __global__ void __launch_bounds__(64) diag_march_kernel(DiagConsts k, Levels<T> 
         tdc = 0;
         fcont = 0;
       } else {
-        double divN = (X - TXm) / (dx * dx) + (Y - TYym) / (dy * dy) + (Z - TZzm) / (dz * dz);
-        double divP = (oX - oXm) / (dx * dx) + (oY - oYym) / (dy * dy) + (oZ - oZzm) / (dz * dz);
+        double divN = dv(X - TXm, dxx, idxx) + dv(Y - TYym, dyy, idyy) + dv(Z - TZzm, dzz, idzz);
+        double divP = dv(oX - oXm, dxx, idxx) + dv(oY - oYym, dyy, idyy) + dv(oZ - oZzm, dzz, idzz);
         double divT = (divN - divP) / dt;
-        G = divT / gg - 2 * gg * (pi * vel[0] - pr * vel[1]);
+        G = dv(divT, gg, igg) - 2 * gg * (pi * vel[0] - pr * vel[1]);
         double ex = (X - oX) / (gg * dt * dx), ey = (Y - oY) / (gg * dt * dy), ez = (Z - oZ) / (gg * dt * dz);
         tdc = 0.5 * (ex * ex + ey * ey + ez * ez);
-        double bxy = (1 - ax_cos(pxy)) / ((gg * dx * dy) * (gg * dx * dy));
-        double bxz = (1 - ax_cos(pxz)) / ((gg * dx * dz) * (gg * dx * dz));
-        double byz = (1 - ax_cos(pyz)) / ((gg * dy * dz) * (gg * dy * dz));
+        double cxy, cxz, cyz;
+        plaq_cos(cxy, cxz, cyz);
+        double bxy = dv(1 - cxy, bxyd, ibxy);
+        double bxz = dv(1 - cxz, bxzd, ibxz);
+        double byz = dv(1 - cyz, byzd, ibyz);
         fcont = bxy + bxz + byz;
       }
       double r_ = vel[0] * vel[0] + vel[1] * vel[1] + gx[0] * gx[0] + gx[1] * gx[1] + gy[0] * gy[0] +
@@ -523,9 +543,11 @@ __global__ void __launch_bounds__(64) diag_march_kernel(DiagConsts k, Levels<T> 
     } else {
       double ex = (X - oX) / dt, ey = (Y - oY) / dt, ez = (Z - oZ) / dt;
       tdc = 0.5 * (ex * ex + ey * ey + ez * ez);
-      double bxy = (1 - ax_cos(pxy)) / ((gg * dx * dy) * (gg * dx * dy));
-      double bxz = (1 - ax_cos(pxz)) / ((gg * dx * dz) * (gg * dx * dz));
-      double byz = (1 - ax_cos(pyz)) / ((gg * dy * dz) * (gg * dy * dz));
+      double cxy, cxz, cyz;
+      plaq_cos(cxy, cxz, cyz);
+      double bxy = dv(1 - cxy, bxyd, ibxy);
+      double bxz = dv(1 - cxz, bxzd, ibxz);
+      double byz = dv(1 - cyz, byzd, ibyz);
       fcont = bxy + bxz + byz;
       rho = (vel[0] * vel[0] + vel[1] * vel[1] + gx[0] * gx[0] + gx[1] * gx[1] + gy[0] * gy[0] +
              gy[1] * gy[1] + gz[0] * gz[0] + gz[1] * gz[1] + tdc + fcont + 0.25 * k.lambda * (vv * vv)) *
@@ -653,10 +675,20 @@ hipError_t launch_diag_march(const DiagConsts &k, const Levels<T> &L, int xseg, 
   g.xseg = (xseg <= 0 || xseg > k.nx) ? k.nx : xseg;
   g.nw = (k.nz + 63) / 64;
   g.work = g.nw * k.ny * ((k.nx + g.xseg - 1) / g.xseg);
-  if (phase)
-    hipLaunchKernelGGL((diag_march_kernel<T, true>), dim3(g.work), dim3(64), 0, st, k, L, g, phase, part);
-  else
-    hipLaunchKernelGGL((diag_march_kernel<T, false>), dim3(g.work), dim3(64), 0, st, k, L, g, phase, part);
+  /* every divisor the P2 kernel multiplies by the reciprocal of must be a
+   * power of two whose reciprocal is finite (then a*(1/d) == a/d exactly) */
+  const double dx = k.dx, dy = k.dy, dz = k.dz, gg = k.g;
+  const double dvs[10] = {dx, dy, dz, dx * dx, dy * dy, dz * dz, gg, (gg * dx * dy) * (gg * dx * dy),
+                          (gg * dx * dz) * (gg * dx * dz), (gg * dy * dz) * (gg * dy * dz)};
+  bool p2 = gg != 0;
+  for (double d : dvs) p2 = p2 && is_pow2(fabs(d)) && std::isfinite(1 / d);
+#define AX_DM(PH, P2) \
+  hipLaunchKernelGGL((diag_march_kernel<T, PH, P2>), dim3(g.work), dim3(64), 0, st, k, L, g, phase, part)
+  if (phase && p2) AX_DM(true, true);
+  else if (phase) AX_DM(true, false);
+  else if (p2) AX_DM(false, true);
+  else AX_DM(false, false);
+#undef AX_DM
   hipLaunchKernelGGL(sum3_kernel, dim3(1), dim3(1024), 0, st, part, g.work, out3);
   return hipGetLastError();
 }

@@ -224,17 +224,24 @@ def test_gpu_matches_reference_fixture(name):
         assert sha(cur) == fx["final_sha256"]
 
 
-@pytest.mark.parametrize("shape,precision", [((24, 20, 64), 64), ((13, 10, 96), 64), ((9, 11, 17), 64),
-                                             ((16, 12, 100), 32)])
-def test_diag_march_equals_site_kernel(shape, precision):
+@pytest.mark.parametrize("shape,precision,dx,g,energy_form", [
+    ((24, 20, 64), 64, 0.5, 1.0, 0), ((13, 10, 96), 64, 0.5, 1.0, 0), ((9, 11, 17), 64, 0.5, 1.0, 0),
+    ((16, 12, 100), 32, 0.5, 1.0, 0),
+    ((20, 18, 64), 64, 0.7, 1.3, 0),     # spacings and coupling not powers of two: true divisions
+    ((16, 16, 64), 64, 0.5, 0.5, 1),     # powers of two, the mpi_evolution.cpp energy form
+    ((12, 14, 64), 32, 0.25, 2.0, 1),
+    ((12, 10, 64), 64, 0.5, 0.75, 0)])   # dx a power of two, g not
+def test_diag_march_equals_site_kernel(shape, precision, dx, g, energy_form):
     """The marching diagnostics (periodic Abelian-Higgs: energy/Gauss with the
     edge phases stored once per edge, windings from them) give the site
     kernels' windings bit for bit and their energy and Gauss sums to 1e-13
-    (same per-site terms, another summation order)."""
+    (same per-site terms, another summation order), with and without the
+    power-of-two reciprocals."""
     import os
     nx, ny, nz = shape
-    p = ax.ah_params(nx, ny=ny, nz=nz, g=1.0, alpha=2.0, damped_nt=3, dampFac=0.5, precision=precision)
-    say(f"[diag-march] {shape} fp{precision}")
+    p = ax.ah_params(nx, ny=ny, nz=nz, dx=dx, dt=0.1 * dx / 0.5, g=g, alpha=2.0, damped_nt=3, dampFac=0.5,
+                     energy_form=energy_form, precision=precision)
+    say(f"[diag-march] {shape} fp{precision} dx={dx} g={g} energy_form={energy_form}")
     with ax.Lattice(p) as lat:
         lat.init_random_phase(17)
         lat.step(7)
