@@ -6,9 +6,9 @@
  * single sin() grows to O(1) within a few hundred steps.  Bit-identical
  * trajectories between the GPU kernels and the CPU oracle therefore need the
  * same trigonometric functions on both sides.  Every routine below is written
- * only with +, -, *, /, fma, rint, floor and compares/selects, all of which
- * are exactly specified by IEEE 754, so a host build and a device build that
- * both use -ffp-contract=off produce identical bits.
+ * only with +, -, *, /, fma, rint, compares/selects and sign-bit flips, all
+ * of which are exactly specified by IEEE 754, so a host build and a device
+ * build that both use -ffp-contract=off produce identical bits.
  *
  * Method (published, not vendor code):
  *   - Cody & Waite argument reduction by pi/2 with a three-part constant;
@@ -36,6 +36,8 @@
 #define AXIONS_AX_TRIG_H
 
 #include <math.h>
+#include <stdint.h>
+#include <string.h>
 #ifndef __cplusplus
 #include <stdbool.h>
 #endif
@@ -45,6 +47,35 @@
 #else
 #define AX_HD static inline
 #endif
+
+AX_HD uint64_t ax_dbits(double v) {
+  uint64_t b;
+  memcpy(&b, &v, sizeof b);
+  return b;
+}
+AX_HD double ax_bitsd(uint64_t b) {
+  double v;
+  memcpy(&v, &b, sizeof v);
+  return v;
+}
+AX_HD uint32_t ax_fbits(float v) {
+  uint32_t b;
+  memcpy(&b, &v, sizeof b);
+  return b;
+}
+AX_HD float ax_bitsf(uint32_t b) {
+  float v;
+  memcpy(&v, &b, sizeof v);
+  return v;
+}
+
+/* k mod 4 (0..3) of an integer-valued k with |k| < 2^51: adding 1.5 * 2^52
+ * is exact and leaves 2^51 + k as the integer held in the low significand
+ * bits, whose two lowest bits are k mod 4.  (Larger |k|, infinities and NaN
+ * give some value in 0..3, the same on every IEEE target.) */
+AX_HD int ax_mod4(double k) { return (int)(ax_dbits(k + 6755399441055744.0) & 3); }
+/* the binary32 form, exact for |k| < 2^22 (1.5 * 2^23) */
+AX_HD int ax_mod4f(float k) { return (int)(ax_fbits(k + 12582912.0f) & 3); }
 
 /* pi/2 split: PIO2_1 = RN(pi/2), PIO2_2 = RN(pi/2 - PIO2_1),
  * PIO2_3 = RN(pi/2 - PIO2_1 - PIO2_2). */
@@ -79,11 +110,9 @@ AX_HD void ax_reduce_pio2(double x, double *rh, double *rl, int *q) {
   double we = fma(k, AX_PIO2_2, -w);   /* exact error of k*PIO2_2 */
   double h = r1 - w;
   double l = ((r1 - h) - w) - fma(k, AX_PIO2_3, we);
-  double kq = k - 4.0 * floor(k * 0.25); /* 0..3 for finite k, NaN otherwise */
-  kq = (kq == kq) ? kq : 0.0;
   *rh = h;
   *rl = l;
-  *q = (int)kq;
+  *q = ax_mod4(k);
 }
 
 AX_HD double ax_sin_kernel(double rh, double rl) {
@@ -126,8 +155,9 @@ AX_HD void ax_sincos(double x, double *s, double *c) {
   sk = (x == 0.0) ? x : sk;            /* sin(-0) = -0 */
   double s0 = (q & 1) ? ck : sk;
   double c0 = (q & 1) ? sk : ck;
-  *s = (q & 2) ? -s0 : s0;
-  *c = ((q + 1) & 2) ? -c0 : c0;
+  /* negation is the sign-bit flip: sin takes it in quadrants 2, 3, cos in 1, 2 */
+  *s = ax_bitsd(ax_dbits(s0) ^ ((uint64_t)(q & 2) << 62));
+  *c = ax_bitsd(ax_dbits(c0) ^ ((uint64_t)((q + 1) & 2) << 62));
 }
 
 AX_HD double ax_sin(double x) {
@@ -239,9 +269,7 @@ AX_HD void ax_sincosf(float x, float *s, float *c) {
   float r = fmaf(-k, AX_F_PIO2_1, x);
   r = fmaf(-k, AX_F_PIO2_2, r);
   r = fmaf(-k, AX_F_PIO2_3, r);
-  float kq = k - 4.0f * floorf(k * 0.25f);
-  kq = (kq == kq) ? kq : 0.0f;
-  int q = (int)kq;
+  int q = ax_mod4f(k);
   float z = r * r;
   float ps = fmaf(z, 2.7557319e-06f, -1.9841270e-04f);
   ps = fmaf(z, ps, 8.3333333e-03f);
@@ -254,8 +282,8 @@ AX_HD void ax_sincosf(float x, float *s, float *c) {
   sk = (x == 0.0f) ? x : sk;
   float s0 = (q & 1) ? ck : sk;
   float c0 = (q & 1) ? sk : ck;
-  *s = (q & 2) ? -s0 : s0;
-  *c = ((q + 1) & 2) ? -c0 : c0;
+  *s = ax_bitsf(ax_fbits(s0) ^ ((uint32_t)(q & 2) << 30));
+  *c = ax_bitsf(ax_fbits(c0) ^ ((uint32_t)((q + 1) & 2) << 30));
 }
 
 AX_HD float ax_sinf(float x) {
