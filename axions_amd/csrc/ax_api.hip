@@ -402,13 +402,24 @@ int ax_init_random_phase(ax_lattice *L, uint32_t seed, int32_t mode) {
 int ax_step(ax_lattice *L, int32_t nsteps) {
   if (!L || nsteps < 0) return fail(AX_EINVAL, "bad step request");
   hipEvent_t e0 = nullptr, e1 = nullptr;
+  auto drop_events = [&] {
+    if (e0) hipEventDestroy(e0);
+    if (e1) hipEventDestroy(e1);
+  };
   if (L->timing && nsteps > 0) {
-    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return fail(AX_EHIP, "event");
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+      drop_events();
+      return fail(AX_EHIP, "event");
+    }
     hipEventRecord(e0, L->stream);
   }
   for (int s = 0; s < nsteps; ++s) {
     int rc = launch_step(L, L->timestep);
-    if (rc) return rc;
+    if (rc) {
+      /* steps before this one were enqueued and counted; no timing for the call */
+      drop_events();
+      return rc;
+    }
     L->timestep++;
   }
   if (e0) {

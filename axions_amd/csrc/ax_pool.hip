@@ -252,7 +252,10 @@ struct ax_pool {
   std::unordered_map<int64_t, std::unique_ptr<Req>> reqs;
   std::vector<Staging> staging;
   std::deque<ChunkRec> inflight;
-  std::vector<std::pair<int64_t, int64_t>> deferred;   /* (request, last chunk that may touch it) */
+  /* arena slots of failed lattices: (offset, last chunk that may still touch
+   * it), released once that chunk is harvested, whatever happens to the
+   * request meanwhile (ax_pool_release may forget it earlier) */
+  std::vector<std::pair<size_t, int64_t>> deferred;
   int64_t next_seq = 0;
   int next_buf = 0;
   int64_t next_id = 1;
@@ -382,7 +385,8 @@ void ax_pool::harvest(ChunkRec &c) {
        * free its memory only once the last of them has been harvested */
       r.t = r.steps;
       resident.erase(std::remove(resident.begin(), resident.end(), r.id), resident.end());
-      deferred.emplace_back(r.id, next_seq - 1);
+      if (r.off >= 0) deferred.emplace_back((size_t)r.off, next_seq - 1);
+      r.off = -1;
     } else if (r.status == AX_REQ_DONE) {
       if (r.flags & AX_POOL_KEEP_STATE)
         resident.erase(std::remove(resident.begin(), resident.end(), r.id), resident.end());
@@ -392,8 +396,7 @@ void ax_pool::harvest(ChunkRec &c) {
   }
   for (size_t i = 0; i < deferred.size();) {
     if (deferred[i].second <= c.seq) {
-      auto f = reqs.find(deferred[i].first);
-      if (f != reqs.end()) free_slot(*f->second);
+      arena.release(deferred[i].first);
       deferred[i] = deferred.back();
       deferred.pop_back();
     } else {

@@ -137,3 +137,27 @@ def test_budget_and_autotune():
         assert len(xs) == 1 and xs.pop() in (4, 8, 16, 32, 64)
         x, ms = pool.tune(32)
         assert x in (4, 8, 16, 32) and ms > 0
+
+
+def test_failed_requests_free_their_memory():
+    """Lattices whose state turns non-finite end as failed; their arena slots
+    are released only after every chunk that may still touch them has
+    finished, also when the caller releases the requests before that."""
+    from axions_amd.abi import AX_REQ_FAILED
+    say("[pool] failed requests")
+    # dt = 5 at dx = 0.5 is far beyond the leapfrog's stability limit: FP32
+    # states overflow within a few steps
+    params = ax.ah_params(8, dt=5.0, precision=32)
+    budget = 3 * lattice_bytes(32, 32) + 1024
+    with ax.Pool(params, hbm_budget=budget, chunk_steps=4, autotune=0, ic_threads=2) as pool:
+        ids = [pool.submit(s, 32, 200) for s in range(7)]
+        assert pool.wait(timeout=120)
+        for i in ids:
+            assert pool.info(i)["status"] == AX_REQ_FAILED
+            pool.release(i)
+        t0 = time.time()
+        while pool.stats()["in_use"] and time.time() - t0 < 30:
+            time.sleep(0.01)
+        s = pool.stats()
+        assert s["failed"] == 7 and s["completed"] == 0
+        assert s["in_use"] == 0 and s["peak_in_use"] <= budget and s["max_resident"] <= 3
