@@ -20,7 +20,10 @@
  *     while link angles drift.
  * Each field value is read from HBM once per step: plane x+1 (current
  * level) is loaded as the next iteration's centre and carried; the y/z
- * neighbour rows are cache hits.
+ * neighbour rows are cache hits.  The loop's loads and stores are raw
+ * buffer accesses (bld/bst below) whose addresses cost no vector
+ * instructions; the previous level's reads and the new level's writes carry
+ * the streaming hint so that they do not push those rows out of L2.
  *
  * A chunk may be partial (nz not a multiple of 64): lanes past the row end
  * load the last site of the row and store nothing.  No lane exchanges data
@@ -108,6 +111,35 @@ __device__ __forceinline__ R ldo(const R *base, uint32_t row, uint32_t lane_off)
 template <typename R>
 __device__ __forceinline__ void sto(R *base, uint32_t row, uint32_t lane_off, R v) {
   *(__attribute__((address_space(1))) R *)(uniform_row(base, row) + lane_off) = v;
+}
+
+/* Raw buffer access for the march's inner loop: one buffer resource per
+ * field plane (its base address in SGPRs), a wave-uniform byte offset
+ * (plane + row) as the scalar offset and the lane's z offset as the vector
+ * offset, so no load or store spends vector instructions on its address.
+ * Every plane is below 4 GiB (the tiled kernels' condition), so all offsets
+ * fit the 32-bit fields; the range check is disabled (num_records = ~0).
+ * kStream (the SLC bit) marks data used once per step -- the previous
+ * level's reads and the new level's writes -- as streaming, which leaves L2
+ * to the current level's rows that neighbouring waves read again. */
+typedef unsigned int ax_u2 __attribute__((ext_vector_type(2)));
+constexpr int kStream = 2;
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_rsrc(const void *plane) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(plane), 0, -1, 0x00020000);
+}
+template <typename R, int AUX = 0>
+__device__ __forceinline__ R bld(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+  if constexpr (sizeof(R) == 8)
+    return __builtin_bit_cast(R, __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, (int)soff, AUX));
+  else
+    return __builtin_bit_cast(R, __builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, AUX));
+}
+template <typename R, int AUX = 0>
+__device__ __forceinline__ void bst(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff, R v) {
+  if constexpr (sizeof(R) == 8)
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(ax_u2, v), r, (int)voff, (int)soff, AUX);
+  else
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned int, v), r, (int)voff, (int)soff, AUX);
 }
 
 /* Everything one site update reads for one x plane besides the carried
@@ -342,8 +374,14 @@ __device__ __forceinline__ void ah_march(const MarchConsts<R> &k, const Levels<R
   const uint32_t PLB = (uint32_t)NY * (uint32_t)NZ * E;          /* plane bytes */
   const uint32_t R0 = (uint32_t)y * NZ * E, RM = (uint32_t)ym * NZ * E, RP = (uint32_t)yp * NZ * E;
   const R *PR = L.cur[0], *PI = L.cur[1], *TX = L.cur[2], *TY = L.cur[3], *TZ = L.cur[4];
-  R *OPR = L.prv[0], *OPI = L.prv[1], *OTX = L.prv[2], *OTY = L.prv[3], *OTZ = L.prv[4];
   const bool plain = PLAIN == 1 || (PLAIN == 2 && k.plain);
+  /* current level c[0..4] = phi_r, phi_i, theta_x, theta_y, theta_z; previous
+   * level o[0..4] (read once, then overwritten with level n+1) */
+  __amdgpu_buffer_rsrc_t c[5], o[5];
+  for (int f = 0; f < 5; ++f) {
+    c[f] = plane_rsrc(L.cur[f]);
+    o[f] = plane_rsrc(L.prv[f]);
+  }
 
   /* FP32: z+-1 operands come from the neighbouring lanes (DPP) instead of
    * reloading shifted rows; only the two chunk-edge lanes load: lane 0 the
@@ -357,35 +395,36 @@ __device__ __forceinline__ void ah_march(const MarchConsts<R> &k, const Levels<R
   auto load_plane = [&](int x, PlaneOps<R> &d, R PRc_, R PIc_, R TXc_, R TYc_, R TZc_) {
     const int xp = x == NX - 1 ? 0 : x + 1;
     const uint32_t b = (uint32_t)x * PLB, bn = (uint32_t)xp * PLB;
-    d.PRn = ldo(PR, bn + R0, oz); d.PIn = ldo(PI, bn + R0, oz);
-    d.TXn = ldo(TX, bn + R0, oz); d.TYn = ldo(TY, bn + R0, oz); d.TZn = ldo(TZ, bn + R0, oz);
-    d.oPR = ldo(OPR, b + R0, oz); d.oPI = ldo(OPI, b + R0, oz);
-    d.oX = ldo(OTX, b + R0, oz); d.oY = ldo(OTY, b + R0, oz); d.oZ = ldo(OTZ, b + R0, oz);
-    d.PRyp = ldo(PR, b + RP, oz); d.PIyp = ldo(PI, b + RP, oz);
-    d.PRym = ldo(PR, b + RM, oz); d.PIym = ldo(PI, b + RM, oz);
-    d.TXyp = ldo(TX, b + RP, oz); d.TZyp = ldo(TZ, b + RP, oz);
-    d.TXym = ldo(TX, b + RM, oz); d.TYym = ldo(TY, b + RM, oz); d.TZym = ldo(TZ, b + RM, oz);
-    d.TYxpym = ldo(TY, bn + RM, oz);
+    const uint32_t s0 = b + R0, sP = b + RP, sM = b + RM, sN = bn + R0, sNM = bn + RM;
+    d.PRn = bld<R>(c[0], oz, sN); d.PIn = bld<R>(c[1], oz, sN);
+    d.TXn = bld<R>(c[2], oz, sN); d.TYn = bld<R>(c[3], oz, sN); d.TZn = bld<R>(c[4], oz, sN);
+    d.oPR = bld<R, kStream>(o[0], oz, s0); d.oPI = bld<R, kStream>(o[1], oz, s0);
+    d.oX = bld<R, kStream>(o[2], oz, s0); d.oY = bld<R, kStream>(o[3], oz, s0); d.oZ = bld<R, kStream>(o[4], oz, s0);
+    d.PRyp = bld<R>(c[0], oz, sP); d.PIyp = bld<R>(c[1], oz, sP);
+    d.PRym = bld<R>(c[0], oz, sM); d.PIym = bld<R>(c[1], oz, sM);
+    d.TXyp = bld<R>(c[2], oz, sP); d.TZyp = bld<R>(c[4], oz, sP);
+    d.TXym = bld<R>(c[2], oz, sM); d.TYym = bld<R>(c[3], oz, sM); d.TZym = bld<R>(c[4], oz, sM);
+    d.TYxpym = bld<R>(c[3], oz, sNM);
     if (sizeof(R) == 8) {
       /* FP64 is issue-bound: the shifts' extra VALU costs more than the
        * reloads (L1 hits) they save; FP32 gains from the fewer requests */
-      d.PRzp = ldo(PR, b + R0, ozp); d.PIzp = ldo(PI, b + R0, ozp);
-      d.PRzm = ldo(PR, b + R0, ozm); d.PIzm = ldo(PI, b + R0, ozm);
-      d.TXzp = ldo(TX, b + R0, ozp); d.TYzp = ldo(TY, b + R0, ozp);
-      d.TYymzp = ldo(TY, b + RM, ozp);
-      d.TXzm = ldo(TX, b + R0, ozm); d.TYzm = ldo(TY, b + R0, ozm); d.TZzm = ldo(TZ, b + R0, ozm);
-      d.TZxpzm = ldo(TZ, bn + R0, ozm); d.TZypzm = ldo(TZ, b + RP, ozm);
+      d.PRzp = bld<R>(c[0], ozp, s0); d.PIzp = bld<R>(c[1], ozp, s0);
+      d.PRzm = bld<R>(c[0], ozm, s0); d.PIzm = bld<R>(c[1], ozm, s0);
+      d.TXzp = bld<R>(c[2], ozp, s0); d.TYzp = bld<R>(c[3], ozp, s0);
+      d.TYymzp = bld<R>(c[3], ozp, sM);
+      d.TXzm = bld<R>(c[2], ozm, s0); d.TYzm = bld<R>(c[3], ozm, s0); d.TZzm = bld<R>(c[4], ozm, s0);
+      d.TZxpzm = bld<R>(c[4], ozm, sN); d.TZypzm = bld<R>(c[4], ozm, sP);
       return;
     }
     R ePR = PRc_, ePI = PIc_, eTX = TXc_, eTY = TYc_, eTZ = TZc_, eTZxp = TZc_, eTZyp = TZc_, eTYym = TYc_;
     if (e0 || eL) {
-      ePR = ldo(PR, b + R0, oe); ePI = ldo(PI, b + R0, oe);
-      eTX = ldo(TX, b + R0, oe); eTY = ldo(TY, b + R0, oe);
+      ePR = bld<R>(c[0], oe, s0); ePI = bld<R>(c[1], oe, s0);
+      eTX = bld<R>(c[2], oe, s0); eTY = bld<R>(c[3], oe, s0);
     }
     if (e0) {
-      eTZ = ldo(TZ, b + R0, oe); eTZxp = ldo(TZ, bn + R0, oe); eTZyp = ldo(TZ, b + RP, oe);
+      eTZ = bld<R>(c[4], oe, s0); eTZxp = bld<R>(c[4], oe, sN); eTZyp = bld<R>(c[4], oe, sP);
     }
-    if (eL) eTYym = ldo(TY, b + RM, oe);
+    if (eL) eTYym = bld<R>(c[3], oe, sM);
     d.PRzm = lane_prev(PRc_, ePR); d.PIzm = lane_prev(PIc_, ePI);
     d.TXzm = lane_prev(TXc_, eTX); d.TYzm = lane_prev(TYc_, eTY); d.TZzm = lane_prev(TZc_, eTZ);
     d.TZxpzm = lane_prev(d.TZn, eTZxp); d.TZypzm = lane_prev(d.TZyp, eTZyp);
@@ -413,14 +452,14 @@ __device__ __forceinline__ void ah_march(const MarchConsts<R> &k, const Levels<R
     PlaneOps<R> D;
     load_plane(x, D, PRc, PIc, TXc, TYc, TZc);
     const uint32_t so = (uint32_t)x * PLB + R0;
-    SiteOut<R> o;
-    ah_site_update<R, POW2, PHASED>(k, plain, RegOps<R>{D, PRc, PIc, TXc, TYc, TZc}, m, o);
+    SiteOut<R> out;
+    ah_site_update<R, POW2, PHASED>(k, plain, RegOps<R>{D, PRc, PIc, TXc, TYc, TZc}, m, out);
     if (act) {
-      sto(OTX, so, oz, o.X);
-      sto(OTY, so, oz, o.Y);
-      sto(OTZ, so, oz, o.Z);
-      sto(OPR, so, oz, o.PR);
-      sto(OPI, so, oz, o.PI);
+      bst<R, kStream>(o[2], oz, so, out.X);
+      bst<R, kStream>(o[3], oz, so, out.Y);
+      bst<R, kStream>(o[4], oz, so, out.Z);
+      bst<R, kStream>(o[0], oz, so, out.PR);
+      bst<R, kStream>(o[1], oz, so, out.PI);
     }
     PRc = D.PRn; PIc = D.PIn; TXc = D.TXn; TYc = D.TYn; TZc = D.TZn;
   }
