@@ -30,7 +30,11 @@ inline void moore_consts_from(const StepConsts &k, MooreConsts<R> &m) {
   m.inv_dx2 = (R)1 / m.dx2; m.inv_dy2 = (R)1 / m.dy2; m.inv_dz2 = (R)1 / m.dz2;
 }
 
-template <typename R, bool POW2>
+/* BUF: the loop's loads and stores are raw buffer accesses, one resource per
+ * field spanning both of its levels (planes 2c, 2c+1; the caller checks that
+ * two planes fit 32-bit offsets), with the streaming hint on the previous
+ * level as in ah_march; otherwise plain global accesses. */
+template <typename R, bool POW2, bool BUF = false>
 __device__ __forceinline__ void moore_march(const MooreConsts<R> &k, const Levels<R> &L, int NX, int NY, int NZ, int y,
                                             int zc, int x0, int x1, bool zfix = false) {
   typedef TrigR<R> T;
@@ -50,6 +54,22 @@ __device__ __forceinline__ void moore_march(const MooreConsts<R> &k, const Level
   R *O[7] = {L.prv[0], L.prv[1], L.prv[2], L.prv[3], L.prv[4], L.prv[5], L.prv[6]};
   const R *TX = F[4], *TY = F[5], *TZ = F[6];
   const R q[2] = {k.q1, k.q2};
+  /* field c's lower plane, and the byte offsets of the two levels from it */
+  __amdgpu_buffer_rsrc_t FR[7];
+  uint32_t offC = 0, offP = 0;
+  if (BUF) {
+    for (int c = 0; c < 7; ++c) FR[c] = plane_rsrc(L.cur[c] < L.prv[c] ? (const void *)L.cur[c] : (const void *)L.prv[c]);
+    const bool curLow = L.cur[0] < L.prv[0];
+    const uint32_t lev = (uint32_t)(curLow ? (const char *)L.prv[0] - (const char *)L.cur[0]
+                                           : (const char *)L.cur[0] - (const char *)L.prv[0]);
+    offC = curLow ? 0u : lev;
+    offP = curLow ? lev : 0u;
+  }
+  /* current-level value of field c at (plane/row byte offset ro, lane offset lo) */
+  auto LC = [&](int c, uint32_t ro, uint32_t lo) -> R {
+    if (BUF) return bld<R>(FR[c], lo, offC + ro);
+    return ldo(F[c], ro, lo);
+  };
 
   /* carry from plane x-1: phi_f(s-x), cos/sin(q_f theta_x(s-x)), sines at s-x */
   const int xm0 = x0 == 0 ? NX - 1 : x0 - 1;
@@ -70,14 +90,14 @@ __device__ __forceinline__ void moore_march(const MooreConsts<R> &k, const Level
     const int xp = x == NX - 1 ? 0 : x + 1;
     const uint32_t b = (uint32_t)x * PLB, bn = (uint32_t)xp * PLB, so = b + R0;
     R Nx[7];
-    for (int c = 0; c < 7; ++c) Nx[c] = ldo(F[c], bn + R0, oz);
+    for (int c = 0; c < 7; ++c) Nx[c] = LC(c, bn + R0, oz);
     const R X = C[4], Y = C[5], Z = C[6];
-    const R TXyp = ldo(TX, b + RP, oz), TZyp = ldo(TZ, b + RP, oz);
-    const R TXzp = ldo(TX, b + R0, ozp), TYzp = ldo(TY, b + R0, ozp);
-    const R TXym = ldo(TX, b + RM, oz), TYym = ldo(TY, b + RM, oz), TZym = ldo(TZ, b + RM, oz);
-    const R TYxpym = ldo(TY, bn + RM, oz), TYymzp = ldo(TY, b + RM, ozp);
-    const R TXzm = ldo(TX, b + R0, ozm), TYzm = ldo(TY, b + R0, ozm), TZzm = ldo(TZ, b + R0, ozm);
-    const R TZxpzm = ldo(TZ, bn + R0, ozm), TZypzm = ldo(TZ, b + RP, ozm);
+    const R TXyp = LC(4, b + RP, oz), TZyp = LC(6, b + RP, oz);
+    const R TXzp = LC(4, b + R0, ozp), TYzp = LC(5, b + R0, ozp);
+    const R TXym = LC(4, b + RM, oz), TYym = LC(5, b + RM, oz), TZym = LC(6, b + RM, oz);
+    const R TYxpym = LC(5, bn + RM, oz), TYymzp = LC(5, b + RM, ozp);
+    const R TXzm = LC(4, b + R0, ozm), TYzm = LC(5, b + R0, ozm), TZzm = LC(6, b + R0, ozm);
+    const R TZxpzm = LC(6, bn + R0, ozm), TZypzm = LC(6, b + RP, ozm);
 
     /* plaquette sines (mooreEvol.cpp:310-333 operand order) */
     const R a_xy = X + Nx[5] - TXyp - Y;
@@ -126,10 +146,10 @@ __device__ __forceinline__ void moore_march(const MooreConsts<R> &k, const Level
       T::sc(q[f] * TZzm, &szm, &czm);
       const R ar = C[cr], ai = C[ci];
       const R rxp = Nx[cr], ixp = Nx[ci];
-      const R ryp = ldo(F[cr], b + RP, oz), iyp = ldo(F[ci], b + RP, oz);
-      const R rym = ldo(F[cr], b + RM, oz), iym = ldo(F[ci], b + RM, oz);
-      const R rzp = ldo(F[cr], b + R0, ozp), izp = ldo(F[ci], b + R0, ozp);
-      const R rzm = ldo(F[cr], b + R0, ozm), izm = ldo(F[ci], b + R0, ozm);
+      const R ryp = LC(cr, b + RP, oz), iyp = LC(ci, b + RP, oz);
+      const R rym = LC(cr, b + RM, oz), iym = LC(ci, b + RM, oz);
+      const R rzp = LC(cr, b + R0, ozp), izp = LC(ci, b + R0, ozp);
+      const R rzm = LC(cr, b + R0, ozm), izm = LC(ci, b + R0, ozm);
       {
         R lxx = mdiv<R, POW2>(cx * rxp + sx * ixp - 2 * ar + cxm[f] * Pm[cr] - sxm[f] * Pm[ci], k.dx2, k.inv_dx2);
         R lyy = mdiv<R, POW2>(cy * ryp + sy * iyp - 2 * ar + cym * rym - sym_ * iym, k.dy2, k.inv_dy2);
@@ -151,14 +171,22 @@ __device__ __forceinline__ void moore_march(const MooreConsts<R> &k, const Level
     const R atx = k.m2gg * (cur[0][0] + cur[1][0]) - fxy - fxz;
     const R aty = k.m2gg * (cur[0][1] + cur[1][1]) - fyx - fyz;
     const R atz = k.m2gg * (cur[0][2] + cur[1][2]) - fzx - fzy;
+    /* previous level at s: read once, overwritten with level n+1 */
+    auto LP = [&](int c) -> R {
+      if (BUF) return bld<R, kStream>(FR[c], oz, offP + so);
+      return ldo(O[c], so, oz);
+    };
     R Onew[7];
 #pragma unroll
-    for (int c = 0; c < 4; ++c) Onew[c] = 2 * C[c] - ldo(O[c], so, oz) + k.dt2 * acc[c >> 1][c & 1];
-    Onew[4] = 2 * X - ldo(O[4], so, oz) + k.dt2 * atx;
-    Onew[5] = 2 * Y - ldo(O[5], so, oz) + k.dt2 * aty;
-    Onew[6] = 2 * Z - ldo(O[6], so, oz) + k.dt2 * atz;
+    for (int c = 0; c < 4; ++c) Onew[c] = 2 * C[c] - LP(c) + k.dt2 * acc[c >> 1][c & 1];
+    Onew[4] = 2 * X - LP(4) + k.dt2 * atx;
+    Onew[5] = 2 * Y - LP(5) + k.dt2 * aty;
+    Onew[6] = 2 * Z - LP(6) + k.dt2 * atz;
     if (act)
-      for (int c = 0; c < 7; ++c) sto(O[c], so, oz, Onew[c]);
+      for (int c = 0; c < 7; ++c) {
+        if (BUF) bst<R, kStream>(FR[c], oz, offP + so, Onew[c]);
+        else sto(O[c], so, oz, Onew[c]);
+      }
     /* carry */
     for (int c = 0; c < 4; ++c) Pm[c] = C[c];
     for (int f = 0; f < 2; ++f) { cxm[f] = cxs[f]; sxm[f] = sxs[f]; }

@@ -45,14 +45,14 @@ __global__ void __launch_bounds__(64) ah_tiled_kernel(MarchConsts<R> k, Levels<R
   ah_march<R, POW2, PLAIN ? 1 : 0>(k, L, nx, ny, nz, y, zc, x0, x1, ez != 0);
 }
 
-template <typename R, bool POW2>
-__global__ void __launch_bounds__(64) moore_tiled_kernel(MooreConsts<R> k, Levels<R> L, int nx, int ny, int nz,
+template <typename R, bool POW2, bool BUF>
+__global__ void __launch_bounds__(64, 3) moore_tiled_kernel(MooreConsts<R> k, Levels<R> L, int nx, int ny, int nz,
                                                          TileGrid g, int ex, int ez) {
   const int w = (int)blockIdx.x;
   if (w >= g.work) return;
   int zc, y, x0, x1;
   if (!tile_range(g, nx, ny, ex, w, zc, y, x0, x1)) return;
-  moore_march<R, POW2>(k, L, nx, ny, nz, y, zc, x0, x1, ez != 0);
+  moore_march<R, POW2, BUF>(k, L, nx, ny, nz, y, zc, x0, x1, ez != 0);
 }
 
 template <typename R>
@@ -64,9 +64,17 @@ hipError_t launch_step_tiled_moore(const StepConsts &k, const Levels<R> &L, int 
   MooreConsts<R> m;
   moore_consts_from<R>(k, m);
   const bool pow2 = is_pow2(k.dx2) && is_pow2(k.dy2) && is_pow2(k.dz2);
+  /* buffer path when both levels of a field lie within 4 GiB of its lower plane */
+  const size_t lev = (size_t)std::llabs((const char *)L.cur[0] - (const char *)L.prv[0]);
+  const bool buf = lev + (size_t)k.nx * k.ny * k.nz * sizeof(R) < ((size_t)1 << 32);
   dim3 grid(g.work), block(64);
-  if (pow2) hipLaunchKernelGGL((moore_tiled_kernel<R, true>), grid, block, 0, st, m, L, k.nx, k.ny, k.nz, g, k.ex, k.ez);
-  else hipLaunchKernelGGL((moore_tiled_kernel<R, false>), grid, block, 0, st, m, L, k.nx, k.ny, k.nz, g, k.ex, k.ez);
+#define AX_MRL(P2, B) \
+  hipLaunchKernelGGL((moore_tiled_kernel<R, P2, B>), grid, block, 0, st, m, L, k.nx, k.ny, k.nz, g, k.ex, k.ez)
+  if (pow2 && buf) AX_MRL(true, true);
+  else if (pow2) AX_MRL(true, false);
+  else if (buf) AX_MRL(false, true);
+  else AX_MRL(false, false);
+#undef AX_MRL
   return hipGetLastError();
 }
 
