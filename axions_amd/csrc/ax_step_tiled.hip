@@ -90,13 +90,14 @@ bool use_tiled_ah(const ax_params &p) {
          getenv("AX_FORCE_SITE_KERNEL") == nullptr;
 }
 
-/* x planes per wave: 32 for Abelian-Higgs (512^3: 4.16 ms vs 4.18 at 16,
- * 4.21 at 64); 8 for Moore, whose heavier site update leaves fewer waves
- * resident and gains from the finer grid (256^3: 1.235 ms vs 1.28 at 32) */
+/* x planes per wave (measured at 512^3 AH, 256^3 Moore): Abelian-Higgs FP64
+ * 32 (4.04 ms; 16: 4.04, 64: 4.09), FP32 16 (1.787 ms; 32: 1.80, 64: 1.87);
+ * Moore 8 (FP64 1.012 ms, FP32 0.422; 32: 1.02 / 0.444) */
 int default_xseg(const ax_params &p) {
   const char *xe = getenv("AX_TILED_XSEG");
   if (xe) return atoi(xe);
-  return p.model == AX_MODEL_MOORE ? 8 : 32;
+  if (p.model == AX_MODEL_MOORE) return 8;
+  return p.precision == 32 ? 16 : 32;
 }
 
 template <typename R>

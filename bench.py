@@ -47,11 +47,11 @@ METRIC = "site-updates/s at 512³ FP64/FP32, % HBM roofline; ensemble p50/p99 la
 ENSEMBLE_NS = (64, 96, 128, 192, 256)
 
 
-def measured_traffic(n, precision):
+def measured_traffic(n, precision, model="ah"):
     """HBM bytes per step launch from the committed rocprofv3 PMC summary of
     this exact workload (FETCH_SIZE/WRITE_SIZE scaled by a copy of known
     size), or None when no summary covers it."""
-    path = os.path.join(REPO, "profiles", f"ah_step_tiled_{n}_fp{precision}.md")
+    path = os.path.join(REPO, "profiles", f"{model}_step_tiled_{n}_fp{precision}.md")
     try:
         for line in open(path):
             if line.startswith("traffic_bytes_per_launch:"):
@@ -153,7 +153,7 @@ def run_lattice(ax, n, precision, steps, warmup, diag_every, seed, log, moore=Fa
         "ms_per_step": elapsed / steps * 1e3,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": None if moore else measured_traffic(n, precision),
+                     "traffic": measured_traffic(n, precision, "moore" if moore else "ah"),
                      "kernel_ms": avg_kernel_s * 1e3, "bytes_per_site": bps},
         "series": series,
     }
