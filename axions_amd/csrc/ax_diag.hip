@@ -435,8 +435,8 @@ template <typename T, bool PH, bool P2>
 __global__ void __launch_bounds__(64) diag_march_kernel(DiagConsts k, Levels<T> L, TileGrid g, double *phase,
                                                         double *part) {
   const int w = (int)blockIdx.x;
-  const int zc = w % g.nw, r = w / g.nw;
-  const int y = r % k.ny, seg = r / k.ny;
+  int zc, y, seg;
+  tile_item(g, k.ny, w, zc, y, seg);
   const int x0 = seg * g.xseg, x1 = min(k.nx, x0 + g.xseg);
   const int NX = k.nx, NY = k.ny, NZ = k.nz;
   const int lane = (int)(threadIdx.x & 63);
@@ -671,10 +671,7 @@ hipError_t launch_crossings(int nx, int ny, int nz, double dx, double dy, double
 template <typename T>
 hipError_t launch_diag_march(const DiagConsts &k, const Levels<T> &L, int xseg, double *phase, double *part,
                              double *out3, hipStream_t st) {
-  TileGrid g;
-  g.xseg = (xseg <= 0 || xseg > k.nx) ? k.nx : xseg;
-  g.nw = (k.nz + 63) / 64;
-  g.work = g.nw * k.ny * ((k.nx + g.xseg - 1) / g.xseg);
+  const TileGrid g = make_tile_grid(k.nx, k.ny, k.nz, xseg);
   /* every divisor the P2 kernel multiplies by the reciprocal of must be a
    * power of two whose reciprocal is finite (then a*(1/d) == a/d exactly) */
   const double dx = k.dx, dy = k.dy, dz = k.dz, gg = k.g;

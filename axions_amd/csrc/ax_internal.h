@@ -6,6 +6,8 @@
 
 #include <cstdarg>
 
+#include <cstdlib>
+
 #include "ax_device.h"
 
 struct ax_lattice;
@@ -18,7 +20,44 @@ struct TileGrid {
   int nw;      /* z chunks per row */
   int xseg;    /* x planes marched by one wave */
   int work;    /* nw * ny * nseg */
+  int per;     /* XCD-aware order: work items per XCD per x segment, 0 = natural order */
 };
+
+/* Work items of the marching kernels, one per (z chunk, y row, x segment).
+ * Workgroups are handed to the eight XCDs round-robin (workgroup w to XCD
+ * w mod 8), each XCD with its own L2.  In the natural order (z chunk
+ * fastest) an XCD receives whole columns of rows only when the row has a
+ * multiple of 8 chunks; otherwise neighbouring rows land on different XCDs
+ * and every y-neighbour read misses in L2.  The XCD-aware order gives XCD x
+ * the x-th eighth of each x segment's (row-fastest) items, i.e. a run of
+ * consecutive rows of one chunk, taken in row order. */
+inline TileGrid make_tile_grid(int nx, int ny, int nz, int xseg) {
+  TileGrid g;
+  g.xseg = (xseg <= 0 || xseg > nx) ? nx : xseg;
+  g.nw = (nz + 63) / 64;
+  g.work = g.nw * ny * ((nx + g.xseg - 1) / g.xseg);
+  const char *o = getenv("AX_TILE_ORDER");
+  const bool natural = o && o[0] == 'n';
+  g.per = (!natural && (g.nw * ny) % 8 == 0) ? g.nw * ny / 8 : 0;
+  return g;
+}
+
+/* (z chunk, y row, x segment) of work item w in the order make_tile_grid chose */
+__device__ __forceinline__ void tile_item(const TileGrid &g, int ny, int w, int &zc, int &y, int &seg) {
+  if (g.per) {
+    const int xcd = w & 7, i = w >> 3, s = i / g.per;
+    const int item = (s * 8 + xcd) * g.per + (i - s * g.per);   /* row + ny * (chunk + nw * segment) */
+    y = item % ny;
+    const int r = item / ny;
+    zc = r % g.nw;
+    seg = r / g.nw;
+  } else {
+    zc = w % g.nw;
+    const int r = w / g.nw;
+    y = r % ny;
+    seg = r / ny;
+  }
+}
 
 struct DiagConsts {
   int nx, ny, nz, ex, ez, model, energy_form, ncomp;

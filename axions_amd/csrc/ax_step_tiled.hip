@@ -21,10 +21,8 @@ namespace ax {
  * wrap), evolution.cpp:161-186.  Returns false for an empty item. */
 __device__ __forceinline__ bool tile_range(const TileGrid &g, int nx, int ny, int ex, int w, int &zc, int &y,
                                            int &x0, int &x1) {
-  zc = w % g.nw;
-  const int r = w / g.nw;
-  y = r % ny;
-  const int seg = r / ny;
+  int seg;
+  tile_item(g, ny, w, zc, y, seg);
   x0 = seg * g.xseg;
   x1 = min(nx, x0 + g.xseg);
   if (ex) {
@@ -57,10 +55,7 @@ __global__ void __launch_bounds__(64, 3) moore_tiled_kernel(MooreConsts<R> k, Le
 
 template <typename R>
 hipError_t launch_step_tiled_moore(const StepConsts &k, const Levels<R> &L, int xseg, hipStream_t st) {
-  TileGrid g;
-  g.xseg = (xseg <= 0 || xseg > k.nx) ? k.nx : xseg;
-  g.nw = (k.nz + 63) / 64;
-  g.work = g.nw * k.ny * ((k.nx + g.xseg - 1) / g.xseg);
+  const TileGrid g = make_tile_grid(k.nx, k.ny, k.nz, xseg);
   MooreConsts<R> m;
   moore_consts_from<R>(k, m);
   const bool pow2 = is_pow2(k.dx2) && is_pow2(k.dy2) && is_pow2(k.dz2);
@@ -102,10 +97,7 @@ int default_xseg(const ax_params &p) {
 
 template <typename R>
 hipError_t launch_step_tiled_ah(const StepConsts &k, const Levels<R> &L, int xseg, hipStream_t st) {
-  TileGrid g;
-  g.xseg = (xseg <= 0 || xseg > k.nx) ? k.nx : xseg;
-  g.nw = (k.nz + 63) / 64;
-  g.work = g.nw * k.ny * ((k.nx + g.xseg - 1) / g.xseg);
+  const TileGrid g = make_tile_grid(k.nx, k.ny, k.nz, xseg);
   MarchConsts<R> m;
   march_consts_from<R>(k, rate_limit<R>((R)k.dt), m);
   const bool pow2 = is_pow2(k.dx2) && is_pow2(k.dy2) && is_pow2(k.dz2);
