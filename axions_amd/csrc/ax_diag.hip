@@ -600,22 +600,12 @@ __global__ void __launch_bounds__(256) wind_phase_kernel(int nx, int ny, int nz,
     const int s = blockIdx.x * kSitesPerBlock + qq * 256 + threadIdx.x;
     if (s >= V) break;
     const int kk = s % nz, r = s / nz, j = r % ny, i = r / ny;
-    const int ip = wrapp(i, nx), jp = wrapp(j, ny), kp = wrapp(kk, nz);
-    auto id = [&](int a, int b, int c) { return (a * ny + b) * nz + c; };
-    const int sa[3] = {id(ip, j, kk), id(i, jp, kk), id(i, j, kp)};
-    const T *th[3] = {tx, ty, tz};
-    const double *D[3] = {phase, phase + V, phase + 2 * (size_t)V};
-    const int pa[3] = {0, 0, 1}, pb[3] = {1, 2, 2};
+    int w[3];
+    windings_from_phases<T>(nx, ny, nz, tx, ty, tz, phase, i, j, kk, w);
 #pragma unroll
     for (int pl = 0; pl < 3; ++pl) {
-      const int a = pa[pl], b = pb[pl];
-      const double th0a = 1.0 * ld(th[a], s), th0b = 1.0 * ld(th[b], s);
-      const double th1ab = 1.0 * ld(th[b], sa[a]), th1ba = 1.0 * ld(th[a], sa[b]);
-      const double P = th0a + th1ab - th1ba - th0b;
-      const double sum = D[a][s] + D[b][sa[a]] - D[a][sa[b]] - D[b][s] + P;
-      const int wv = (int)rint(sum * 0.15915494309189535);
-      out[(size_t)pl * V + s] = (int8_t)wv;
-      tot[pl] += (double)(wv < 0 ? -wv : wv);
+      out[(size_t)pl * V + s] = (int8_t)w[pl];
+      tot[pl] += (double)(w[pl] < 0 ? -w[pl] : w[pl]);
     }
   }
   double a = block_sum256(tot[0], sh);

@@ -13,10 +13,12 @@
  *                      each chunk, so the launch itself never changes and a
  *                      chunk of K steps is one captured hipGraph replayed
  *                      chunk after chunk.
- *   pool_wind_kernel   wound-plaquette counts of every resident lattice
- *                      (ax_wind.h), summed per lattice with integer atomics
- *                      (order-independent, so deterministic), plus a
- *                      non-finite flag.
+ *   pool_phase_kernel  the forward edge phases of every resident lattice
+ *                      into its slot, plus a non-finite flag;
+ *   pool_wind_kernel   wound-plaquette counts from those phases (ax_wind.h,
+ *                      the arithmetic of the single-lattice diagnostics),
+ *                      summed per lattice with integer atomics
+ *                      (order-independent, so deterministic).
  *   pool_zero_kernel   clears those counters (a kernel, not a memset, so the
  *                      graph holds kernel nodes only).
  *
@@ -62,7 +64,8 @@ struct BatchEntry {
   int32_t t0;           /* TimeStep at the start of the chunk */
   int32_t steps;        /* TimeSteps to take in this chunk (<= chunk length) */
   int32_t pre;          /* 1: count windings before the first step (t0 == 0) */
-  int32_t pad[3];
+  int32_t pad;
+  uint64_t phase;       /* device address of the lattice's edge phases [3][n^3] */
 };
 static_assert(sizeof(BatchEntry) == 64, "entry layout");
 
@@ -125,40 +128,75 @@ __device__ __forceinline__ int wave_isum(int v) {
   return v;
 }
 
+/* Block-sum of four per-thread counts, added to the entry's counters. */
+__device__ __forceinline__ void add_counts(int v[4], int (*sh)[4], unsigned long long *dst) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+#pragma unroll
+  for (int c = 0; c < 4; ++c) v[c] = wave_isum(v[c]);
+  __syncthreads();                    /* sh free from the previous block */
+  if (lane == 0)
+    for (int c = 0; c < 4; ++c) sh[wave][c] = v[c];
+  __syncthreads();
+  if (threadIdx.x < 4) {
+    const int c = threadIdx.x;
+    const int tot = sh[0][c] + sh[1][c] + sh[2][c] + sh[3][c];
+    if (tot) atomicAdd(&dst[c], (unsigned long long)tot);
+  }
+}
+
+/* which = 0: the state before the chunk's first step (lattices with pre
+ * set), 1: the state after the chunk's steps. */
 template <typename T>
-__global__ void __launch_bounds__(256) pool_wind_kernel(const BatchHeader *__restrict__ hdr,
-                                                        const BatchEntry *__restrict__ ent,
-                                                        unsigned long long *counts, int phase) {
+__global__ void __launch_bounds__(256) pool_phase_kernel(const BatchHeader *__restrict__ hdr,
+                                                         const BatchEntry *__restrict__ ent,
+                                                         unsigned long long *counts, int which) {
   __shared__ int sh[4][4];
   const int count = hdr->count, nb = hdr->wblocks;
   for (int b = (int)blockIdx.x; b < nb; b += (int)gridDim.x) {
     const int e = find_entry(ent, count, b, false);
     const BatchEntry E = ent[e];
-    if (phase == 0 && !E.pre) continue;
+    if (which == 0 && !E.pre) continue;
     const int n = E.n;
     const int V = n * n * n;
     const int s = (b - E.wblk_begin) * 256 + (int)threadIdx.x;
-    const int t = E.t0 + (phase == 0 ? 0 : E.steps);   /* state after t TimeSteps */
-    int w[3] = {0, 0, 0}, bad = 0;
+    const int t = E.t0 + (which == 0 ? 0 : E.steps);   /* state after t TimeSteps */
+    int v[4] = {0, 0, 0, 0};
     if (s < V) {
       const Levels<T> L = entry_levels<T>(E, t);
       const int kk = s % n, rr = s / n, jj = rr % n, ii = rr / n;
-      site_winding<T>(n, n, n, L.cur[0], L.cur[1], L.cur[2], L.cur[3], L.cur[4], 1.0, ii, jj, kk, w);
-      for (int c = 0; c < 5; ++c) bad |= !isfinite((double)L.cur[c][s]);
+      double d[3];
+      site_edge_phases<T>(n, n, n, L.cur[0], L.cur[1], L.cur[2], L.cur[3], L.cur[4], ii, jj, kk, d);
+      double *ph = (double *)E.phase;
+      for (int a = 0; a < 3; ++a) ph[(size_t)a * V + s] = d[a];
+      for (int c = 0; c < 5; ++c) v[3] |= !isfinite((double)L.cur[c][s]);
     }
-    int v[4] = {abs(w[0]), abs(w[1]), abs(w[2]), bad};
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-#pragma unroll
-    for (int c = 0; c < 4; ++c) v[c] = wave_isum(v[c]);
-    __syncthreads();                    /* sh free from the previous block */
-    if (lane == 0)
-      for (int c = 0; c < 4; ++c) sh[wave][c] = v[c];
-    __syncthreads();
-    if (threadIdx.x < 4) {
-      const int c = threadIdx.x;
-      const int tot = sh[0][c] + sh[1][c] + sh[2][c] + sh[3][c];
-      if (tot) atomicAdd(&counts[(size_t)e * kCnt + phase * 4 + c], (unsigned long long)tot);
+    add_counts(v, sh, counts + (size_t)e * kCnt + which * 4);
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) pool_wind_kernel(const BatchHeader *__restrict__ hdr,
+                                                        const BatchEntry *__restrict__ ent,
+                                                        unsigned long long *counts, int which) {
+  __shared__ int sh[4][4];
+  const int count = hdr->count, nb = hdr->wblocks;
+  for (int b = (int)blockIdx.x; b < nb; b += (int)gridDim.x) {
+    const int e = find_entry(ent, count, b, false);
+    const BatchEntry E = ent[e];
+    if (which == 0 && !E.pre) continue;
+    const int n = E.n;
+    const int V = n * n * n;
+    const int s = (b - E.wblk_begin) * 256 + (int)threadIdx.x;
+    const int t = E.t0 + (which == 0 ? 0 : E.steps);
+    int v[4] = {0, 0, 0, 0};
+    if (s < V) {
+      const Levels<T> L = entry_levels<T>(E, t);
+      const int kk = s % n, rr = s / n, jj = rr % n, ii = rr / n;
+      int w[3];
+      windings_from_phases<T>(n, n, n, L.cur[2], L.cur[3], L.cur[4], (const double *)E.phase, ii, jj, kk, w);
+      for (int c = 0; c < 3; ++c) v[c] = abs(w[c]);
     }
+    add_counts(v, sh, counts + (size_t)e * kCnt + which * 4);
   }
 }
 
@@ -239,7 +277,7 @@ struct ax_pool {
   static constexpr int kBufs = 3;
   char *h_tab[kBufs] = {};          /* pinned: header + entries */
   unsigned long long *h_cnt[kBufs] = {};
-  int grid_step = 0, grid_wind = 0;
+  int grid_step = 0, grid_wind = 0, grid_phase = 0;
   hipGraphExec_t gexec = nullptr;
   hipGraph_t graph = nullptr;
 
@@ -265,6 +303,7 @@ struct ax_pool {
   std::vector<std::thread> icw;
 
   int launch_chunk_kernels(hipStream_t s);
+  void launch_winding(hipStream_t s, int which);
   int capture_graph();
   void scheduler();
   void ic_worker();
@@ -294,11 +333,21 @@ void build_ic(uint32_t seed, size_t V, double eta, R *re, R *im) {
 
 }  // namespace
 
+/* edge phases, then windings from them (which: 0 before the steps, 1 after) */
+void ax_pool::launch_winding(hipStream_t s, int which) {
+  if (esize == 8) {
+    hipLaunchKernelGGL((pool_phase_kernel<double>), dim3(grid_phase), dim3(256), 0, s, d_hdr, d_ent, d_cnt, which);
+    hipLaunchKernelGGL((pool_wind_kernel<double>), dim3(grid_wind), dim3(256), 0, s, d_hdr, d_ent, d_cnt, which);
+  } else {
+    hipLaunchKernelGGL((pool_phase_kernel<float>), dim3(grid_phase), dim3(256), 0, s, d_hdr, d_ent, d_cnt, which);
+    hipLaunchKernelGGL((pool_wind_kernel<float>), dim3(grid_wind), dim3(256), 0, s, d_hdr, d_ent, d_cnt, which);
+  }
+}
+
 int ax_pool::launch_chunk_kernels(hipStream_t s) {
   const int K = cfg.chunk_steps;
   hipLaunchKernelGGL(pool_zero_kernel, dim3(1), dim3(256), 0, s, d_hdr, d_cnt);
-  if (esize == 8) hipLaunchKernelGGL((pool_wind_kernel<double>), dim3(grid_wind), dim3(256), 0, s, d_hdr, d_ent, d_cnt, 0);
-  else hipLaunchKernelGGL((pool_wind_kernel<float>), dim3(grid_wind), dim3(256), 0, s, d_hdr, d_ent, d_cnt, 0);
+  launch_winding(s, 0);
   for (int j = 0; j < K; ++j) {
     if (esize == 8) {
       if (pow2) hipLaunchKernelGGL((pool_step_kernel<double, true>), dim3(grid_step), dim3(64), 0, s, p, lim64, d_hdr, d_ent, j);
@@ -308,8 +357,7 @@ int ax_pool::launch_chunk_kernels(hipStream_t s) {
       else hipLaunchKernelGGL((pool_step_kernel<float, false>), dim3(grid_step), dim3(64), 0, s, p, lim32, d_hdr, d_ent, j);
     }
   }
-  if (esize == 8) hipLaunchKernelGGL((pool_wind_kernel<double>), dim3(grid_wind), dim3(256), 0, s, d_hdr, d_ent, d_cnt, 1);
-  else hipLaunchKernelGGL((pool_wind_kernel<float>), dim3(grid_wind), dim3(256), 0, s, d_hdr, d_ent, d_cnt, 1);
+  launch_winding(s, 1);
   hipError_t e = hipGetLastError();
   return e == hipSuccess ? AX_OK : hip_fail(e, "pool chunk launch");
 }
@@ -476,6 +524,8 @@ int ax_pool::enqueue_chunk() {
     E.t0 = r.t;
     E.steps = std::min(K, r.steps - r.t);
     E.pre = r.pre_done ? 0 : 1;
+    E.pad = 0;
+    E.phase = (uint64_t)(uintptr_t)(arena_base + r.off + lattice_phase_offset(r.n, esize, pad_bytes));
     work += E.nw * r.n * E.nseg;
     wblk += (int)((V + 255) / 256);
     c.items.push_back({id, E.t0, E.steps, E.pre});
@@ -806,12 +856,17 @@ int ax_pool_create(const ax_params *tmpl, const ax_pool_config *cfg_in, ax_pool 
   else
     e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&bs, P->pow2 ? (const void *)pool_step_kernel<float, true>
                                                                   : (const void *)pool_step_kernel<float, false>, 64, 0);
+  int bp = 0;
   if (e == hipSuccess)
     e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&bw, P->esize == 8 ? (const void *)pool_wind_kernel<double>
                                                                          : (const void *)pool_wind_kernel<float>, 256, 0);
+  if (e == hipSuccess)
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&bp, P->esize == 8 ? (const void *)pool_phase_kernel<double>
+                                                                         : (const void *)pool_phase_kernel<float>, 256, 0);
   if (e != hipSuccess) return cleanup(hip_fail(e, "occupancy"));
   P->grid_step = std::max(1, ncu) * std::max(1, bs);
   P->grid_wind = std::max(1, ncu) * std::max(1, bw);
+  P->grid_phase = std::max(1, ncu) * std::max(1, bp);
   if (cfg.use_graph) {
     int rc = P->capture_graph();
     if (rc) return cleanup(rc);

@@ -5,7 +5,8 @@
  *  Arena       first-fit sub-allocator over one device allocation of the
  *              pool's HBM budget; blocks are 256-byte aligned and freed
  *              blocks coalesce with their neighbours.
- *  lattice_bytes  device bytes one request of edge N occupies.
+ *  lattice_bytes  device bytes one request of edge N occupies (state and
+ *                 the edge phases of its winding pass).
  *  xi_of       string-length density from the three plaquette-plane counts,
  *              the expression ax_diagnostics uses.
  */
@@ -91,15 +92,21 @@ class Arena {
   std::map<size_t, size_t> used_;   /* offset -> length */
 };
 
-/* Abelian-Higgs state of one n^3 lattice: 5 components x 2 levels of
- * planes, each plane padded by pad_bytes (rounded to the element size) and
- * the whole block rounded to the arena alignment. */
+/* Arena slot of one n^3 Abelian-Higgs lattice: its state (5 components x 2
+ * levels of planes, each plane padded by pad_bytes rounded to the element
+ * size), then, 256-byte aligned, the forward edge phases of the winding pass
+ * (3 x n^3 doubles); the whole block rounded to the arena alignment. */
 inline size_t lattice_pitch(size_t sites, size_t esize, size_t pad_bytes) {
   return sites + (pad_bytes + esize - 1) / esize;
 }
-inline size_t lattice_bytes(int n, size_t esize, size_t pad_bytes) {
+inline size_t lattice_phase_offset(int n, size_t esize, size_t pad_bytes) {
   const size_t V = (size_t)n * n * n;
   const size_t b = 10 * lattice_pitch(V, esize, pad_bytes) * esize;
+  return (b + 255) / 256 * 256;
+}
+inline size_t lattice_bytes(int n, size_t esize, size_t pad_bytes) {
+  const size_t V = (size_t)n * n * n;
+  const size_t b = lattice_phase_offset(n, esize, pad_bytes) + 3 * V * sizeof(double);
   return (b + Arena::kAlign - 1) / Arena::kAlign * Arena::kAlign;
 }
 

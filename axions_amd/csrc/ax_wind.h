@@ -77,6 +77,46 @@ __device__ __forceinline__ void site_winding(int nx, int ny, int nz, const T *fr
   }
 }
 
+/* The three forward edge phases delta_a(s), a = x, y, z, of site (i,j,kk)
+ * (charge 1): the edges that windings_from_phases combines.  Same
+ * arithmetic as link_phase, and as the marching diagnostics that store them
+ * with the link's sin/cos computed once. */
+template <typename T>
+__device__ __forceinline__ void site_edge_phases(int nx, int ny, int nz, const T *fr, const T *fi, const T *tx,
+                                                 const T *ty, const T *tz, int i, int j, int kk, double d[3]) {
+  auto id = [&](int a, int b, int c) { return (a * ny + b) * nz + c; };
+  const int s = id(i, j, kk);
+  const int sa[3] = {id(wrapp(i, nx), j, kk), id(i, wrapp(j, ny), kk), id(i, j, wrapp(kk, nz))};
+  const T *th[3] = {tx, ty, tz};
+  const double cr = ld(fr, s), ci = ld(fi, s);
+#pragma unroll
+  for (int a = 0; a < 3; ++a) d[a] = link_phase(cr, ci, ld(fr, sa[a]), ld(fi, sa[a]), 1.0 * ld(th[a], s));
+}
+
+/* Windings of the xy, xz, yz plaquettes at site s = (i,j,kk) from stored
+ * forward edge phases D[a] (V doubles each): the arithmetic of site_winding
+ * with q = 1, phases and angles in its order, so the results are its bits. */
+template <typename T>
+__device__ __forceinline__ void windings_from_phases(int nx, int ny, int nz, const T *tx, const T *ty, const T *tz,
+                                                     const double *phase, int i, int j, int kk, int w[3]) {
+  auto id = [&](int a, int b, int c) { return (a * ny + b) * nz + c; };
+  const size_t V = (size_t)nx * ny * nz;
+  const int s = id(i, j, kk);
+  const int sa[3] = {id(wrapp(i, nx), j, kk), id(i, wrapp(j, ny), kk), id(i, j, wrapp(kk, nz))};
+  const T *th[3] = {tx, ty, tz};
+  const double *D[3] = {phase, phase + V, phase + 2 * V};
+  const int pa[3] = {0, 0, 1}, pb[3] = {1, 2, 2};
+#pragma unroll
+  for (int pl = 0; pl < 3; ++pl) {
+    const int a = pa[pl], b = pb[pl];
+    const double th0a = 1.0 * ld(th[a], s), th0b = 1.0 * ld(th[b], s);
+    const double th1ab = 1.0 * ld(th[b], sa[a]), th1ba = 1.0 * ld(th[a], sa[b]);
+    const double P = th0a + th1ab - th1ba - th0b;
+    const double sum = D[a][s] + D[b][sa[a]] - D[a][sa[b]] - D[b][s] + P;
+    w[pl] = (int)rint(sum * 0.15915494309189535);
+  }
+}
+
 }  // namespace ax
 
 #endif
