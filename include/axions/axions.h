@@ -275,7 +275,7 @@ int ax_init_straight_string(ax_lattice *lat, const double *profile, int32_t npro
 typedef struct ax_pool ax_pool;
 
 typedef struct ax_pool_config {
-  size_t hbm_budget;       /* device bytes for lattice state; 0 = 80% of free memory */
+  size_t hbm_budget;       /* device bytes for lattice slots (state + winding edge phases); 0 = 80% of free memory */
   int32_t max_lattices;    /* resident at once, 1..4096 (0 = 1024) */
   int32_t chunk_steps;     /* TimeSteps per graph replay and xi cadence (0 = 10) */
   int32_t ic_threads;      /* host threads building initial conditions (0 = 8) */
@@ -317,7 +317,7 @@ int ax_pool_create(const ax_params *tmpl, const ax_pool_config *cfg, ax_pool **o
 /* Stops the scheduler (pending requests are dropped) and frees everything. */
 void ax_pool_destroy(ax_pool *pool);
 /* Queue a realization: random-phase IC from seed (as ax_init_random_phase),
- * then steps >= 1 TimeSteps on an n^3 lattice.  AX_EBUDGET if its state
+ * then steps >= 1 TimeSteps on an n^3 lattice.  AX_EBUDGET if its slot
  * alone exceeds the budget. */
 int ax_pool_submit(ax_pool *pool, uint32_t seed, int32_t n, int32_t steps, int32_t flags, int64_t *id);
 /* Wait until request id (id < 0: every request not yet released) is done
