@@ -21,30 +21,41 @@ struct TileGrid {
   int xseg;    /* x planes marched by one wave */
   int work;    /* nw * ny * nseg */
   int per;     /* XCD-aware order: work items per XCD per x segment, 0 = natural order */
+  int rows;    /* 1: within an XCD's share, all chunks of a row together (chunk fastest) */
 };
 
 /* Work items of the marching kernels, one per (z chunk, y row, x segment).
  * Workgroups are handed to the eight XCDs round-robin (workgroup w to XCD
  * w mod 8), each XCD with its own L2.  In the natural order (z chunk
- * fastest) an XCD receives whole columns of rows only when the row has a
- * multiple of 8 chunks; otherwise neighbouring rows land on different XCDs
- * and every y-neighbour read misses in L2.  The XCD-aware order gives XCD x
- * the x-th eighth of each x segment's (row-fastest) items, i.e. a run of
- * consecutive rows of one chunk, taken in row order. */
+ * fastest) XCD x receives chunk x mod 8 of every row: neighbouring chunks of
+ * a row, which share the cache lines at their z edges, sit on different
+ * XCDs, and with fewer than 8 chunks per row even neighbouring rows do.  The
+ * XCD-aware order gives XCD x the x-th eighth of each x segment's items:
+ * with ny divisible by 8, a band of ny/8 consecutive rows with all their
+ * chunks, chunk fastest (512^3 FP64 reads 15.4 -> 12.9 GB per step, FP32
+ * 9.0 -> 6.1 GB); otherwise a run of consecutive rows of one chunk. */
 inline TileGrid make_tile_grid(int nx, int ny, int nz, int xseg) {
   TileGrid g;
   g.xseg = (xseg <= 0 || xseg > nx) ? nx : xseg;
   g.nw = (nz + 63) / 64;
   g.work = g.nw * ny * ((nx + g.xseg - 1) / g.xseg);
-  const char *o = getenv("AX_TILE_ORDER");
+  const char *o = getenv("AX_TILE_ORDER");        /* "natural" or "chunks" for measurements */
   const bool natural = o && o[0] == 'n';
   g.per = (!natural && (g.nw * ny) % 8 == 0) ? g.nw * ny / 8 : 0;
+  g.rows = !(o && o[0] == 'c') && ny % 8 == 0;
   return g;
 }
 
 /* (z chunk, y row, x segment) of work item w in the order make_tile_grid chose */
 __device__ __forceinline__ void tile_item(const TileGrid &g, int ny, int w, int &zc, int &y, int &seg) {
-  if (g.per) {
+  if (g.per && g.rows) {
+    const int xcd = w & 7, i = w >> 3, s = i / g.per;
+    const int item = (s * 8 + xcd) * g.per + (i - s * g.per);   /* chunk + nw * (row + ny * segment) */
+    zc = item % g.nw;
+    const int r = item / g.nw;
+    y = r % ny;
+    seg = r / ny;
+  } else if (g.per) {
     const int xcd = w & 7, i = w >> 3, s = i / g.per;
     const int item = (s * 8 + xcd) * g.per + (i - s * g.per);   /* row + ny * (chunk + nw * segment) */
     y = item % ny;
