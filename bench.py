@@ -72,6 +72,17 @@ def physical_cores():
     return os.cpu_count() or 1
 
 
+def cpu_model():
+    """The host CPU's model name (/proc/cpuinfo), or None."""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(n, steps, threads, log):
     """Oracle step (reference arithmetic, libm trig) at n^3 for `steps` steps."""
     from axions_amd.abi import ah_params
@@ -85,6 +96,7 @@ def cpu_baseline(n, steps, threads, log):
     dt = time.perf_counter() - t0
     del st
     return {"value": n ** 3 * steps / dt, "unit": "site-updates/s", "cores": threads, "kind": "port",
+            "cpu": cpu_model(),
             "sample": f"oracle (C++/OpenMP restatement of mpi_evolution.cpp:381-501, glibc sin/cos) "
                       f"AH {n}^3 FP64, {steps} steps after 1 untimed, {dt:.2f} s"}
 
@@ -221,7 +233,7 @@ def sor_leg(ax, cpu_sweeps, log):
         orc.sor_profile(p.nx, p.h, p.w, 0.0, p.n, p.lambda_, p.eta, p.g, cpu_sweeps)
         cdt = _t.perf_counter() - t0
         out["cpu_baseline"] = {"value": p.nx * cpu_sweeps / cdt, "unit": "point-updates/s", "cores": 1,
-                               "kind": "port",
+                               "kind": "port", "cpu": cpu_model(),
                                "sample": f"oracle one-thread SOR (singleSemiAnalytic.cpp:82-134), first "
                                          f"{cpu_sweeps} sweeps of the same problem, {cdt:.2f} s"}
     return out
