@@ -212,7 +212,7 @@ int ax_create(const ax_params *p, ax_lattice **out) {
     const size_t pad = pe ? (size_t)atol(pe) : 4352;
     L->pitch = L->V + (pad + L->esize - 1) / L->esize;
   }
-  hipGetDevice(&L->device);
+  (void)hipGetDevice(&L->device);   /* hipGetDeviceCount succeeded above */
   hipError_t e;
   if ((e = hipStreamCreateWithFlags(&L->stream, hipStreamNonBlocking)) != hipSuccess) {
     delete L;
@@ -223,37 +223,37 @@ int ax_create(const ax_params *p, ax_lattice **out) {
   if (hipMalloc(&L->state, bytes) != hipSuccess || hipMalloc((void **)&L->part, 3 * nb * sizeof(double)) != hipSuccess ||
       hipMalloc((void **)&L->dres, 16 * sizeof(double)) != hipSuccess ||
       hipHostMalloc((void **)&L->hres, 16 * sizeof(double), hipHostMallocDefault) != hipSuccess) {
-    hipGetLastError();
+    (void)hipGetLastError();   /* clear the sticky error of the failed call */
     ax_destroy(L);
     return fail(AX_ENOMEM, "device allocation of %zu bytes failed", bytes);
   }
-  if ((e = hipMemsetAsync(L->state, 0, bytes, L->stream)) != hipSuccess) {
+  if ((e = hipMemsetAsync(L->state, 0, bytes, L->stream)) != hipSuccess ||
+      (e = hipStreamSynchronize(L->stream)) != hipSuccess) {
     ax_destroy(L);
-    return hip_fail(e, "hipMemset");
+    return hip_fail(e, "clearing the state");
   }
-  hipStreamSynchronize(L->stream);
   *out = L;
   return AX_OK;
 }
 
 void ax_destroy(ax_lattice *L) {
   if (!L) return;
-  if (L->stream) hipStreamSynchronize(L->stream);
+  if (L->stream) (void)hipStreamSynchronize(L->stream);   /* drain before freeing; errors were reported earlier */
   for (auto &pr : L->ev) {
-    hipEventDestroy(pr.first);
-    hipEventDestroy(pr.second);
+    (void)hipEventDestroy(pr.first);
+    (void)hipEventDestroy(pr.second);
   }
-  if (L->state) hipFree(L->state);
-  if (L->part) hipFree(L->part);
-  if (L->dres) hipFree(L->dres);
-  if (L->hres) hipHostFree(L->hres);
-  if (L->wind) hipFree(L->wind);
-  if (L->phase) hipFree(L->phase);
-  if (L->bcount) hipFree(L->bcount);
-  if (L->boff) hipFree(L->boff);
-  if (L->ctotal) hipFree(L->ctotal);
-  if (L->cxyz) hipFree(L->cxyz);
-  if (L->stream) hipStreamDestroy(L->stream);
+  if (L->state) (void)hipFree(L->state);
+  if (L->part) (void)hipFree(L->part);
+  if (L->dres) (void)hipFree(L->dres);
+  if (L->hres) (void)hipHostFree(L->hres);
+  if (L->wind) (void)hipFree(L->wind);
+  if (L->phase) (void)hipFree(L->phase);
+  if (L->bcount) (void)hipFree(L->bcount);
+  if (L->boff) (void)hipFree(L->boff);
+  if (L->ctotal) (void)hipFree(L->ctotal);
+  if (L->cxyz) (void)hipFree(L->cxyz);
+  if (L->stream) (void)hipStreamDestroy(L->stream);
   delete L;
 }
 
@@ -403,15 +403,15 @@ int ax_step(ax_lattice *L, int32_t nsteps) {
   if (!L || nsteps < 0) return fail(AX_EINVAL, "bad step request");
   hipEvent_t e0 = nullptr, e1 = nullptr;
   auto drop_events = [&] {
-    if (e0) hipEventDestroy(e0);
-    if (e1) hipEventDestroy(e1);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
   };
   if (L->timing && nsteps > 0) {
     if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
       drop_events();
       return fail(AX_EHIP, "event");
     }
-    hipEventRecord(e0, L->stream);
+    (void)hipEventRecord(e0, L->stream);   /* a failure shows in ax_kernel_time */
   }
   for (int s = 0; s < nsteps; ++s) {
     int rc = launch_step(L, L->timestep);
@@ -423,7 +423,7 @@ int ax_step(ax_lattice *L, int32_t nsteps) {
     L->timestep++;
   }
   if (e0) {
-    hipEventRecord(e1, L->stream);
+    (void)hipEventRecord(e1, L->stream);
     L->ev.emplace_back(e0, e1);
     L->timed_steps += nsteps;
   }
@@ -454,14 +454,20 @@ int ax_kernel_time(ax_lattice *L, double *ms, int64_t *steps) {
   hipError_t e = hipStreamSynchronize(L->stream);
   if (e != hipSuccess) return hip_fail(e, "sync");
   double tot = 0;
+  hipError_t te = hipSuccess;
   for (auto &pr : L->ev) {
     float t = 0;
-    hipEventElapsedTime(&t, pr.first, pr.second);
+    const hipError_t x = hipEventElapsedTime(&t, pr.first, pr.second);
+    if (x != hipSuccess && te == hipSuccess) te = x;
     tot += t;
-    hipEventDestroy(pr.first);
-    hipEventDestroy(pr.second);
+    (void)hipEventDestroy(pr.first);
+    (void)hipEventDestroy(pr.second);
   }
   L->ev.clear();
+  if (te != hipSuccess) {
+    L->timed_steps = 0;
+    return hip_fail(te, "kernel timer");
+  }
   *ms = tot;
   *steps = L->timed_steps;
   L->timed_steps = 0;
@@ -560,16 +566,16 @@ int ax_crossings(ax_lattice *L, double *xyz, size_t cap, size_t *count) {
     if (hipMalloc((void **)&L->bcount, nb * sizeof(int)) != hipSuccess ||
         hipMalloc((void **)&L->boff, nb * sizeof(long long)) != hipSuccess ||
         hipMalloc((void **)&L->ctotal, sizeof(long long)) != hipSuccess) {
-      hipGetLastError();
+      (void)hipGetLastError();   /* clear the sticky error of the failed call */
       return fail(AX_ENOMEM, "crossing scratch");
     }
   }
   if (cap > L->cxyz_cap) {
-    if (L->cxyz) hipFree(L->cxyz);
+    if (L->cxyz) (void)hipFree(L->cxyz);
     L->cxyz = nullptr;
     L->cxyz_cap = 0;
     if (hipMalloc((void **)&L->cxyz, cap * 3 * sizeof(double)) != hipSuccess) {
-      hipGetLastError();
+      (void)hipGetLastError();   /* clear the sticky error of the failed call */
       return fail(AX_ENOMEM, "crossing output of %zu points", cap);
     }
     L->cxyz_cap = cap;
@@ -637,7 +643,7 @@ size_t diag_part_blocks(const ax_params &p) {
 int ensure_phase_buffer(ax_lattice *L) {
   if (L->phase) return AX_OK;
   if (hipMalloc((void **)&L->phase, 3 * L->V * sizeof(double)) != hipSuccess) {
-    hipGetLastError();
+    (void)hipGetLastError();   /* clear the sticky error of the failed call */
     L->phase = nullptr;
     return fail(AX_ENOMEM, "edge-phase buffer of %zu bytes", 3 * L->V * sizeof(double));
   }
@@ -671,7 +677,7 @@ hipError_t launch_phase_winding(ax_lattice *L, int8_t *out, double *res3) {
 int ensure_winding_buffer(ax_lattice *L) {
   if (L->wind) return AX_OK;
   if (hipMalloc((void **)&L->wind, 3 * L->V) != hipSuccess) {
-    hipGetLastError();
+    (void)hipGetLastError();   /* clear the sticky error of the failed call */
     return fail(AX_ENOMEM, "winding buffer");
   }
   return AX_OK;

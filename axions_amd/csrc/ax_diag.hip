@@ -393,7 +393,7 @@ __global__ void __launch_bounds__(256) cross_write_kernel(int nx, int ny, int nz
   sh[threadIdx.x] = n;
   __syncthreads();
   for (int off = 1; off < 256; off <<= 1) {
-    int v = (threadIdx.x >= off) ? sh[threadIdx.x - off] : 0;
+    int v = ((int)threadIdx.x >= off) ? sh[threadIdx.x - off] : 0;
     __syncthreads();
     sh[threadIdx.x] += v;
     __syncthreads();

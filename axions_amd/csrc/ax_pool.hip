@@ -378,7 +378,7 @@ void ax_pool::free_slot(Req &r) {
   if (r.off >= 0) arena.release((size_t)r.off);
   r.off = -1;
   if (r.upload) {
-    hipEventDestroy(r.upload);
+    (void)hipEventDestroy(r.upload);
     r.upload = nullptr;
   }
   resident.erase(std::remove(resident.begin(), resident.end(), r.id), resident.end());
@@ -452,7 +452,7 @@ void ax_pool::harvest(ChunkRec &c) {
     }
   }
   st.site_updates += c.site_updates;
-  hipEventDestroy(c.done);
+  (void)hipEventDestroy(c.done);
   cv_done.notify_all();
 }
 
@@ -560,8 +560,9 @@ int ax_pool::enqueue_chunk() {
 }
 
 void ax_pool::scheduler() {
-  hipSetDevice(device);
+  const hipError_t de = hipSetDevice(device);
   std::unique_lock<std::mutex> lk(mu);
+  if (de != hipSuccess) fail_all(std::string("scheduler device: ") + hipGetErrorString(de));
   while (!stop) {
     /* staging buffers whose upload finished go back to the IC workers */
     for (auto &sg : staging)
@@ -615,7 +616,7 @@ void ax_pool::scheduler() {
     if (!inflight.empty()) {
       hipEvent_t ev = inflight.front().done;
       lk.unlock();
-      hipEventSynchronize(ev);
+      (void)hipEventSynchronize(ev);   /* its status is read by hipEventQuery above */
       lk.lock();
     } else {
       cv_sched.wait_for(lk, std::chrono::milliseconds(2));
@@ -623,8 +624,8 @@ void ax_pool::scheduler() {
   }
   /* drain what is in flight before the streams go away */
   lk.unlock();
-  hipStreamSynchronize(cs);
-  hipStreamSynchronize(us);
+  (void)hipStreamSynchronize(cs);
+  (void)hipStreamSynchronize(us);
 }
 
 void ax_pool::ic_worker() {
@@ -659,7 +660,7 @@ void ax_pool::ic_worker() {
     lk.unlock();
     bool ok = true;
     if (grow) {
-      if (buf) hipHostFree(buf);
+      if (buf) (void)hipHostFree(buf);
       buf = nullptr;
       ok = hipHostMalloc(&buf, need, hipHostMallocDefault) == hipSuccess;
     }
@@ -735,7 +736,7 @@ int ax_pool::tune(int n, int *xseg, double *ms) {
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     for (int rep = 0; rep < reps && e == hipSuccess; ++rep) {
       for (int j = 0; j < warm + timed; ++j) {
-        if (j == warm) hipEventRecord(e0, s);
+        if (j == warm) (void)hipEventRecord(e0, s);   /* failures show in the event synchronize below */
         if (esize == 8) {
           if (pow2) hipLaunchKernelGGL((pool_step_kernel<double, true>), dim3(grid_step), dim3(64), 0, s, p, lim64, dh, (const BatchEntry *)((char *)dh + 64), j);
           else hipLaunchKernelGGL((pool_step_kernel<double, false>), dim3(grid_step), dim3(64), 0, s, p, lim64, dh, (const BatchEntry *)((char *)dh + 64), j);
@@ -744,7 +745,7 @@ int ax_pool::tune(int n, int *xseg, double *ms) {
           else hipLaunchKernelGGL((pool_step_kernel<float, false>), dim3(grid_step), dim3(64), 0, s, p, lim32, dh, (const BatchEntry *)((char *)dh + 64), j);
         }
       }
-      hipEventRecord(e1, s);
+      (void)hipEventRecord(e1, s);
       e = hipEventSynchronize(e1);
       float t = 0;
       if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
@@ -755,11 +756,11 @@ int ax_pool::tune(int n, int *xseg, double *ms) {
     }
   }
   if (e != hipSuccess) rc = hip_fail(e, "tune");
-  if (s) hipStreamSynchronize(s);
-  if (dh) hipFree(dh);
-  if (e0) hipEventDestroy(e0);
-  if (e1) hipEventDestroy(e1);
-  if (s) hipStreamDestroy(s);
+  if (s) (void)hipStreamSynchronize(s);
+  if (dh) (void)hipFree(dh);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (s) (void)hipStreamDestroy(s);
   std::lock_guard<std::mutex> g(mu);
   arena.release((size_t)off);
   if (!rc) {
@@ -813,7 +814,7 @@ int ax_pool_create(const ax_params *tmpl, const ax_pool_config *cfg_in, ax_pool 
   P->lim64 = rate_limit<double>(tmpl->dt);
   P->lim32 = rate_limit<float>((float)tmpl->dt);
   P->t0 = std::chrono::steady_clock::now();
-  hipGetDevice(&P->device);
+  (void)hipGetDevice(&P->device);
 
   auto cleanup = [&](int rc) {
     ax_pool_destroy(P.release());
@@ -828,7 +829,7 @@ int ax_pool_create(const ax_params *tmpl, const ax_pool_config *cfg_in, ax_pool 
   if ((e = hipStreamCreateWithFlags(&P->cs, hipStreamNonBlocking)) != hipSuccess) return cleanup(hip_fail(e, "stream"));
   if ((e = hipStreamCreateWithFlags(&P->us, hipStreamNonBlocking)) != hipSuccess) return cleanup(hip_fail(e, "stream"));
   if (hipMalloc((void **)&P->arena_base, budget) != hipSuccess) {
-    hipGetLastError();
+    (void)hipGetLastError();   /* clear the sticky error of the failed call */
     return cleanup(fail(AX_ENOMEM, "pool arena of %zu bytes", budget));
   }
   P->arena.reset(budget);
@@ -836,7 +837,7 @@ int ax_pool_create(const ax_params *tmpl, const ax_pool_config *cfg_in, ax_pool 
   const size_t tab = 64 + (size_t)cfg.max_lattices * sizeof(BatchEntry);
   if (hipMalloc((void **)&P->d_hdr, tab) != hipSuccess ||
       hipMalloc((void **)&P->d_cnt, (size_t)cfg.max_lattices * kCnt * sizeof(unsigned long long)) != hipSuccess) {
-    hipGetLastError();
+    (void)hipGetLastError();   /* clear the sticky error of the failed call */
     return cleanup(fail(AX_ENOMEM, "pool tables"));
   }
   P->d_ent = (BatchEntry *)((char *)P->d_hdr + 64);
@@ -844,12 +845,12 @@ int ax_pool_create(const ax_params *tmpl, const ax_pool_config *cfg_in, ax_pool 
     if (hipHostMalloc((void **)&P->h_tab[b], tab, hipHostMallocDefault) != hipSuccess ||
         hipHostMalloc((void **)&P->h_cnt[b], (size_t)cfg.max_lattices * kCnt * sizeof(unsigned long long),
                       hipHostMallocDefault) != hipSuccess) {
-      hipGetLastError();
+      (void)hipGetLastError();   /* clear the sticky error of the failed call */
       return cleanup(fail(AX_ENOMEM, "pinned pool tables"));
     }
   /* persistent grids: every workgroup slot the device can hold at once */
   int ncu = 0, bs = 0, bw = 0;
-  hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, P->device);
+  (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, P->device);   /* 0 on failure: grid of 1 per slot */
   if (P->esize == 8)
     e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&bs, P->pow2 ? (const void *)pool_step_kernel<double, true>
                                                                   : (const void *)pool_step_kernel<double, false>, 64, 0);
@@ -892,27 +893,27 @@ void ax_pool_destroy(ax_pool *P) {
   if (P->sched.joinable()) P->sched.join();
   for (auto &t : P->icw)
     if (t.joinable()) t.join();
-  if (P->cs) hipStreamSynchronize(P->cs);
-  if (P->us) hipStreamSynchronize(P->us);
+  if (P->cs) (void)hipStreamSynchronize(P->cs);
+  if (P->us) (void)hipStreamSynchronize(P->us);
   for (auto &c : P->inflight)
-    if (c.done) hipEventDestroy(c.done);
+    if (c.done) (void)hipEventDestroy(c.done);
   for (auto &kv : P->reqs)
-    if (kv.second->upload) hipEventDestroy(kv.second->upload);
+    if (kv.second->upload) (void)hipEventDestroy(kv.second->upload);
   for (auto &sg : P->staging) {
-    if (sg.done) hipEventDestroy(sg.done);
-    if (sg.host) hipHostFree(sg.host);
+    if (sg.done) (void)hipEventDestroy(sg.done);
+    if (sg.host) (void)hipHostFree(sg.host);
   }
-  if (P->gexec) hipGraphExecDestroy(P->gexec);
-  if (P->graph) hipGraphDestroy(P->graph);
+  if (P->gexec) (void)hipGraphExecDestroy(P->gexec);
+  if (P->graph) (void)hipGraphDestroy(P->graph);
   for (int b = 0; b < ax_pool::kBufs; ++b) {
-    if (P->h_tab[b]) hipHostFree(P->h_tab[b]);
-    if (P->h_cnt[b]) hipHostFree(P->h_cnt[b]);
+    if (P->h_tab[b]) (void)hipHostFree(P->h_tab[b]);
+    if (P->h_cnt[b]) (void)hipHostFree(P->h_cnt[b]);
   }
-  if (P->d_hdr) hipFree(P->d_hdr);
-  if (P->d_cnt) hipFree(P->d_cnt);
-  if (P->arena_base) hipFree(P->arena_base);
-  if (P->cs) hipStreamDestroy(P->cs);
-  if (P->us) hipStreamDestroy(P->us);
+  if (P->d_hdr) (void)hipFree(P->d_hdr);
+  if (P->d_cnt) (void)hipFree(P->d_cnt);
+  if (P->arena_base) (void)hipFree(P->arena_base);
+  if (P->cs) (void)hipStreamDestroy(P->cs);
+  if (P->us) (void)hipStreamDestroy(P->us);
   delete P;
 }
 
@@ -1023,7 +1024,7 @@ int ax_pool_get_state(ax_pool *P, int64_t id, double *host, size_t n) {
       for (size_t i = 0; i < V && e == hipSuccess; ++i) host[(size_t)m * V + i] = tmp[i];
     }
   }
-  hipStreamDestroy(s);
+  (void)hipStreamDestroy(s);
   return e == hipSuccess ? AX_OK : hip_fail(e, "pool get_state");
 }
 
