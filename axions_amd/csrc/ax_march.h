@@ -386,12 +386,17 @@ __device__ __forceinline__ void ah_march(const MarchConsts<R> &k, const Levels<R
   /* FP32: z+-1 operands come from the neighbouring lanes (DPP) instead of
    * reloading shifted rows; only the two chunk-edge lanes load: lane 0 the
    * site before the chunk, lane `last` (the last active lane) the site after
-   * it, both in the same instructions.  Requires last >= 1 (z tails of 0 or
-   * >= 32 sites, as the callers guarantee). */
+   * it, both in the same instructions (separate ones when the chunk has a
+   * single site). */
   const int z0 = zc * 64;
   const int last = min(63, NZ - 1 - z0);
   const bool e0 = lane == 0, eL = lane == last;
-  const uint32_t oe = (uint32_t)(e0 ? (z0 == 0 ? NZ - 1 : z0 - 1) : (z0 + last + 1 == NZ ? 0 : z0 + last + 1)) * E;
+  const uint32_t oe0 = (uint32_t)(z0 == 0 ? NZ - 1 : z0 - 1) * E;              /* site before the chunk */
+  const uint32_t oeL = (uint32_t)(z0 + last + 1 == NZ ? 0 : z0 + last + 1) * E;  /* site after it */
+  const uint32_t oe = e0 ? oe0 : oeL;
+  /* a one-site chunk (NZ = 64 m + 1): lane 0 is both edges and needs both
+   * sites (wave-uniform) */
+  const bool single = last == 0;
   auto load_plane = [&](int x, PlaneOps<R> &d, R PRc_, R PIc_, R TXc_, R TYc_, R TZc_) {
     const int xp = x == NX - 1 ? 0 : x + 1;
     const uint32_t b = (uint32_t)x * PLB, bn = (uint32_t)xp * PLB;
@@ -424,7 +429,12 @@ __device__ __forceinline__ void ah_march(const MarchConsts<R> &k, const Levels<R
     if (e0) {
       eTZ = bld<R>(c[4], oe, s0); eTZxp = bld<R>(c[4], oe, sN); eTZyp = bld<R>(c[4], oe, sP);
     }
-    if (eL) eTYym = bld<R>(c[3], oe, sM);
+    if (eL) eTYym = bld<R>(c[3], oeL, sM);
+    R pPR = ePR, pPI = ePI, pTX = eTX, pTY = eTY;   /* the site after the chunk, for the edge lane */
+    if (single) {
+      pPR = bld<R>(c[0], oeL, s0); pPI = bld<R>(c[1], oeL, s0);
+      pTX = bld<R>(c[2], oeL, s0); pTY = bld<R>(c[3], oeL, s0);
+    }
     d.PRzm = lane_prev(PRc_, ePR); d.PIzm = lane_prev(PIc_, ePI);
     d.TXzm = lane_prev(TXc_, eTX); d.TYzm = lane_prev(TYc_, eTY); d.TZzm = lane_prev(TZc_, eTZ);
     d.TZxpzm = lane_prev(d.TZn, eTZxp); d.TZypzm = lane_prev(d.TZyp, eTZyp);
@@ -432,8 +442,8 @@ __device__ __forceinline__ void ah_march(const MarchConsts<R> &k, const Levels<R
      * returns the old operand), then the edge lane takes its own load */
     const R nPR = lane_next(PRc_, ePR), nPI = lane_next(PIc_, ePI), nTX = lane_next(TXc_, eTX);
     const R nTY = lane_next(TYc_, eTY), nTYym = lane_next(d.TYym, eTYym);
-    d.PRzp = eL ? ePR : nPR; d.PIzp = eL ? ePI : nPI;
-    d.TXzp = eL ? eTX : nTX; d.TYzp = eL ? eTY : nTY;
+    d.PRzp = eL ? pPR : nPR; d.PIzp = eL ? pPI : nPI;
+    d.TXzp = eL ? pTX : nTX; d.TYzp = eL ? pTY : nTY;
     d.TYymzp = eL ? eTYym : nTYym;
   };
 

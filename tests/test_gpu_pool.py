@@ -161,3 +161,21 @@ def test_failed_requests_free_their_memory():
         s = pool.stats()
         assert s["failed"] == 7 and s["completed"] == 0
         assert s["in_use"] == 0 and s["peak_in_use"] <= budget and s["max_resident"] <= 3
+
+
+@pytest.mark.parametrize("precision", [32, 64])
+def test_pool_short_z_chunks(precision):
+    """Lattice edges whose last z chunk is short -- one site (65), two sites
+    (66) -- or a whole row of 33: the pool marches them (the single-lattice
+    path steps such rows with the site kernel), and each realization matches
+    its single-lattice run bit for bit."""
+    say(f"[pool] short chunks fp{precision}")
+    reqs = [(65, 21, 9), (66, 22, 7), (33, 23, 12)]
+    with ax.Pool(ax.ah_params(8, precision=precision), hbm_budget=2 * GB, chunk_steps=4, autotune=0,
+                 ic_threads=2) as pool:
+        ids = [pool.submit(seed, n, steps, keep_state=True) for n, seed, steps in reqs]
+        assert pool.wait(timeout=120)
+        for rid, (n, seed, steps) in zip(ids, reqs):
+            assert pool.info(rid)["status"] == AX_REQ_DONE
+            check_same(pool, rid, single_run(n, seed, steps, 4, precision))
+            pool.release(rid)
