@@ -15,7 +15,7 @@ struct ax_lattice;
 namespace ax {
 
 /* Work items of a marching kernel: one 64-lane wave per (64-site z chunk,
- * row, x segment of xseg planes), z chunk fastest. */
+ * row, x segment of xseg planes), numbered as make_tile_grid describes. */
 struct TileGrid {
   int nw;      /* z chunks per row */
   int xseg;    /* x planes marched by one wave */

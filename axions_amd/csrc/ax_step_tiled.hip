@@ -1,11 +1,13 @@
 /*
- * ax_step_tiled.hip - Abelian-Higgs step of one periodic lattice, one 64-lane
- * wave per (64-site z chunk, y row, x segment) marching along x (the update
- * itself is ah_march in ax_march.h).  FP64 and FP32.
+ * ax_step_tiled.hip - Abelian-Higgs and Moore steps of one lattice, one
+ * 64-lane wave per (64-site z chunk, y row, x segment) marching along x (the
+ * updates themselves are ah_march in ax_march.h and moore_march in
+ * ax_march_moore.h).  FP64 and FP32, every boundary (boundary planes and rows
+ * are left to the boundary kernels).
  *
- * Work item order: z chunk fastest, then y, then x segment, so the waves
- * resident at one time cover whole neighbouring rows of a few x planes and
- * their y-neighbour reads hit in L2.
+ * Work items are numbered in the XCD-aware order of make_tile_grid
+ * (ax_internal.h): each XCD takes bands of consecutive rows with all their
+ * chunks, so y-neighbour rows and chunk-edge lines are read from its own L2.
  */
 #include <cstdlib>
 #include <limits>
