@@ -181,7 +181,7 @@ AX_HD bool ax_sin_is_small(double x) { return fabs(x * AX_TWO_OVER_PI) <= 0.5; }
 /* ax_sin_kernel(x, +0) without the r_lo term: for finite z that term is
  * fma(-z/2, +0, +0) = +0, and fma(v, p, +0) rounds like v*p except when
  * v*p = -0, which only happens for x = +0 where both sums give +0 (and the
- * zero test below returns x anyway). */
+ * sign copy below returns x's zero anyway). */
 AX_HD double ax_sin_small(double x) {
   double z = x * x;
   double v = z * x;
@@ -191,8 +191,9 @@ AX_HD double ax_sin_small(double x) {
   p = fma(z, p, AX_S2);
   p = fma(z, p, AX_S1);
   p = fma(z, p, AX_S0);
-  double s = x + v * p;
-  return (x == 0.0) ? x : s;
+  /* for 0 < |x| <= pi/4 the sum has the sign of x (|v p| < |x|), so copying
+   * x's sign changes nothing there and gives sin(-0) = -0 */
+  return copysign(x + v * p, x);
 }
 
 /* Both members of ax_sincos(x) for ax_sin_is_small(x): with k = 0 the
@@ -302,8 +303,7 @@ AX_HD float ax_sinf_small(float x) {
   float ps = fmaf(z, 2.7557319e-06f, -1.9841270e-04f);
   ps = fmaf(z, ps, 8.3333333e-03f);
   ps = fmaf(z, ps, -1.6666667e-01f);
-  float sk = fmaf(z * x, ps, x);
-  return (x == 0.0f) ? x : sk;
+  return copysignf(fmaf(z * x, ps, x), x);   /* sign of x, as in ax_sin_small */
 }
 
 /* Both members of ax_sincosf(x) for ax_sinf_is_small(x) (r = x, quadrant 0). */
