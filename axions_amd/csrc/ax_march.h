@@ -10,8 +10,12 @@
  *   - x-backward values (phi(s-x), cos/sin of theta_x(s-x), and the
  *     plaquette sines whose copy at s-x is needed) come from the previous
  *     iteration of the march, kept in registers;
- *   - y- and z-backward values are recomputed from their operands with the
- *     same expressions the neighbour uses for its own forward values;
+ *   - z-backward trig (cos/sin theta_z(s-z) and the F_xz, F_yz sines at
+ *     s-z) is lane - 1's own value, shifted across by DPP; the chunk's first
+ *     lane takes values evaluated for the site before the chunk in one
+ *     vector pass per 64 planes (one plane per lane, kept in LDS);
+ *   - y-backward values are recomputed from their operands with the same
+ *     expressions the neighbour uses for its own forward values;
  *   - the six plaquette sines at s are computed once (the second sine of
  *     each F term in mpi_evolution.cpp:471-487 is the first one of the
  *     neighbour, written with the same operands in the same order), and take
@@ -26,8 +30,9 @@
  * the streaming hint so that they do not push those rows out of L2.
  *
  * A chunk may be partial (nz not a multiple of 64): lanes past the row end
- * load the last site of the row and store nothing.  No lane exchanges data
- * with another, so partial chunks need no special handling beyond that.
+ * load the last site of the row and store nothing; values shifted from
+ * lane - 1 only reach active lanes from active lanes, so partial chunks
+ * need no special handling beyond that.
  */
 #ifndef AXIONS_AX_MARCH_H
 #define AXIONS_AX_MARCH_H
@@ -151,7 +156,6 @@ struct PlaneOps {
   R oPR, oPI, oX, oY, oZ;
   R PRyp, PIyp, PRym, PIym, PRzp, PIzp, PRzm, PIzm;
   R TXyp, TZyp, TXzp, TYzp, TXym, TYym, TZym, TYxpym, TYymzp;
-  R TXzm, TYzm, TZzm, TZxpzm, TZypzm;
 };
 
 /* Values carried from plane x-1 into plane x of the march. */
@@ -160,6 +164,15 @@ struct MarchCarry {
   R PRm, PIm;          /* phi(s-x) */
   R cxm, sxm;          /* cos/sin theta_x(s-x) */
   R Qyx_m, Qzx_m;      /* plaquette sines at s-x needed by F_yx, F_zx */
+};
+
+/* What the site before a chunk (z0 - 1) contributes to the chunk's first
+ * lane: sin/cos theta_z there and its F_xz, F_yz plaquette sines.  Every
+ * other lane takes them from lane - 1, which evaluates them for its own
+ * site with the same operands in the same order. */
+template <typename R>
+struct ZEdge {
+  R s, c, qxz, qyz;
 };
 
 /* The new previous-level values (level n+1) of one site. */
@@ -176,8 +189,7 @@ enum Op {
   kPRn, kPIn, kTXn, kTYn, kTZn,
   koPR, koPI, koX, koY, koZ,
   kPRyp, kPIyp, kPRym, kPIym, kPRzp, kPIzp, kPRzm, kPIzm,
-  kTXyp, kTZyp, kTXzp, kTYzp, kTXym, kTYym, kTZym, kTYxpym, kTYymzp,
-  kTXzm, kTYzm, kTZzm, kTZxpzm, kTZypzm
+  kTXyp, kTZyp, kTXzp, kTYzp, kTXym, kTYym, kTZym, kTYxpym, kTYymzp
 };
 
 /* Operands held in registers (the register march). */
@@ -194,9 +206,7 @@ struct RegOps {
       case kPRym: return D.PRym; case kPIym: return D.PIym; case kPRzp: return D.PRzp; case kPIzp: return D.PIzp;
       case kPRzm: return D.PRzm; case kPIzm: return D.PIzm; case kTXyp: return D.TXyp; case kTZyp: return D.TZyp;
       case kTXzp: return D.TXzp; case kTYzp: return D.TYzp; case kTXym: return D.TXym; case kTYym: return D.TYym;
-      case kTZym: return D.TZym; case kTYxpym: return D.TYxpym; case kTYymzp: return D.TYymzp;
-      case kTXzm: return D.TXzm; case kTYzm: return D.TYzm; case kTZzm: return D.TZzm;
-      case kTZxpzm: return D.TZxpzm; default: return D.TZypzm;
+      case kTZym: return D.TZym; case kTYxpym: return D.TYxpym; default: return D.TYymzp;
     }
   }
 };
@@ -230,22 +240,23 @@ __device__ __forceinline__ void phase_fence() {
 }
 
 /* One site of the fused in-place update (mpi_evolution.cpp:418-501) from its
- * operands g(Op) and the carry from s-x, which it advances to s.  Every lane
- * of the wave must call it (it votes with __all/__any). */
+ * operands g(Op), the carry from s-x, which it advances to s, and the
+ * chunk-edge values ze (used by lane 0 only).  Every lane of the wave must
+ * call it (it votes with __all/__any and shifts values between lanes). */
 template <typename R, bool POW2, bool PHASED, typename G>
 __device__ __forceinline__ void ah_site_update(const MarchConsts<R> &k, bool plain, const G &g, MarchCarry<R> &m,
-                                               SiteOut<R> &o) {
+                                               const ZEdge<R> &ze, SiteOut<R> &o) {
   typedef TrigR<R> T;
   const R dx2 = k.dx2, dy2 = k.dy2, dz2 = k.dz2, dt = k.dt, dt2 = k.dt2;
   const R hl = k.hl, eta2 = k.eta2, m2gg = k.m2gg, fric = k.fric, damp = k.damp;
-  /* links at s, s-y and s-z */
-  const R TXc = g(kTXc), TYc = g(kTYc), TZc = g(kTZc), TYym = g(kTYym), TZzm = g(kTZzm);
-  R cx, sx, cy, sy, cz, sz, cym, sym_, czm, szm;
+  /* links at s and s-y; theta_z(s-z) is lane - 1's theta_z(s) */
+  const R TXc = g(kTXc), TYc = g(kTYc), TZc = g(kTZc), TYym = g(kTYym);
+  R cx, sx, cy, sy, cz, sz, cym, sym_;
   T::sc(TXc, &sx, &cx);
   T::sc(TYc, &sy, &cy);
   T::sc(TZc, &sz, &cz);
   T::sc(TYym, &sym_, &cym);
-  T::sc(TZzm, &szm, &czm);
+  const R szm = lane_prev(sz, ze.s), czm = lane_prev(cz, ze.c);
   phase_fence<PHASED>();
   /* plaquette angles, reference operand order */
   const R TYn = g(kTYn), TZn = g(kTZn), TXyp = g(kTXyp), TXzp = g(kTXzp), TZyp = g(kTZyp), TYzp = g(kTYzp);
@@ -257,23 +268,21 @@ __device__ __forceinline__ void ah_site_update(const MarchConsts<R> &k, bool pla
   const R a_zy = TZc + TYzp - TZyp - TYc;
   const R b_xy = g(kTXym) + g(kTYxpym) - TXc - TYym;
   const R b_zy = g(kTZym) + g(kTYymzp) - TZc - TYym;
-  const R b_xz = g(kTXzm) + g(kTZxpzm) - TXc - TZzm;
-  const R b_yz = g(kTYzm) + g(kTZypzm) - TYc - TZzm;
-  R Qxy, Qxz, Qyx, Qyz, Qzx, Qzy, Qxy_m, Qzy_m, Qxz_m, Qyz_m;
+  R Qxy, Qxz, Qyx, Qyz, Qzx, Qzy, Qxy_m, Qzy_m;
   const bool small = T::small(a_xy) && T::small(a_xz) && T::small(a_yx) && T::small(a_yz) &&
-                     T::small(a_zx) && T::small(a_zy) && T::small(b_xy) && T::small(b_zy) &&
-                     T::small(b_xz) && T::small(b_yz);
+                     T::small(a_zx) && T::small(a_zy) && T::small(b_xy) && T::small(b_zy);
   if (__all(small)) {
     Qxy = T::sn_small(a_xy); Qxz = T::sn_small(a_xz); Qyx = T::sn_small(a_yx);
     Qyz = T::sn_small(a_yz); Qzx = T::sn_small(a_zx); Qzy = T::sn_small(a_zy);
     Qxy_m = T::sn_small(b_xy); Qzy_m = T::sn_small(b_zy);
-    Qxz_m = T::sn_small(b_xz); Qyz_m = T::sn_small(b_yz);
   } else {
     Qxy = T::sn(a_xy); Qxz = T::sn(a_xz); Qyx = T::sn(a_yx);
     Qyz = T::sn(a_yz); Qzx = T::sn(a_zx); Qzy = T::sn(a_zy);
     Qxy_m = T::sn(b_xy); Qzy_m = T::sn(b_zy);
-    Qxz_m = T::sn(b_xz); Qyz_m = T::sn(b_yz);
   }
+  /* the second sines of F_xz, F_yz (arguments TXzm + TZxpzm - TXc - TZzm and
+   * TYzm + TZypzm - TYc - TZzm) are lane - 1's first ones */
+  const R Qxz_m = lane_prev(Qxz, ze.qxz), Qyz_m = lane_prev(Qyz, ze.qyz);
   phase_fence<PHASED>();
 
   /* gauge field first (frees the plaquette sines early) */
@@ -384,10 +393,10 @@ __device__ __forceinline__ void ah_march(const MarchConsts<R> &k, const Levels<R
   }
 
   /* FP32: z+-1 operands come from the neighbouring lanes (DPP) instead of
-   * reloading shifted rows; only the two chunk-edge lanes load: lane 0 the
-   * site before the chunk, lane `last` (the last active lane) the site after
-   * it, both in the same instructions (separate ones when the chunk has a
-   * single site). */
+   * reloading shifted rows; only the two chunk-edge lanes load: lane 0 phi
+   * at the site before the chunk, lane `last` (the last active lane) phi,
+   * theta_x and theta_y at the site after it (the phi loads of both in the
+   * same instructions, separate ones when the chunk has a single site). */
   const int z0 = zc * 64;
   const int last = min(63, NZ - 1 - z0);
   const bool e0 = lane == 0, eL = lane == last;
@@ -417,33 +426,27 @@ __device__ __forceinline__ void ah_march(const MarchConsts<R> &k, const Levels<R
       d.PRzm = bld<R>(c[0], ozm, s0); d.PIzm = bld<R>(c[1], ozm, s0);
       d.TXzp = bld<R>(c[2], ozp, s0); d.TYzp = bld<R>(c[3], ozp, s0);
       d.TYymzp = bld<R>(c[3], ozp, sM);
-      d.TXzm = bld<R>(c[2], ozm, s0); d.TYzm = bld<R>(c[3], ozm, s0); d.TZzm = bld<R>(c[4], ozm, s0);
-      d.TZxpzm = bld<R>(c[4], ozm, sN); d.TZypzm = bld<R>(c[4], ozm, sP);
       return;
     }
-    R ePR = PRc_, ePI = PIc_, eTX = TXc_, eTY = TYc_, eTZ = TZc_, eTZxp = TZc_, eTZyp = TZc_, eTYym = TYc_;
+    R ePR = PRc_, ePI = PIc_, eTX = TXc_, eTY = TYc_, eTYym = TYc_;
     if (e0 || eL) {
       ePR = bld<R>(c[0], oe, s0); ePI = bld<R>(c[1], oe, s0);
-      eTX = bld<R>(c[2], oe, s0); eTY = bld<R>(c[3], oe, s0);
     }
-    if (e0) {
-      eTZ = bld<R>(c[4], oe, s0); eTZxp = bld<R>(c[4], oe, sN); eTZyp = bld<R>(c[4], oe, sP);
+    if (eL) {
+      eTX = bld<R>(c[2], oeL, s0); eTY = bld<R>(c[3], oeL, s0);
+      eTYym = bld<R>(c[3], oeL, sM);
     }
-    if (eL) eTYym = bld<R>(c[3], oeL, sM);
-    R pPR = ePR, pPI = ePI, pTX = eTX, pTY = eTY;   /* the site after the chunk, for the edge lane */
+    R pPR = ePR, pPI = ePI;   /* phi after the chunk, for the edge lane */
     if (single) {
       pPR = bld<R>(c[0], oeL, s0); pPI = bld<R>(c[1], oeL, s0);
-      pTX = bld<R>(c[2], oeL, s0); pTY = bld<R>(c[3], oeL, s0);
     }
     d.PRzm = lane_prev(PRc_, ePR); d.PIzm = lane_prev(PIc_, ePI);
-    d.TXzm = lane_prev(TXc_, eTX); d.TYzm = lane_prev(TYc_, eTY); d.TZzm = lane_prev(TZc_, eTZ);
-    d.TZxpzm = lane_prev(d.TZn, eTZxp); d.TZypzm = lane_prev(d.TZyp, eTZyp);
     /* the shifts run with every lane active (a DPP read of an inactive lane
      * returns the old operand), then the edge lane takes its own load */
     const R nPR = lane_next(PRc_, ePR), nPI = lane_next(PIc_, ePI), nTX = lane_next(TXc_, eTX);
     const R nTY = lane_next(TYc_, eTY), nTYym = lane_next(d.TYym, eTYym);
     d.PRzp = eL ? pPR : nPR; d.PIzp = eL ? pPI : nPI;
-    d.TXzp = eL ? pTX : nTX; d.TYzp = eL ? pTY : nTY;
+    d.TXzp = eL ? eTX : nTX; d.TYzp = eL ? eTY : nTY;
     d.TYymzp = eL ? eTYym : nTYym;
   };
 
@@ -458,12 +461,38 @@ __device__ __forceinline__ void ah_march(const MarchConsts<R> &k, const Levels<R
   R PRc = ldo(PR, b0 + R0, oz), PIc = ldo(PI, b0 + R0, oz);
   R TXc = ldo(TX, b0 + R0, oz), TYc = ldo(TY, b0 + R0, oz), TZc = ldo(TZ, b0 + R0, oz);
 
+  /* Lane 0's z-backward values (ZEdge) for up to 64 planes at a time: lane
+   * j evaluates them at (xb + j, y, z0 - 1) with the operands and
+   * expressions that site's own update uses, into LDS (the workgroup is
+   * this one wave).  One vector evaluation per 64 planes instead of a
+   * sincos and two sines per plane. */
+  __shared__ ZEdge<R> zedge[64];
+  const uint32_t em = R0 + oe0, emP = RP + oe0, ec = R0 + (uint32_t)z0 * E;
+  auto edge_block = [&](int xb, int xe) {
+    const int xj = min(xb + lane, xe - 1);
+    const int xjp = xj == NX - 1 ? 0 : xj + 1;
+    const uint32_t pj = (uint32_t)xj * PLB, pn = (uint32_t)xjp * PLB;
+    const R TZe = bld<R>(c[4], pj + em, 0), TXe = bld<R>(c[2], pj + em, 0), TYe = bld<R>(c[3], pj + em, 0);
+    const R TZxe = bld<R>(c[4], pn + em, 0), TZye = bld<R>(c[4], pj + emP, 0);
+    const R TX0 = bld<R>(c[2], pj + ec, 0), TY0 = bld<R>(c[3], pj + ec, 0);
+    ZEdge<R> e;
+    TrigR<R>::sc(TZe, &e.s, &e.c);
+    e.qxz = TrigR<R>::sn(TXe + TZxe - TX0 - TZe);   /* a_xz at (xj, y, z0 - 1) */
+    e.qyz = TrigR<R>::sn(TYe + TZye - TY0 - TZe);   /* a_yz there */
+    __syncthreads();   /* the previous block's values have been read */
+    if (lane < xe - xb) zedge[lane] = e;
+    __syncthreads();
+  };
+
   for (int x = x0; x < x1; ++x) {
+    const int i = (x - x0) & 63;
+    if (i == 0) edge_block(x, min(x1, x + 64));
+    const ZEdge<R> ze = zedge[i];
     PlaneOps<R> D;
     load_plane(x, D, PRc, PIc, TXc, TYc, TZc);
     const uint32_t so = (uint32_t)x * PLB + R0;
     SiteOut<R> out;
-    ah_site_update<R, POW2, PHASED>(k, plain, RegOps<R>{D, PRc, PIc, TXc, TYc, TZc}, m, out);
+    ah_site_update<R, POW2, PHASED>(k, plain, RegOps<R>{D, PRc, PIc, TXc, TYc, TZc}, m, ze, out);
     if (act) {
       bst<R, kStream>(o[2], oz, so, out.X);
       bst<R, kStream>(o[3], oz, so, out.Y);
