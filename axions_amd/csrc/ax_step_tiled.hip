@@ -87,14 +87,15 @@ bool use_tiled_ah(const ax_params &p) {
          getenv("AX_FORCE_SITE_KERNEL") == nullptr;
 }
 
-/* x planes per wave (measured at 512^3 AH, 256^3 Moore): Abelian-Higgs FP64
- * 32 (4.04 ms; 16: 4.04, 64: 4.09), FP32 16 (1.787 ms; 32: 1.80, 64: 1.87);
- * Moore 8 (FP64 1.012 ms, FP32 0.422; 32: 1.02 / 0.444) */
+/* x planes per wave (measured at 512^3 AH, 256^3 Moore, on one box each):
+ * Abelian-Higgs FP64 16 (3.94 ms; 32: 3.98, 64: 4.09), FP32 16 (1.834 ms;
+ * 32: 1.89, 64: 1.99); Moore 8 (FP64 1.012 ms, FP32 0.422; 32: 1.02 /
+ * 0.444) */
 int default_xseg(const ax_params &p) {
   const char *xe = getenv("AX_TILED_XSEG");
   if (xe) return atoi(xe);
   if (p.model == AX_MODEL_MOORE) return 8;
-  return p.precision == 32 ? 16 : 32;
+  return 16;
 }
 
 template <typename R>
