@@ -88,13 +88,6 @@ __device__ __forceinline__ R mdiv(R a, R d, R inv) {
   return a / d;
 }
 
-/* fric*((a-b)/dt) when fric is +0 (plain run): 0*(a-b) has the sign and
- * NaN-ness of the quotient unless (a-b)/dt overflows, which *ovf reports. */
-template <typename R>
-__device__ __forceinline__ R zero_rate(R diff, R lim, bool *ovf) {
-  *ovf = *ovf || !(fabs(diff) <= lim);
-  return (R)0 * diff;
-}
 
 /* base + wave-uniform row byte offset + per-lane byte offset; both offsets
  * are 32-bit (every field plane is below 4 GiB), which lets the compiler use
@@ -232,6 +225,18 @@ __device__ __forceinline__ float lane_next(float v, float edge) {
   return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(edge), __float_as_int(v), 0x130, 0xF, 0xF, false));
 }
 
+/* The lanes whose sine argument a takes the short form. */
+template <typename R>
+__device__ __forceinline__ uint64_t small_lanes(R a) {
+  return __builtin_amdgcn_ballot_w64(TrigR<R>::small(a));
+}
+
+/* The lanes where |d| exceeds lim or is NaN. */
+template <typename R>
+__device__ __forceinline__ uint64_t big_lanes(R d, R lim) {
+  return __builtin_amdgcn_ballot_w64(!(fabs(d) <= lim));
+}
+
 /* Keeps the compiler from moving instructions across phase boundaries, so
  * operands fetched on demand (LDS) are not all hoisted to the top. */
 template <bool PHASED>
@@ -269,9 +274,11 @@ __device__ __forceinline__ void ah_site_update(const MarchConsts<R> &k, bool pla
   const R b_xy = g(kTXym) + g(kTYxpym) - TXc - TYym;
   const R b_zy = g(kTZym) + g(kTYymzp) - TZc - TYym;
   R Qxy, Qxz, Qyx, Qyz, Qzx, Qzy, Qxy_m, Qzy_m;
-  const bool small = T::small(a_xy) && T::small(a_xz) && T::small(a_yx) && T::small(a_yz) &&
-                     T::small(a_zx) && T::small(a_zy) && T::small(b_xy) && T::small(b_zy);
-  if (__all(small)) {
+  /* the wave takes the short form when all its lanes' arguments qualify:
+   * one ballot per argument, combined on the scalar unit */
+  const uint64_t small = small_lanes(a_xy) & small_lanes(a_xz) & small_lanes(a_yx) & small_lanes(a_yz) &
+                         small_lanes(a_zx) & small_lanes(a_zy) & small_lanes(b_xy) & small_lanes(b_zy);
+  if (small == __builtin_amdgcn_read_exec()) {
     Qxy = T::sn_small(a_xy); Qxz = T::sn_small(a_xz); Qyx = T::sn_small(a_yx);
     Qyz = T::sn_small(a_yz); Qzx = T::sn_small(a_zx); Qzy = T::sn_small(a_zy);
     Qxy_m = T::sn_small(b_xy); Qzy_m = T::sn_small(b_zy);
@@ -301,14 +308,15 @@ __device__ __forceinline__ void ah_site_update(const MarchConsts<R> &k, bool pla
   R gX, gY, gZ, f0, f1;
   bool literal = !plain;
   if (plain) {
-    bool ovf = false;
-    gX = zero_rate(TXc - oX, k.lim, &ovf);
-    gY = zero_rate(TYc - oY, k.lim, &ovf);
-    gZ = zero_rate(TZc - oZ, k.lim, &ovf);
-    f0 = zero_rate(PRc - oPR, k.lim, &ovf);
-    f1 = zero_rate(PIc - oPI, k.lim, &ovf);
-    /* some |a-b|/dt overflows: take the literal products (never in a sane state) */
-    literal = __any(ovf);
+    /* fric*((a-b)/dt) with fric = +0: 0*(a-b) has the sign and NaN-ness of
+     * the quotient unless (a-b)/dt overflows */
+    const R dX = TXc - oX, dY = TYc - oY, dZ = TZc - oZ, d0 = PRc - oPR, d1 = PIc - oPI;
+    gX = (R)0 * dX; gY = (R)0 * dY; gZ = (R)0 * dZ;
+    f0 = (R)0 * d0; f1 = (R)0 * d1;
+    /* some |a-b| above k.lim (or NaN) on some lane: take the literal
+     * products (never in a sane state) */
+    literal = (big_lanes(dX, k.lim) | big_lanes(dY, k.lim) | big_lanes(dZ, k.lim) | big_lanes(d0, k.lim) |
+               big_lanes(d1, k.lim)) != 0;
   }
   if (literal) {
     f0 = fric * ((PRc - oPR) / dt);
