@@ -111,10 +111,10 @@ __device__ __forceinline__ void moore_march(const MooreConsts<R> &k, const Level
     const R b_xz = TXzm + TZxpzm - X - TZzm;
     const R b_yz = TYzm + TZypzm - Y - TZzm;
     R Qxy, Qxz, Qyx, Qyz, Qzx, Qzy, Qxy_m, Qzy_m, Qxz_m, Qyz_m;
-    const bool small = T::small(a_xy) && T::small(a_xz) && T::small(a_yx) && T::small(a_yz) &&
-                       T::small(a_zx) && T::small(a_zy) && T::small(b_xy) && T::small(b_zy) &&
-                       T::small(b_xz) && T::small(b_yz);
-    if (__all(small)) {
+    const uint64_t small = small_lanes(a_xy) & small_lanes(a_xz) & small_lanes(a_yx) & small_lanes(a_yz) &
+                           small_lanes(a_zx) & small_lanes(a_zy) & small_lanes(b_xy) & small_lanes(b_zy) &
+                           small_lanes(b_xz) & small_lanes(b_yz);
+    if (small == __builtin_amdgcn_read_exec()) {
       Qxy = T::sn_small(a_xy); Qxz = T::sn_small(a_xz); Qyx = T::sn_small(a_yx);
       Qyz = T::sn_small(a_yz); Qzx = T::sn_small(a_zx); Qzy = T::sn_small(a_zy);
       Qxy_m = T::sn_small(b_xy); Qzy_m = T::sn_small(b_zy); Qxz_m = T::sn_small(b_xz); Qyz_m = T::sn_small(b_yz);
