@@ -13,7 +13,7 @@
  *   - z-backward trig (cos/sin theta_z(s-z) and the F_xz, F_yz sines at
  *     s-z) is lane - 1's own value, shifted across by DPP; the chunk's first
  *     lane takes values evaluated for the site before the chunk in one
- *     vector pass per 64 planes (one plane per lane, kept in LDS);
+ *     vector pass per few planes (one plane per lane, kept in LDS);
  *   - y-backward values are recomputed from their operands with the same
  *     expressions the neighbour uses for its own forward values;
  *   - the six plaquette sines at s are computed once (the second sine of
@@ -469,12 +469,16 @@ __device__ __forceinline__ void ah_march(const MarchConsts<R> &k, const Levels<R
   R PRc = ldo(PR, b0 + R0, oz), PIc = ldo(PI, b0 + R0, oz);
   R TXc = ldo(TX, b0 + R0, oz), TYc = ldo(TY, b0 + R0, oz), TZc = ldo(TZ, b0 + R0, oz);
 
-  /* Lane 0's z-backward values (ZEdge) for up to 64 planes at a time: lane
-   * j evaluates them at (xb + j, y, z0 - 1) with the operands and
-   * expressions that site's own update uses, into LDS (the workgroup is
-   * this one wave).  One vector evaluation per 64 planes instead of a
-   * sincos and two sines per plane. */
-  __shared__ ZEdge<R> zedge[64];
+  /* Lane 0's z-backward values (ZEdge) for EB planes at a time: lane j
+   * evaluates them at (xb + j, y, z0 - 1) with the operands and expressions
+   * that site's own update uses, into LDS (the workgroup is this one wave).
+   * One vector evaluation per EB planes instead of a sincos and two sines
+   * per plane.  The pass reads its planes' lines ahead of the march; over
+   * longer blocks (16 planes: +0.4 GB FP64, +1.7 GB FP32 per 512^3 step)
+   * they are evicted from L2 before the march reaches them and are read
+   * twice, so the blocks are short (measured best: FP64 8, FP32 4). */
+  constexpr int EB = sizeof(R) == 8 ? 8 : 4;
+  __shared__ ZEdge<R> zedge[EB];
   const uint32_t em = R0 + oe0, emP = RP + oe0, ec = R0 + (uint32_t)z0 * E;
   auto edge_block = [&](int xb, int xe) {
     const int xj = min(xb + lane, xe - 1);
@@ -493,8 +497,8 @@ __device__ __forceinline__ void ah_march(const MarchConsts<R> &k, const Levels<R
   };
 
   for (int x = x0; x < x1; ++x) {
-    const int i = (x - x0) & 63;
-    if (i == 0) edge_block(x, min(x1, x + 64));
+    const int i = (x - x0) % EB;
+    if (i == 0) edge_block(x, min(x1, x + EB));
     const ZEdge<R> ze = zedge[i];
     PlaneOps<R> D;
     load_plane(x, D, PRc, PIc, TXc, TYc, TZc);
