@@ -430,9 +430,13 @@ __device__ __forceinline__ double link_phase_sc(double cr, double ci, double nr,
 
 /* P2: dx, dy, dz and g (when g != 0) are powers of two, so every division by
  * dx, dx*dx, g or (g*dx*dy)^2 is the multiplication by its exact reciprocal
- * (the same correctly rounded value).  Divisions by dt stay divisions. */
+ * (the same correctly rounded value).  Divisions by dt stay divisions.
+ * FP64 is held to 3 waves per SIMD (the phase-storing variant needs 173
+ * VGPRs otherwise, i.e. 2 waves; capped it spills 4 and runs 3.5 % faster
+ * per 512^3 call); FP32 keeps the compiler's choice (already 3 waves; the
+ * same cap there changed its schedule and cost 36 %). */
 template <typename T, bool PH, bool P2>
-__global__ void __launch_bounds__(64) diag_march_kernel(DiagConsts k, Levels<T> L, TileGrid g, double *phase,
+__global__ void __launch_bounds__(64, sizeof(T) == 8 ? 3 : 1) diag_march_kernel(DiagConsts k, Levels<T> L, TileGrid g, double *phase,
                                                         double *part) {
   const int w = (int)blockIdx.x;
   int zc, y, seg;
