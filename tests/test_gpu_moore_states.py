@@ -1,0 +1,58 @@
+"""Moore FP64 marching step with a non-power-of-two dx^2 on states with an
+exact vacuum block (Laplacian numerators +-0, plaquette sines 0) and a block
+of tiny fields (numerators near 1e-280, products underflowing): the marching
+kernel must equal the site kernel bit for bit, as it does on random
+states."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import axions_amd as ax
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def run(p, st, nsteps, site):
+    if site:
+        os.environ["AX_FORCE_SITE_KERNEL"] = "1"
+    try:
+        with ax.Lattice(p) as lat:
+            assert lat.tile()[1] == (not site)
+            lat.set_state(st, timestep=0)
+            lat.step(nsteps)
+            return lat.get_state()
+    finally:
+        os.environ.pop("AX_FORCE_SITE_KERNEL", None)
+
+
+@pytest.mark.parametrize("case", ["vacuum_block", "tiny_block", "random"])
+def test_moore_march_equals_site_kernel_on_special_states(case):
+    p = ax.moore_params(12, ny=10, nz=64, dt=0.2)   # dx = 0.7: dx^2 is not a power of two
+    say(f"[moore div] {case}")
+    with ax.Lattice(p) as lat:
+        lat.init_random_phase(5)
+        lat.step(3)
+        st = lat.get_state()
+    f = st.reshape(7, 2, p.nx, p.ny, p.nz)
+    if case == "vacuum_block":
+        # phi_1 = phi_2 = (1, 0), theta = 0 on both levels over a block that
+        # covers whole z chunks' interiors: Laplacian numerators there are 0
+        f[:, :, 2:9, 2:8, 5:40] = 0.0
+        f[0, :, 2:9, 2:8, 5:40] = 1.0
+        f[2, :, 2:9, 2:8, 5:40] = 1.0
+    elif case == "tiny_block":
+        # fields of order 1e-280 (and theta 0) over a block
+        rng = np.random.default_rng(7)
+        blk = f[:, :, 3:8, 2:7, 10:30]
+        blk[:4] = rng.standard_normal(blk[:4].shape) * 1e-280
+        blk[4:] = 0.0
+    out_m = run(p, st, 4, site=False)
+    out_s = run(p, st, 4, site=True)
+    assert np.array_equal(out_m.view(np.uint64), out_s.view(np.uint64))
