@@ -88,7 +88,6 @@ __device__ __forceinline__ R mdiv(R a, R d, R inv) {
   return a / d;
 }
 
-
 /* base + wave-uniform row byte offset + per-lane byte offset; both offsets
  * are 32-bit (every field plane is below 4 GiB), which lets the compiler use
  * the scalar-base + 32-bit vector-offset load form. */
@@ -237,18 +236,12 @@ __device__ __forceinline__ uint64_t big_lanes(R d, R lim) {
   return __builtin_amdgcn_ballot_w64(!(fabs(d) <= lim));
 }
 
-/* Keeps the compiler from moving instructions across phase boundaries, so
- * operands fetched on demand (LDS) are not all hoisted to the top. */
-template <bool PHASED>
-__device__ __forceinline__ void phase_fence() {
-  if (PHASED) __builtin_amdgcn_sched_barrier(0);
-}
-
 /* One site of the fused in-place update (mpi_evolution.cpp:418-501) from its
  * operands g(Op), the carry from s-x, which it advances to s, and the
  * chunk-edge values ze (used by lane 0 only).  Every lane of the wave must
- * call it (it votes with __all/__any and shifts values between lanes). */
-template <typename R, bool POW2, bool PHASED, typename G>
+ * call it (it votes across the wave with ballots and shifts values between
+ * lanes). */
+template <typename R, bool POW2, typename G>
 __device__ __forceinline__ void ah_site_update(const MarchConsts<R> &k, bool plain, const G &g, MarchCarry<R> &m,
                                                const ZEdge<R> &ze, SiteOut<R> &o) {
   typedef TrigR<R> T;
@@ -262,7 +255,6 @@ __device__ __forceinline__ void ah_site_update(const MarchConsts<R> &k, bool pla
   T::sc(TZc, &sz, &cz);
   T::sc(TYym, &sym_, &cym);
   const R szm = lane_prev(sz, ze.s), czm = lane_prev(cz, ze.c);
-  phase_fence<PHASED>();
   /* plaquette angles, reference operand order */
   const R TYn = g(kTYn), TZn = g(kTZn), TXyp = g(kTXyp), TXzp = g(kTXzp), TZyp = g(kTZyp), TYzp = g(kTYzp);
   const R a_xy = TXc + TYn - TXyp - TYc;
@@ -290,7 +282,6 @@ __device__ __forceinline__ void ah_site_update(const MarchConsts<R> &k, bool pla
   /* the second sines of F_xz, F_yz (arguments TXzm + TZxpzm - TXc - TZzm and
    * TYzm + TZypzm - TYc - TZzm) are lane - 1's first ones */
   const R Qxz_m = lane_prev(Qxz, ze.qxz), Qyz_m = lane_prev(Qyz, ze.qyz);
-  phase_fence<PHASED>();
 
   /* gauge field first (frees the plaquette sines early) */
   const R fxy = mdiv<R, POW2>(Qxy - Qxy_m, dy2, k.inv_dy2);
@@ -333,7 +324,6 @@ __device__ __forceinline__ void ah_site_update(const MarchConsts<R> &k, bool pla
   o.X = 2 * TXc - oX + dt2 * ax_;
   o.Y = 2 * TYc - oY + dt2 * ay_;
   o.Z = 2 * TZc - oZ + dt2 * az_;
-  phase_fence<PHASED>();
 
   /* scalar field */
   const R PRym = g(kPRym), PIym = g(kPIym), PRzm = g(kPRzm), PIzm = g(kPIzm);
@@ -374,7 +364,7 @@ __device__ __forceinline__ void march_prologue(Get G, int xm0, int x0, MarchCarr
 
 /* PLAIN: 0 friction products always evaluated, 1 fric = damp = +0 known at
  * compile time, 2 decided at run time from k.plain (wave-uniform). */
-template <typename R, bool POW2, int PLAIN, bool PHASED = false>
+template <typename R, bool POW2, int PLAIN>
 __device__ __forceinline__ void ah_march(const MarchConsts<R> &k, const Levels<R> &L, int NX, int NY, int NZ,
                                          int y, int zc, int x0, int x1, bool zfix = false) {
   const int lane = (int)(threadIdx.x & 63);
@@ -504,7 +494,7 @@ __device__ __forceinline__ void ah_march(const MarchConsts<R> &k, const Levels<R
     load_plane(x, D, PRc, PIc, TXc, TYc, TZc);
     const uint32_t so = (uint32_t)x * PLB + R0;
     SiteOut<R> out;
-    ah_site_update<R, POW2, PHASED>(k, plain, RegOps<R>{D, PRc, PIc, TXc, TYc, TZc}, m, ze, out);
+    ah_site_update<R, POW2>(k, plain, RegOps<R>{D, PRc, PIc, TXc, TYc, TZc}, m, ze, out);
     if (act) {
       bst<R, kStream>(o[2], oz, so, out.X);
       bst<R, kStream>(o[3], oz, so, out.Y);
