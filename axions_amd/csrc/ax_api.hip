@@ -114,6 +114,8 @@ struct ax_lattice {
   long long *ctotal = nullptr;
   double *cxyz = nullptr;
   size_t cxyz_cap = 0;
+  int *band = nullptr;      /* Moore: per-level band flags of the FP64 march (ax_march_moore.h), TimeStep mod 3 */
+  int band_steps = 0;       /* marching steps since the state or TimeStep was last set from the host */
   bool timing = false;                         /* ax_kernel_timer */
   std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
   int64_t timed_steps = 0;
@@ -141,7 +143,10 @@ const ax_params &lattice_params(const ax_lattice *L) { return L->p; }
 void *lattice_state(ax_lattice *L) { return L->state; }
 size_t lattice_sites(const ax_lattice *L) { return L->V; }
 size_t lattice_pitch(const ax_lattice *L) { return L->pitch; }
-void lattice_reset_timestep(ax_lattice *L) { L->timestep = 0; }
+void lattice_reset_timestep(ax_lattice *L) {
+  L->timestep = 0;
+  L->band_steps = 0;
+}
 
 }  // namespace ax
 
@@ -222,6 +227,7 @@ int ax_create(const ax_params *p, ax_lattice **out) {
   const size_t nb = diag_part_blocks(*p);
   if (hipMalloc(&L->state, bytes) != hipSuccess || hipMalloc((void **)&L->part, 3 * nb * sizeof(double)) != hipSuccess ||
       hipMalloc((void **)&L->dres, 16 * sizeof(double)) != hipSuccess ||
+      (p->model == AX_MODEL_MOORE && hipMalloc((void **)&L->band, 3 * sizeof(int)) != hipSuccess) ||
       hipHostMalloc((void **)&L->hres, 16 * sizeof(double), hipHostMallocDefault) != hipSuccess) {
     (void)hipGetLastError();   /* clear the sticky error of the failed call */
     ax_destroy(L);
@@ -246,6 +252,7 @@ void ax_destroy(ax_lattice *L) {
   if (L->state) (void)hipFree(L->state);
   if (L->part) (void)hipFree(L->part);
   if (L->dres) (void)hipFree(L->dres);
+  if (L->band) (void)hipFree(L->band);
   if (L->hres) (void)hipHostFree(L->hres);
   if (L->wind) (void)hipFree(L->wind);
   if (L->phase) (void)hipFree(L->phase);
@@ -266,6 +273,7 @@ int ax_get_params(const ax_lattice *L, ax_params *out) {
 int ax_set_timestep(ax_lattice *L, int32_t ts) {
   if (!L || ts < 0) return fail(AX_EINVAL, "bad timestep");
   L->timestep = ts;
+  L->band_steps = 0;
   return AX_OK;
 }
 
@@ -315,6 +323,7 @@ int ax_set_state(ax_lattice *L, const double *host, size_t n) {
     if (rc) return rc;
   }
   hipError_t e = hipStreamSynchronize(L->stream);
+  L->band_steps = 0;
   return e == hipSuccess ? AX_OK : hip_fail(e, "set_state");
 }
 
@@ -396,6 +405,7 @@ int ax_init_random_phase(ax_lattice *L, uint32_t seed, int32_t mode) {
     if (e != hipSuccess) return hip_fail(e, "init");
   }
   L->timestep = 0;
+  L->band_steps = 0;
   return AX_OK;
 }
 
@@ -729,17 +739,22 @@ int launch_step(ax_lattice *L, int n) {
                       : launch_absorbing<float>(k, L->p, L->state, L->pitch, tN, tP, L->stream);
     if (e != hipSuccess) return hip_fail(e, "absorbing boundary launch");
   }
+  /* the band flag of this step's current level is usable when the step
+   * before it was a marching step that kept the flags */
+  bool banded = false;
+  const bool use_band = L->band_steps > 0;
   if (L->esize == 8) {
     Levels<double> lv = levels_of<double>(L, n);
     e = !tiled ? launch_step_site<double>(k, lv, L->p.model, L->stream)
-               : moore ? launch_step_tiled_moore<double>(k, lv, L->xseg, L->stream)
+               : moore ? launch_step_tiled_moore<double>(k, lv, L->xseg, L->stream, L->band, n, use_band, &banded)
                        : launch_step_tiled_ah<double>(k, lv, L->xseg, L->stream);
   } else {
     Levels<float> lv = levels_of<float>(L, n);
     e = !tiled ? launch_step_site<float>(k, lv, L->p.model, L->stream)
-               : moore ? launch_step_tiled_moore<float>(k, lv, L->xseg, L->stream)
+               : moore ? launch_step_tiled_moore<float>(k, lv, L->xseg, L->stream, L->band, n, use_band, &banded)
                        : launch_step_tiled_ah<float>(k, lv, L->xseg, L->stream);
   }
+  L->band_steps = banded ? L->band_steps + 1 : 0;
   return e == hipSuccess ? AX_OK : hip_fail(e, "step launch");
 }
 

@@ -106,8 +106,12 @@ bool use_tiled_ah(const ax_params &p);
 int default_xseg(const ax_params &p);
 template <typename R>
 hipError_t launch_step_tiled_ah(const StepConsts &k, const Levels<R> &L, int xseg, hipStream_t st);
+/* band: the lattice's three per-level band flags (nullptr: none); t the
+ * TimeStep; use_band: band[t % 3] was written by the previous step; *fd_used
+ * tells whether the kernel kept the flags (FP64 FixedDivisor path) */
 template <typename R>
-hipError_t launch_step_tiled_moore(const StepConsts &k, const Levels<R> &L, int xseg, hipStream_t st);
+hipError_t launch_step_tiled_moore(const StepConsts &k, const Levels<R> &L, int xseg, hipStream_t st, int *band, int t,
+                                   bool use_band, bool *fd_used);
 
 template <typename R>
 hipError_t launch_global_step(const ax_params &p, void *state, size_t pitch, int n, hipStream_t st);

@@ -22,6 +22,44 @@ struct MooreConsts {
   R inv_dx2, inv_dy2, inv_dz2;
 };
 
+/* Division by a divisor that is the same for the whole launch, with the
+ * bits of the compiler's IEEE division a / d.  That division is: y = rcp(d)
+ * refined by two Newton steps, q = a*y, r = fma(-d, q, a), fma(r, y, q),
+ * with range scaling (v_div_scale) and special-case repair (v_div_fixup)
+ * around it that change nothing while d and a/d are far from the exponent
+ * limits.  With d in [2^-100, 2^100] (checked by the launcher) and a = 0 or
+ * |a| in [2^-800, 2^600] the steps after y are done here as they stand, and
+ * y once per wave (+-0 take a's sign: the bare steps return +0 for -0).
+ *
+ * The numerators are sums of products of the current level's values and
+ * their sines and cosines.  When every value of that level is 0 or has
+ * magnitude in [2^-250, 2^250] (kMooreBand), every such product is 0 or a
+ * multiple of 2^-605, so every numerator is 0 or in [2^-605, 2^254]: the
+ * step that writes a level checks its outputs against that band and ORs a
+ * per-level flag, and the next step divides this way only when the flag of
+ * its current level is clear. */
+struct FixedDivisor {
+  double d, y;
+};
+__device__ __forceinline__ FixedDivisor fixed_divisor(double d) {
+  const double r0 = __builtin_amdgcn_rcp(d);
+  const double r1 = fma(r0, fma(-d, r0, 1.0), r0);
+  const double y = fma(r1, fma(-d, r1, 1.0), r1);
+  /* the same on every lane: kept in scalar registers */
+  const uint64_t b = __double_as_longlong(y);
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)b), hi = __builtin_amdgcn_readfirstlane((uint32_t)(b >> 32));
+  return {d, __longlong_as_double((long long)(((uint64_t)hi << 32) | lo))};
+}
+__device__ __forceinline__ double fixed_div(double a, const FixedDivisor &v) {
+  const double q = a * v.y;
+  return copysign(fma(fma(-v.d, q, a), v.y, q), a);
+}
+inline bool fixed_divisor_range(double d) { return d >= 0x1p-100 && d <= 0x1p100; }
+__device__ __forceinline__ bool in_moore_band(double v) {
+  const double m = fabs(v);
+  return m <= 0x1p250 && (m >= 0x1p-250 || m == 0);
+}
+
 template <typename R>
 inline void moore_consts_from(const StepConsts &k, MooreConsts<R> &m) {
   m.dx2 = (R)k.dx2; m.dy2 = (R)k.dy2; m.dz2 = (R)k.dz2; m.dt2 = (R)k.dt2; m.m2gg = (R)k.m2gg;
@@ -33,10 +71,14 @@ inline void moore_consts_from(const StepConsts &k, MooreConsts<R> &m) {
 /* BUF: the loop's loads and stores are raw buffer accesses, one resource per
  * field spanning both of its levels (planes 2c, 2c+1; the caller checks that
  * two planes fit 32-bit offsets), with the streaming hint on the previous
- * level as in ah_march; otherwise plain global accesses. */
-template <typename R, bool POW2, bool BUF = false>
+ * level as in ah_march; otherwise plain global accesses.
+ * FASTDIV (FP64, divisors not powers of two): divisions by dx^2, dy^2,
+ * dz^2 through FixedDivisor; the caller has checked the current level's
+ * band flag.  CHECK: OR 1 into *bad when a written value lies outside the
+ * band (every lane of the wave must reach the end). */
+template <typename R, bool POW2, bool BUF = false, bool FASTDIV = false, bool CHECK = false>
 __device__ __forceinline__ void moore_march(const MooreConsts<R> &k, const Levels<R> &L, int NX, int NY, int NZ, int y,
-                                            int zc, int x0, int x1, bool zfix = false) {
+                                            int zc, int x0, int x1, bool zfix = false, int *bad = nullptr) {
   typedef TrigR<R> T;
   const int lane = (int)(threadIdx.x & 63);
   const int zr = zc * 64 + lane;
@@ -54,6 +96,17 @@ __device__ __forceinline__ void moore_march(const MooreConsts<R> &k, const Level
   R *O[7] = {L.prv[0], L.prv[1], L.prv[2], L.prv[3], L.prv[4], L.prv[5], L.prv[6]};
   const R *TX = F[4], *TY = F[5], *TZ = F[6];
   const R q[2] = {k.q1, k.q2};
+  FixedDivisor fdx = {}, fdy = {}, fdz = {};
+  if constexpr (FASTDIV) {
+    fdx = fixed_divisor((double)k.dx2);
+    fdy = fixed_divisor((double)k.dy2);
+    fdz = fixed_divisor((double)k.dz2);
+  }
+  /* a / d_a for axis a (x, y, z) */
+  auto dvx = [&](R a) -> R { if constexpr (FASTDIV) return fixed_div(a, fdx); else return mdiv<R, POW2>(a, k.dx2, k.inv_dx2); };
+  auto dvy = [&](R a) -> R { if constexpr (FASTDIV) return fixed_div(a, fdy); else return mdiv<R, POW2>(a, k.dy2, k.inv_dy2); };
+  auto dvz = [&](R a) -> R { if constexpr (FASTDIV) return fixed_div(a, fdz); else return mdiv<R, POW2>(a, k.dz2, k.inv_dz2); };
+  bool inband = true;
   /* field c's lower plane, and the byte offsets of the two levels from it */
   __amdgpu_buffer_rsrc_t FR[7];
   uint32_t offC = 0, offP = 0;
@@ -123,12 +176,12 @@ __device__ __forceinline__ void moore_march(const MooreConsts<R> &k, const Level
       Qyz = T::sn(a_yz); Qzx = T::sn(a_zx); Qzy = T::sn(a_zy);
       Qxy_m = T::sn(b_xy); Qzy_m = T::sn(b_zy); Qxz_m = T::sn(b_xz); Qyz_m = T::sn(b_yz);
     }
-    const R fxy = mdiv<R, POW2>(Qxy - Qxy_m, k.dy2, k.inv_dy2);
-    const R fxz = mdiv<R, POW2>(Qxz - Qxz_m, k.dz2, k.inv_dz2);
-    const R fyx = mdiv<R, POW2>(Qyx - Qyx_m, k.dx2, k.inv_dx2);
-    const R fyz = mdiv<R, POW2>(Qyz - Qyz_m, k.dz2, k.inv_dz2);
-    const R fzx = mdiv<R, POW2>(Qzx - Qzx_m, k.dx2, k.inv_dx2);
-    const R fzy = mdiv<R, POW2>(Qzy - Qzy_m, k.dy2, k.inv_dy2);
+    const R fxy = dvy(Qxy - Qxy_m);
+    const R fxz = dvz(Qxz - Qxz_m);
+    const R fyx = dvx(Qyx - Qyx_m);
+    const R fyz = dvz(Qyz - Qyz_m);
+    const R fzx = dvx(Qzx - Qzx_m);
+    const R fzy = dvy(Qzy - Qzy_m);
 
     const R m1 = C[0] * C[0] + C[1] * C[1];
     const R m2 = C[2] * C[2] + C[3] * C[3];
@@ -151,15 +204,15 @@ __device__ __forceinline__ void moore_march(const MooreConsts<R> &k, const Level
       const R rzp = LC(cr, b + R0, ozp), izp = LC(ci, b + R0, ozp);
       const R rzm = LC(cr, b + R0, ozm), izm = LC(ci, b + R0, ozm);
       {
-        R lxx = mdiv<R, POW2>(cx * rxp + sx * ixp - 2 * ar + cxm[f] * Pm[cr] - sxm[f] * Pm[ci], k.dx2, k.inv_dx2);
-        R lyy = mdiv<R, POW2>(cy * ryp + sy * iyp - 2 * ar + cym * rym - sym_ * iym, k.dy2, k.inv_dy2);
-        R lzz = mdiv<R, POW2>(cz * rzp + sz * izp - 2 * ar + czm * rzm - szm * izm, k.dz2, k.inv_dz2);
+        R lxx = dvx(cx * rxp + sx * ixp - 2 * ar + cxm[f] * Pm[cr] - sxm[f] * Pm[ci]);
+        R lyy = dvy(cy * ryp + sy * iyp - 2 * ar + cym * rym - sym_ * iym);
+        R lzz = dvz(cz * rzp + sz * izp - 2 * ar + czm * rzm - szm * izm);
         acc[f][0] = lxx + lyy + lzz - pot[f] * ar;
       }
       {
-        R lxx = mdiv<R, POW2>(cx * ixp + (-sx) * rxp - 2 * ai + cxm[f] * Pm[ci] - (-sxm[f]) * Pm[cr], k.dx2, k.inv_dx2);
-        R lyy = mdiv<R, POW2>(cy * iyp + (-sy) * ryp - 2 * ai + cym * iym - (-sym_) * rym, k.dy2, k.inv_dy2);
-        R lzz = mdiv<R, POW2>(cz * izp + (-sz) * rzp - 2 * ai + czm * izm - (-szm) * rzm, k.dz2, k.inv_dz2);
+        R lxx = dvx(cx * ixp + (-sx) * rxp - 2 * ai + cxm[f] * Pm[ci] - (-sxm[f]) * Pm[cr]);
+        R lyy = dvy(cy * iyp + (-sy) * ryp - 2 * ai + cym * iym - (-sym_) * rym);
+        R lzz = dvz(cz * izp + (-sz) * rzp - 2 * ai + czm * izm - (-szm) * rzm);
         acc[f][1] = lxx + lyy + lzz - pot[f] * ai;
       }
       cur[f][0] = q[f] * (cx * (ai * rxp - ar * ixp) + sx * (ar * rxp + ai * ixp));
@@ -182,17 +235,23 @@ __device__ __forceinline__ void moore_march(const MooreConsts<R> &k, const Level
     Onew[4] = 2 * X - LP(4) + k.dt2 * atx;
     Onew[5] = 2 * Y - LP(5) + k.dt2 * aty;
     Onew[6] = 2 * Z - LP(6) + k.dt2 * atz;
-    if (act)
+    if (act) {
       for (int c = 0; c < 7; ++c) {
         if (BUF) bst<R, kStream>(FR[c], oz, offP + so, Onew[c]);
         else sto(O[c], so, oz, Onew[c]);
       }
+      if constexpr (CHECK)
+        for (int c = 0; c < 7; ++c) inband = inband && in_moore_band((double)Onew[c]);
+    }
     /* carry */
     for (int c = 0; c < 4; ++c) Pm[c] = C[c];
     for (int f = 0; f < 2; ++f) { cxm[f] = cxs[f]; sxm[f] = sxs[f]; }
     Qyx_m = Qyx;
     Qzx_m = Qzx;
     for (int c = 0; c < 7; ++c) C[c] = Nx[c];
+  }
+  if constexpr (CHECK) {
+    if (__builtin_amdgcn_ballot_w64(!inband) != 0 && (threadIdx.x & 63) == 0) atomicOr(bad, 1);
   }
 }
 

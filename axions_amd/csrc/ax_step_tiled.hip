@@ -45,18 +45,34 @@ __global__ void __launch_bounds__(64) ah_tiled_kernel(MarchConsts<R> k, Levels<R
   ah_march<R, POW2, PLAIN ? 1 : 0>(k, L, nx, ny, nz, y, zc, x0, x1, ez != 0);
 }
 
-template <typename R, bool POW2, bool BUF>
+/* FD: FP64 with divisors in the FixedDivisor range on a periodic lattice.
+ * band[3] are the per-level band flags (ax_march_moore.h), indexed by
+ * TimeStep mod 3: this step (TimeStep t) divides through FixedDivisor when
+ * use_band is set and band[t % 3] is clear, ORs its output's check into
+ * band[(t + 1) % 3] and clears band[(t + 2) % 3] for the next step. */
+template <typename R, bool POW2, bool BUF, bool FD>
 __global__ void __launch_bounds__(64, 3) moore_tiled_kernel(MooreConsts<R> k, Levels<R> L, int nx, int ny, int nz,
-                                                         TileGrid g, int ex, int ez) {
+                                                         TileGrid g, int ex, int ez, int *band, int t,
+                                                         int use_band) {
   const int w = (int)blockIdx.x;
+  if (FD && w == 0 && threadIdx.x == 0) band[(t + 2) % 3] = 0;
   if (w >= g.work) return;
   int zc, y, x0, x1;
   if (!tile_range(g, nx, ny, ex, w, zc, y, x0, x1)) return;
-  moore_march<R, POW2, BUF>(k, L, nx, ny, nz, y, zc, x0, x1, ez != 0);
+  if constexpr (FD) {
+    int *out = band + (t + 1) % 3;
+    if (use_band && band[t % 3] == 0)
+      moore_march<R, POW2, BUF, true, true>(k, L, nx, ny, nz, y, zc, x0, x1, ez != 0, out);
+    else
+      moore_march<R, POW2, BUF, false, true>(k, L, nx, ny, nz, y, zc, x0, x1, ez != 0, out);
+  } else {
+    moore_march<R, POW2, BUF>(k, L, nx, ny, nz, y, zc, x0, x1, ez != 0);
+  }
 }
 
 template <typename R>
-hipError_t launch_step_tiled_moore(const StepConsts &k, const Levels<R> &L, int xseg, hipStream_t st) {
+hipError_t launch_step_tiled_moore(const StepConsts &k, const Levels<R> &L, int xseg, hipStream_t st, int *band,
+                                   int t, bool use_band, bool *fd_used) {
   const TileGrid g = make_tile_grid(k.nx, k.ny, k.nz, xseg);
   MooreConsts<R> m;
   moore_consts_from<R>(k, m);
@@ -64,19 +80,29 @@ hipError_t launch_step_tiled_moore(const StepConsts &k, const Levels<R> &L, int 
   /* buffer path when both levels of a field lie within 4 GiB of its lower plane */
   const size_t lev = (size_t)std::llabs((const char *)L.cur[0] - (const char *)L.prv[0]);
   const bool buf = lev + (size_t)k.nx * k.ny * k.nz * sizeof(R) < ((size_t)1 << 32);
+  /* FP64 divisions by dx^2, dy^2, dz^2 through FixedDivisor, guarded by the
+   * band flags (every site written by this kernel: periodic lattices only) */
+  const bool fd = sizeof(R) == 8 && !pow2 && band != nullptr && !k.ex && !k.ez && fixed_divisor_range(k.dx2) &&
+                  fixed_divisor_range(k.dy2) && fixed_divisor_range(k.dz2) && getenv("AX_MOORE_PLAIN_DIV") == nullptr;
+  *fd_used = fd;
   dim3 grid(g.work), block(64);
-#define AX_MRL(P2, B) \
-  hipLaunchKernelGGL((moore_tiled_kernel<R, P2, B>), grid, block, 0, st, m, L, k.nx, k.ny, k.nz, g, k.ex, k.ez)
-  if (pow2 && buf) AX_MRL(true, true);
-  else if (pow2) AX_MRL(true, false);
-  else if (buf) AX_MRL(false, true);
-  else AX_MRL(false, false);
+#define AX_MRL(P2, B, FD)                                                                                     \
+  hipLaunchKernelGGL((moore_tiled_kernel<R, P2, B, FD>), grid, block, 0, st, m, L, k.nx, k.ny, k.nz, g, k.ex, k.ez, \
+                     band, t, use_band ? 1 : 0)
+  if (pow2 && buf) AX_MRL(true, true, false);
+  else if (pow2) AX_MRL(true, false, false);
+  else if (buf && fd) AX_MRL(false, true, true);
+  else if (buf) AX_MRL(false, true, false);
+  else if (fd) AX_MRL(false, false, true);
+  else AX_MRL(false, false, false);
 #undef AX_MRL
   return hipGetLastError();
 }
 
-template hipError_t launch_step_tiled_moore<double>(const StepConsts &, const Levels<double> &, int, hipStream_t);
-template hipError_t launch_step_tiled_moore<float>(const StepConsts &, const Levels<float> &, int, hipStream_t);
+template hipError_t launch_step_tiled_moore<double>(const StepConsts &, const Levels<double> &, int, hipStream_t,
+                                                    int *, int, bool, bool *);
+template hipError_t launch_step_tiled_moore<float>(const StepConsts &, const Levels<float> &, int, hipStream_t, int *,
+                                                   int, bool, bool *);
 
 bool use_tiled_ah(const ax_params &p) {
   const int tail = p.nz % 64;
