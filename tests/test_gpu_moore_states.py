@@ -1,8 +1,9 @@
 """Moore FP64 marching step with a non-power-of-two dx^2 on states with an
 exact vacuum block (Laplacian numerators +-0, plaquette sines 0) and a block
-of tiny fields (numerators near 1e-280, products underflowing): the marching
-kernel must equal the site kernel bit for bit, as it does on random
-states."""
+of tiny fields (numerators near 1e-280, products underflowing, outside the
+band that allows the march's short divisions): the marching kernel must
+equal the site kernel bit for bit, as it does on random states; and the
+march's two division paths must agree."""
 import os
 import sys
 
@@ -56,3 +57,27 @@ def test_moore_march_equals_site_kernel_on_special_states(case):
     out_m = run(p, st, 4, site=False)
     out_s = run(p, st, 4, site=True)
     assert np.array_equal(out_m.view(np.uint64), out_s.view(np.uint64))
+
+
+def test_moore_band_divisions_equal_plain_divisions():
+    """The FP64 march divides by dx^2 through the once-per-wave reciprocal
+    when the previous step's band flag allows it; with AX_MOORE_PLAIN_DIV it
+    always divides.  Both must give the same bits over a run that restarts
+    from the host mid-way (the flags restart with it)."""
+    p = ax.moore_params(16, ny=12, nz=64, dt=0.2)
+    say("[moore div] band vs plain")
+    outs = []
+    for plain in ("", "1"):
+        if plain:
+            os.environ["AX_MOORE_PLAIN_DIV"] = "1"
+        try:
+            with ax.Lattice(p) as lat:
+                lat.init_random_phase(9)
+                lat.step(17)
+                mid = lat.get_state()
+                lat.set_state(mid, timestep=17)
+                lat.step(14)
+                outs.append(lat.get_state())
+        finally:
+            os.environ.pop("AX_MOORE_PLAIN_DIV", None)
+    assert np.array_equal(outs[0].view(np.uint64), outs[1].view(np.uint64))
