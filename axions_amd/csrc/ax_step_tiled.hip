@@ -115,12 +115,12 @@ bool use_tiled_ah(const ax_params &p) {
 
 /* x planes per wave (measured at 512^3 AH, 256^3 Moore, on one box each):
  * Abelian-Higgs FP64 32 (3.875 ms; 8: 3.95, 16: 3.90, 24: 3.88), FP32 16
- * (1.816 ms; 8: 1.834, 12: 1.823, 32: 1.826); Moore 8 (FP64 1.012 ms, FP32
- * 0.422; 32: 1.02 / 0.444) */
+ * (1.816 ms; 8: 1.834, 12: 1.823, 32: 1.826); Moore FP64 16 (0.963 ms; 4:
+ * 0.993, 8: 0.979, 32: 0.974), FP32 8 (0.422 ms; 32: 0.444) */
 int default_xseg(const ax_params &p) {
   const char *xe = getenv("AX_TILED_XSEG");
   if (xe) return atoi(xe);
-  if (p.model == AX_MODEL_MOORE) return 8;
+  if (p.model == AX_MODEL_MOORE) return p.precision == 32 ? 8 : 16;
   return p.precision == 32 ? 16 : 32;
 }
 
