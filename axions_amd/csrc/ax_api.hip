@@ -317,6 +317,8 @@ int ax_set_state(ax_lattice *L, const double *host, size_t n) {
   if (!L || !host) return fail(AX_EINVAL, "NULL argument");
   const size_t want = (size_t)host_planes(L) * L->V;
   if (n != want) return fail(AX_EINVAL, "state length %zu != %zu", n, want);
+  /* no band flag describes the state from here on, even if an upload fails */
+  L->band_steps = 0;
   std::vector<float> tmp;
   for (int m = 0; m < host_planes(L); ++m) {
     int rc = upload_plane(L, device_plane(L, m), host + (size_t)m * L->V, tmp);
@@ -754,7 +756,8 @@ int launch_step(ax_lattice *L, int n) {
                : moore ? launch_step_tiled_moore<float>(k, lv, L->xseg, L->stream, L->band, n, use_band, &banded)
                        : launch_step_tiled_ah<float>(k, lv, L->xseg, L->stream);
   }
-  L->band_steps = banded ? L->band_steps + 1 : 0;
+  /* a failed launch wrote no flag */
+  L->band_steps = (banded && e == hipSuccess) ? L->band_steps + 1 : 0;
   return e == hipSuccess ? AX_OK : hip_fail(e, "step launch");
 }
 

@@ -33,11 +33,18 @@ struct MooreConsts {
  *
  * The numerators are sums of products of the current level's values and
  * their sines and cosines.  When every value of that level is 0 or has
- * magnitude in [2^-250, 2^250] (kMooreBand), every such product is 0 or a
- * multiple of 2^-605, so every numerator is 0 or in [2^-605, 2^254]: the
- * step that writes a level checks its outputs against that band and ORs a
- * per-level flag, and the next step divides this way only when the flag of
- * its current level is clear. */
+ * magnitude in [2^-250, 2^40] (in_moore_band) and the charges satisfy
+ * |q| <= 2^8 (checked by the launcher), every trig argument -- q*theta or a
+ * plaquette angle, a sum of four link values -- is below 2^50 in
+ * magnitude, where ax_reduce_pio2 still finds the quadrant exactly and
+ * leaves |r| <~ pi/4, so every sine and cosine lies in [-1, 1] (tested on
+ * the host up to 2^50).  Then every term of a numerator (a product of a
+ * value and a sine or cosine, or 2 times a value) is 0 or a multiple of
+ * 2^-605 and at most 2^41 in magnitude, and every numerator (at most five
+ * such terms) is 0 or in [2^-605, 2^44]: the step that
+ * writes a level checks its outputs against that band and ORs a per-level
+ * flag, and the next step divides this way only when the flag of its
+ * current level is clear. */
 struct FixedDivisor {
   double d, y;
 };
@@ -55,9 +62,11 @@ __device__ __forceinline__ double fixed_div(double a, const FixedDivisor &v) {
   return copysign(fma(fma(-v.d, q, a), v.y, q), a);
 }
 inline bool fixed_divisor_range(double d) { return d >= 0x1p-100 && d <= 0x1p100; }
+/* charges for which q*theta stays below 2^50 for every in-band theta */
+inline bool moore_band_charge(int q) { return q >= -256 && q <= 256; }
 __device__ __forceinline__ bool in_moore_band(double v) {
   const double m = fabs(v);
-  return m <= 0x1p250 && (m >= 0x1p-250 || m == 0);
+  return m <= 0x1p40 && (m >= 0x1p-250 || m == 0);
 }
 
 template <typename R>

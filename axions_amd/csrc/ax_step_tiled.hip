@@ -83,7 +83,8 @@ hipError_t launch_step_tiled_moore(const StepConsts &k, const Levels<R> &L, int 
   /* FP64 divisions by dx^2, dy^2, dz^2 through FixedDivisor, guarded by the
    * band flags (every site written by this kernel: periodic lattices only) */
   const bool fd = sizeof(R) == 8 && !pow2 && band != nullptr && !k.ex && !k.ez && fixed_divisor_range(k.dx2) &&
-                  fixed_divisor_range(k.dy2) && fixed_divisor_range(k.dz2) && getenv("AX_MOORE_PLAIN_DIV") == nullptr;
+                  fixed_divisor_range(k.dy2) && fixed_divisor_range(k.dz2) && moore_band_charge((int)k.q1) &&
+                  moore_band_charge((int)k.q2) && getenv("AX_MOORE_PLAIN_DIV") == nullptr;
   *fd_used = fd;
   dim3 grid(g.work), block(64);
 #define AX_MRL(P2, B, FD)                                                                                     \

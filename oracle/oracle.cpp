@@ -1372,6 +1372,8 @@ void orc_trig(int fn, const double *x, const double *y, double *out, size_t n) {
         out[i] = ax_sinf_is_small((float)x[i]) ? (double)cv : NAN;
         break;
       }
+      case 9: out[i] = ::sin(x[i]); break;    /* glibc, as the reference calls it */
+      case 10: out[i] = ::cos(x[i]); break;
       case 3: { float s, c; ax_sincosf((float)x[i], &s, &c); out[i] = s; break; }
       default: { float s, c; ax_sincosf((float)x[i], &s, &c); out[i] = c; break; }
     }

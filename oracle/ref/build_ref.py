@@ -56,6 +56,15 @@ CONFIGS = {
                                             stationary_ic="true", finalOut="true",
                                             stringDetect="true", detectBuffer="true",
                                             makeStringPosGif="true"), True),
+    # the 512^3 run's couplings and cadence (plain AH, dx=0.5 dt=0.1 lambda=2
+    # g=eta=1, energy and Gauss every step, crossings every 20 steps) at 64^3
+    # over 1000 TimeSteps
+    "evo_c2_1000_shared": ("evolution.cpp", dict(nx=64, ny=64, nz=64, nt=1000, dx=0.5, dy=0.5, dz=0.5,
+                                                 dt=0.1, **{"lambda": 2}, eta=1, g=1, damped_nt=0,
+                                                 dampFac=0.5, alpha=0, scaling=1, saveFreq=20,
+                                                 stationary_ic="true", finalOut="true",
+                                                 stringDetect="true", detectBuffer="true",
+                                                 makeStringPosGif="true"), True),
     # Moore model as shipped (fixed edges), reference couplings
     "moore_shared": ("mooreEvol.cpp", dict(nx=12, ny=12, nz=12, nt=30, makeGif="false"), True),
     "moore_libm": ("mooreEvol.cpp", dict(nx=12, ny=12, nz=12, nt=30, makeGif="false"), False),

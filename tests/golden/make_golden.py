@@ -48,6 +48,8 @@ SPECS = {
                                        theta_damp_form=1), 40, 5, 10, "shared"),
     "evo_c1_shared": (lambda: ah_params(64, dx=0.5, dt=0.1, lam=2.0, eta=1.0, g=1.0, alpha=0.0,
                                         damped_nt=0, dampFac=0.5, theta_damp_form=1), 200, 10, 0, "shared"),
+    "evo_c2_1000_shared": (lambda: ah_params(64, dx=0.5, dt=0.1, lam=2.0, eta=1.0, g=1.0, alpha=0.0,
+                                             damped_nt=0, dampFac=0.5, theta_damp_form=1), 1000, 20, 0, "shared"),
     "evo_bc_neumann": (lambda: ah_params(14, ny=12, nz=10, dx=0.5, dt=0.1, lam=2.0, g=1.0, alpha=2.0, damped_nt=10,
                                          dampFac=0.5, theta_damp_form=1, boundary=AX_BC_NEUMANN,
                                          boundary_z=AX_BC_PERIODIC), 40, 5, 10, "shared"),

@@ -2,8 +2,9 @@
 output covers (the per-plaquette winding has no reference counterpart) and
 cross-check the rest: vacuum, lattice translation, gauge invariance,
 straight-string winding, cube sum rule, linear dispersion, energy
-conservation.  CPU only (the oracle); the GPU versions are in
-test_gpu_properties.py."""
+conservation.  CPU only (the oracle).  On the GPU the windings are compared
+with the oracle's bit for bit (test_gpu_parity.py) and the cube sum rule is
+checked on the 512^3 lattice (test_gpu_fullsize.py)."""
 import math
 
 import numpy as np

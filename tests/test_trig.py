@@ -32,6 +32,22 @@ def test_sin_cos_accuracy(lo, hi):
     assert es.max() < 0.9 and ec.max() < 0.9, (es.max(), ec.max())
 
 
+def test_sin_cos_bounded_below_2_pow_50():
+    """The premise of the Moore march's division band: for every argument
+    below 2^50 in magnitude (q*theta with in-band theta <= 2^40 and |q| <=
+    2^8, or a plaquette angle) the reduction leaves |r| <~ pi/4 and sin/cos
+    stay within [-1, 1]; far beyond it they do not (which is why the band
+    stops at 2^40)."""
+    rng = np.random.default_rng(21)
+    x = np.concatenate([rng.uniform(-1, 1, 20000) * 2.0 ** rng.integers(0, 51, 20000),
+                        np.array([2.0 ** 50, -2.0 ** 50, 2.0 ** 50 - 1, 2.0 ** 48 * 3])])
+    s, c = orc.trig(0, x), orc.trig(1, x)
+    assert np.max(np.abs(s)) <= 1.0 and np.max(np.abs(c)) <= 1.0
+    assert np.max(np.abs(s * s + c * c - 1)) < 1e-13
+    far = orc.trig(0, np.array([2.0 ** 60, 2.0 ** 200]))
+    assert not np.all(np.abs(far) <= 1.0)
+
+
 def test_sin_cos_special_values():
     x = np.array([0.0, -0.0, 1e-300, -1e-300, 5e-324, np.inf, -np.inf, np.nan])
     s, c = orc.trig(0, x), orc.trig(1, x)
