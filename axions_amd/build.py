@@ -14,7 +14,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 REPO = os.path.dirname(HERE)
 SRC = ["ax_api.hip", "ax_step.hip", "ax_step_tiled.hip", "ax_diag.hip", "ax_pool.hip", "ax_global.hip", "ax_length.hip",
-       "ax_profile.hip", "ax_bound.hip"]
+       "ax_profile.hip", "ax_bound.hip", "ax_noalloc.hip"]
 LIB = os.path.join(HERE, "libaxions.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("AX_OFFLOAD_ARCH", "gfx950")

@@ -143,6 +143,8 @@ const ax_params &lattice_params(const ax_lattice *L) { return L->p; }
 void *lattice_state(ax_lattice *L) { return L->state; }
 size_t lattice_sites(const ax_lattice *L) { return L->V; }
 size_t lattice_pitch(const ax_lattice *L) { return L->pitch; }
+int lattice_device(const ax_lattice *L) { return L->device; }
+int lattice_timestep(const ax_lattice *L) { return L->timestep; }
 void lattice_reset_timestep(ax_lattice *L) {
   L->timestep = 0;
   L->band_steps = 0;

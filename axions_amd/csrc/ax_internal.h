@@ -90,6 +90,8 @@ const ax_params &lattice_params(const ax_lattice *L);
 void *lattice_state(ax_lattice *L);
 size_t lattice_sites(const ax_lattice *L);
 size_t lattice_pitch(const ax_lattice *L);
+int lattice_device(const ax_lattice *L);
+int lattice_timestep(const ax_lattice *L);
 void lattice_reset_timestep(ax_lattice *L);
 
 int launch_step(ax_lattice *L, int n);

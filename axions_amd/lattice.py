@@ -49,6 +49,7 @@ def _load():
         "ax_winding": (ctypes.c_int, [vp, vp, sz]),
         "ax_winding_field": (ctypes.c_int, [vp, i32, vp, sz]),
         "ax_crossings": (ctypes.c_int, [vp, vp, sz, ctypes.POINTER(sz)]),
+        "ax_crossings_noalloc": (ctypes.c_int, [vp, vp, sz, vp]),
         "ax_random_phases": (ctypes.c_int, [ctypes.c_uint32, sz, vp]),
         "ax_kernel_timer": (ctypes.c_int, [vp, i32]),
         "ax_set_tile": (ctypes.c_int, [vp, i32]),

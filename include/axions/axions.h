@@ -199,6 +199,18 @@ int ax_winding_field(ax_lattice *lat, int32_t field, int8_t *out, size_t n);
  * total count to *count. */
 int ax_crossings(ax_lattice *lat, double *xyz, size_t cap, size_t *count);
 
+/* ax_crossings without any device allocation, for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops).  No
+ * counterpart in the reference.  The same points in the same order as
+ * ax_crossings, written into xyz, a DEVICE buffer of cap points (3*cap
+ * doubles) owned by the caller; the total count goes to *count, a DEVICE
+ * int64.  Enqueued on the handle's stream only (ax_sync before reading).
+ * The lattice's device must be current.  Its per-block scratch is one fixed
+ * array per GPU inside the library (6 MiB, held until the process ends), so
+ * lattices above 2^27 sites (512^3) are refused (AX_EINVAL), and calls that
+ * overlap on one GPU race on that array. */
+int ax_crossings_noalloc(ax_lattice *lat, double *xyz, size_t cap, int64_t *count);
+
 /* String length from a list of crossing points (x,y,z triples, e.g. from
  * ax_crossings) by the greedy nearest-neighbour chaining of
  * ic_w_evolution.cpp:961-1087: out[0] = total length, out[1] / out[2] = the
