@@ -64,8 +64,15 @@ def consts_ah(p, fx, extra):
 BC_NAME = {0: "periodic", 1: "fixed", 2: "neumann", 3: "absorbing"}
 
 
-@pytest.mark.parametrize("name", ["evo_ah_g1_shared", "evo_bc_absorbing_zfixed", "evo_bc_neumann_z"])
-def test_evolution_program(tmp_path, name):
+@pytest.mark.parametrize("name,split", [("evo_ah_g1_shared", False), ("evo_ah_g1_shared", True),
+                                        ("evo_bc_absorbing_zfixed", False), ("evo_bc_neumann_z", True)])
+def test_evolution_program(tmp_path, name, split):
+    """Energy and Gauss columns against the reference's valsPerLoop, crossing
+    frames and final field bit for bit; the string-length column(s) -- the
+    chained length of ic_w_evolution.cpp:961-1087 (total, or inner and
+    outer with splitLength) of the GPU crossings of every TimeStep after the
+    damping window -- against the oracle's all-pairs chaining of the
+    oracle's crossings on the oracle's trajectory, bit for bit."""
     fx, p = load(name)
     d = str(tmp_path)
     os.makedirs(os.path.join(d, "Data"), exist_ok=True)
@@ -73,12 +80,23 @@ def test_evolution_program(tmp_path, name):
     write_ic(os.path.join(d, "Data", "ic.txt"), st, p, 5)
     run(["--program", "evolution", "--tag", "golden",
          *consts_ah(p, fx, ["saveFreq=5", "stationary_ic=true", "finalOut=true", f"xyBC={BC_NAME[p.boundary]}",
-                            f"zBC={BC_NAME[p.boundary_z]}"])], d)
+                            f"zBC={BC_NAME[p.boundary_z]}", f"splitLength={'true' if split else 'false'}"])], d)
     vals = np.loadtxt(os.path.join(d, "Data", "valsPerLoop_golden.txt"))
-    assert vals.shape == (fx["nt"], 3)
+    assert vals.shape == (fx["nt"], 4 if split else 3)
+    pxy, pz = p.boundary == 0, p.boundary_z == 0
+    nonzero = 0
     for n in range(fx["nt"]):
         assert rel(vals[n, 0], fx["energy"][n]) < 1e-12, n
         assert rel(vals[n, 1], fx["gauss"][n]) < 1e-12, n
+        want = (0.0, 0.0, 0.0)
+        if n >= p.damped_nt:                       # detectBuffer: no detection while damped
+            pts, _ = orc.crossings(p, st, n)
+            want = orc.string_length(pts, p.nx, p.ny, p.nz, p.dx, p.dy, p.dz, pxy, pz)
+        got = tuple(vals[n, 2:])
+        assert got == ((want[1], want[2]) if split else (want[0],)), (n, got, want)
+        nonzero += want[0] > 0
+        orc.step(p, st, n, 1, 0, 16)
+    assert nonzero > 0
     for k, fr in enumerate(fx["frames"]):
         path = os.path.join(d, "GifData", f"gifStringPosData_{k}.txt")
         pts = np.loadtxt(path, ndmin=2) if os.path.getsize(path) else np.zeros((0, 3))
