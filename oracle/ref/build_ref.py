@@ -65,6 +65,15 @@ CONFIGS = {
                                                  stationary_ic="true", finalOut="true",
                                                  stringDetect="true", detectBuffer="true",
                                                  makeStringPosGif="true"), True),
+    # the CPU baseline of the benchmark: the program as shipped (glibc trig,
+    # OpenMP) on a 128^3 periodic lattice with the benchmark's couplings,
+    # no string output; two step counts, so that the difference of their run
+    # times is the time of 20 TimeSteps without the ic.txt read
+    **{name: ("evolution.cpp", dict(nx=128, ny=128, nz=128, nt=nt, dx=0.5, dy=0.5, dz=0.5, dt=0.1,
+                                    **{"lambda": 2}, eta=1, g=1, damped_nt=0, dampFac=0.5, alpha=0, scaling=1,
+                                    stationary_ic="true", finalOut="false", stringDetect="false",
+                                    makeStringPosGif="false", makeGif="false"), False)
+       for name, nt in (("cpu_bench_nt2", 2), ("cpu_bench_nt22", 22))},
     # Moore model as shipped (fixed edges), reference couplings
     "moore_shared": ("mooreEvol.cpp", dict(nx=12, ny=12, nz=12, nt=30, makeGif="false"), True),
     "moore_libm": ("mooreEvol.cpp", dict(nx=12, ny=12, nz=12, nt=30, makeGif="false"), False),

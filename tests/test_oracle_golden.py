@@ -24,7 +24,9 @@ FIXTURES = sorted(os.path.basename(p)[:-5] for p in glob.glob(os.path.join(GOLDE
                   if not os.path.basename(p).startswith(("global_", "icw_", "sor_", "straight_")))
 # global_*: leapfrog.cpp (test_global_oracle.py); icw_*: string length (test_string_length.py);
 # sor_*, straight_*: straight-string pipeline (test_profile_oracle.py)
-FAST = [f for f in FIXTURES if not f.startswith("evo_c1")]
+# evo_c1_* (64^3 x 200) and evo_c2_* (64^3 x 1000) take minutes in the oracle:
+# the GPU runs them against the fixture directly (test_gpu_parity.py)
+FAST = [f for f in FIXTURES if not f.startswith(("evo_c1", "evo_c2"))]
 
 
 def sha(a):
