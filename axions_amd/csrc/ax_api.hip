@@ -642,7 +642,6 @@ int prepare_faces(ax_lattice *L) {
 
 bool use_diag_march(const ax_params &p) {
   return p.model == AX_MODEL_AH && p.boundary == AX_BC_PERIODIC && p.boundary_z == AX_BC_PERIODIC &&
-         (double)p.nx * p.ny * p.nz * (p.precision == 64 ? 8.0 : 4.0) < 4294967296.0 &&
          getenv("AX_DIAG_SITE_KERNEL") == nullptr;
 }
 

@@ -35,14 +35,14 @@ __device__ __forceinline__ bool tile_range(const TileGrid &g, int nx, int ny, in
   return x0 < x1;
 }
 
-template <typename R, bool POW2, bool PLAIN>
+template <typename R, bool POW2, bool PLAIN, bool BIG, bool PF = false>
 __global__ void __launch_bounds__(64) ah_tiled_kernel(MarchConsts<R> k, Levels<R> L, int nx, int ny, int nz,
                                                       TileGrid g, int ex, int ez) {
   const int w = (int)blockIdx.x;
   if (w >= g.work) return;
   int zc, y, x0, x1;
   if (!tile_range(g, nx, ny, ex, w, zc, y, x0, x1)) return;
-  ah_march<R, POW2, PLAIN ? 1 : 0>(k, L, nx, ny, nz, y, zc, x0, x1, ez != 0);
+  ah_march<R, POW2, PLAIN ? 1 : 0, BIG, PF>(k, L, nx, ny, nz, y, zc, x0, x1, ez != 0);
 }
 
 /* FD: FP64 with divisors in the FixedDivisor range on a periodic lattice.
@@ -105,12 +105,14 @@ template hipError_t launch_step_tiled_moore<double>(const StepConsts &, const Le
 template hipError_t launch_step_tiled_moore<float>(const StepConsts &, const Levels<float> &, int, hipStream_t, int *,
                                                    int, bool, bool *);
 
+/* Every Abelian-Higgs and Moore lattice marches, whatever its z tail (a
+ * last chunk of 1..63 sites runs with its other lanes idle) and, for
+ * Abelian-Higgs, whatever its plane size (planes of 4 GiB or more take the
+ * 64-bit plane addressing of ah_march).  The Moore march keeps 32-bit plane
+ * offsets: its planes must stay below 4 GiB. */
 bool use_tiled_ah(const ax_params &p) {
-  const int tail = p.nz % 64;
-  return (p.model == AX_MODEL_AH || p.model == AX_MODEL_MOORE) &&
-         (tail == 0 || tail >= 32) &&
-         p.nx >= 3 && p.ny >= 3 &&
-         (double)p.nx * p.ny * p.nz * (p.precision == 64 ? 8.0 : 4.0) < 4294967296.0 &&
+  const bool big = (double)p.nx * p.ny * p.nz * (p.precision == 64 ? 8.0 : 4.0) >= 4294967296.0;
+  return (p.model == AX_MODEL_AH || (p.model == AX_MODEL_MOORE && !big)) && p.nx >= 3 && p.ny >= 3 &&
          getenv("AX_FORCE_SITE_KERNEL") == nullptr;
 }
 
@@ -131,13 +133,34 @@ hipError_t launch_step_tiled_ah(const StepConsts &k, const Levels<R> &L, int xse
   MarchConsts<R> m;
   march_consts_from<R>(k, rate_limit<R>((R)k.dt), m);
   const bool pow2 = is_pow2(k.dx2) && is_pow2(k.dy2) && is_pow2(k.dz2);
+  /* a field plane of 4 GiB or more: 64-bit plane addressing (AX_BIG_PLANES
+   * forces it on smaller lattices, for tests) */
+  const bool big = (size_t)k.nx * k.ny * k.nz * sizeof(R) >= ((size_t)1 << 32) || getenv("AX_BIG_PLANES") != nullptr;
   dim3 grid(g.work), block(64);
-#define AX_AHL(P2, PL) \
-  hipLaunchKernelGGL((ah_tiled_kernel<R, P2, PL>), grid, block, 0, st, m, L, k.nx, k.ny, k.nz, g, k.ex, k.ez)
-  if (pow2 && m.plain) AX_AHL(true, true);
-  else if (pow2) AX_AHL(true, false);
-  else if (m.plain) AX_AHL(false, true);
-  else AX_AHL(false, false);
+#define AX_AHL(P2, PL, BG) \
+  hipLaunchKernelGGL((ah_tiled_kernel<R, P2, PL, BG>), grid, block, 0, st, m, L, k.nx, k.ny, k.nz, g, k.ex, k.ez)
+  /* FP64 plain runs with power-of-two spacings (the variant that keeps 3
+   * waves per SIMD with it: 166 VGPRs; the others reach 169-179 and drop to
+   * 2), whole chunks and three planes within 32-bit offsets: the previous
+   * level and the next plane are prefetched into LDS (ah_march) */
+  const bool pf = sizeof(R) == 8 && !big && pow2 && m.plain && k.nz % 64 == 0 &&
+                  (size_t)k.nx * k.ny * k.nz * sizeof(R) * 3 + ((size_t)1 << 20) < ((size_t)1 << 32) &&
+                  getenv("AX_NO_PREFETCH") == nullptr;
+  if (pf) {
+    if constexpr (sizeof(R) == 8)
+      hipLaunchKernelGGL((ah_tiled_kernel<double, true, true, false, true>), grid, block, 0, st, m, L, k.nx, k.ny,
+                         k.nz, g, k.ex, k.ez);
+  } else if (big) {
+    if (pow2 && m.plain) AX_AHL(true, true, true);
+    else if (pow2) AX_AHL(true, false, true);
+    else if (m.plain) AX_AHL(false, true, true);
+    else AX_AHL(false, false, true);
+  } else {
+    if (pow2 && m.plain) AX_AHL(true, true, false);
+    else if (pow2) AX_AHL(true, false, false);
+    else if (m.plain) AX_AHL(false, true, false);
+    else AX_AHL(false, false, false);
+  }
 #undef AX_AHL
   return hipGetLastError();
 }

@@ -36,10 +36,11 @@ F32 = {
     "f32_tiled_aniso_form1": (lambda: ax.ah_params(9, ny=6, nz=128, dx=0.7, dy=0.6, dz=0.55, dt=0.13, lam=1.5,
                                                    eta=0.9, g=0.8, alpha=2.0, damped_nt=5, dampFac=0.3,
                                                    theta_damp_form=1, precision=32), 12, True),
-    # site kernel: short rows and fixed edges
+    # short rows (one partial chunk) and fixed edges; the last field says
+    # the case also runs on the site kernel (AX_FORCE_SITE_KERNEL)
     "f32_site_odd": (lambda: ax.ah_params(11, ny=13, nz=17, precision=32), 16, False),
     "f32_site_fixed": (lambda: ax.ah_params(10, boundary=AX_BC_FIXED, precision=32), 16, False),
-    # non-periodic boundaries (site kernel + boundary kernels)
+    # non-periodic boundaries (boundary kernels + bulk update)
     "f32_neumann_zneu": (lambda: ax.ah_params(11, ny=9, nz=8, dx=0.6, dz=0.55, g=0.8, alpha=2.0, damped_nt=5,
                                               dampFac=0.4, boundary=AX_BC_NEUMANN, boundary_z=AX_BC_NEUMANN,
                                               precision=32), 16, False),
@@ -58,14 +59,21 @@ F32 = {
 }
 
 
-@pytest.mark.parametrize("name", list(F32))
-def test_fp32_step_bitexact_vs_oracle(name):
-    build, nsteps, tiled = F32[name]
+CASES32 = [(n, False) for n in F32] + [(n, True) for n, v in F32.items() if not v[2]]
+
+
+@pytest.mark.parametrize("name,site", CASES32)
+def test_fp32_step_bitexact_vs_oracle(name, site, monkeypatch):
+    """Every lattice marches (whatever its z tail or boundary); the short and
+    non-periodic cases also run on the one-thread-per-site kernel."""
+    build, nsteps, _ = F32[name]
     p = build()
-    say(f"[fp32] {name} {p.nx}x{p.ny}x{p.nz} steps={nsteps}")
+    say(f"[fp32] {name} {p.nx}x{p.ny}x{p.nz} steps={nsteps} {'site' if site else 'march'}")
+    if site:
+        monkeypatch.setenv("AX_FORCE_SITE_KERNEL", "1")
     st = orc.init_random_phase(p, 42).astype(np.float32)
     with ax.Lattice(p) as lat:
-        assert lat.tile()[1] == tiled
+        assert lat.tile()[1] == (not site)
         lat.init_random_phase(42)
         assert np.array_equal(lat.get_state(), st.astype(np.float64)), "initial condition differs"
         done = 0

@@ -160,3 +160,31 @@ def test_pool_128_192_256_alone_equals_mixed():
             assert np.array_equal(px.view(np.uint64), xi.view(np.uint64))
             assert bits_equal(pool.get_state(rid), st)
             pool.release(rid)
+
+
+def test_big_planes_march_equals_site_kernel():
+    """A lattice whose field planes exceed 4 GiB (520 x 1024 x 1024 FP64,
+    4.36 GB per plane, 43.6 GB of state): the march with 64-bit plane
+    addressing against the site kernel, and the marching diagnostics against
+    the site diagnostics; and the reference's production lattice (1001^3
+    FP64, 80 GB) takes the marching kernel."""
+    p = ax.ah_params(520, ny=1024, nz=1024)
+    say("[fullsize] AH 520x1024x1024 fp64 (4.36 GB planes): 4 steps, then 1 march vs 1 site")
+    a, b = march_vs_site(p, 4, 1)
+    assert np.all(np.isfinite(a))
+    assert bits_equal(a, b)
+    del a, b
+    with ax.Lattice(p) as lat:
+        lat.init_random_phase(42)
+        lat.step(3)
+        d_m = lat.diagnostics(winding=False)
+        with env("AX_DIAG_SITE_KERNEL"):
+            d_s = lat.diagnostics(winding=False)
+    assert d_m["finite"] == d_s["finite"] == 1
+    assert rel(d_m["energy"], d_s["energy"]) < 1e-13
+    assert rel(d_m["gauss_mean"], d_s["gauss_mean"]) < 1e-13
+    say("[fullsize] AH 1001^3 fp64 (80 GB): marching kernel")
+    with ax.Lattice(ax.ah_params(1001)) as lat:
+        assert lat.tile()[1]
+        lat.step(1)
+        lat.sync()

@@ -78,13 +78,21 @@ CASES = {
 }
 
 
-@pytest.mark.parametrize("name", list(CASES))
-def test_step_bitexact_vs_oracle(name):
+# the one-thread-per-site kernel too (every lattice marches by default)
+SITE_CASES = ["ah_plain_16", "ah_g0_16", "ah_damp_form0_16", "ah_damp_form1_16", "ah_aniso", "ah_odd_sizes",
+              "moore_periodic_12", "moore_fixed_12", "march_neumann_zneu", "march_absorbing_zfixed"]
+
+
+@pytest.mark.parametrize("name,site", [(n, False) for n in CASES] + [(n, True) for n in SITE_CASES])
+def test_step_bitexact_vs_oracle(name, site, monkeypatch):
     build, nsteps = CASES[name]
     p = build()
-    say(f"[parity] {name} {p.nx}x{p.ny}x{p.nz} steps={nsteps}")
+    say(f"[parity] {name} {p.nx}x{p.ny}x{p.nz} steps={nsteps} {'site' if site else 'march'}")
+    if site:
+        monkeypatch.setenv("AX_FORCE_SITE_KERNEL", "1")
     st = orc.init_random_phase(p, 42, 0)
     with ax.Lattice(p) as lat:
+        assert lat.tile()[1] == (not site)
         lat.init_random_phase(42, 0)
         assert bits_equal(lat.get_state(), st), "initial condition differs"
         done = 0
@@ -190,6 +198,7 @@ def test_nonfinite_is_flagged():
 
 
 GOLDEN_SHARED = [n for n in ("evo_ah_g0", "evo_ah_g1_shared", "evo_ah_aniso", "moore_shared", "evo_c1_shared",
+                              "evo_c2_1000_shared",
                               "evo_bc_neumann", "evo_bc_neumann_z", "evo_bc_absorbing", "evo_bc_absorbing_zfixed",
                               "evo_bc_fixed_zper", "evo_bc_fixed_zneu", "moore_neumann")
                  if os.path.exists(os.path.join(GOLDEN, n + ".json"))]
@@ -199,7 +208,9 @@ GOLDEN_SHARED = [n for n in ("evo_ah_g0", "evo_ah_g1_shared", "evo_ah_aniso", "m
 def test_gpu_matches_reference_fixture(name):
     """The GPU run reproduces the reference program's own outputs: its final
     field bit for bit, its energy and Gauss series to 1e-12, its crossing
-    frames bit for bit."""
+    frames bit for bit.  evo_c2_1000_shared is the 512^3 run's couplings and
+    cadence (plain, dx = 0.5, dt = 0.1, lambda = 2, g = eta = 1; energy and
+    Gauss every step, crossing frames every 20) at 64^3 over 1000 TimeSteps."""
     fx = json.load(open(os.path.join(GOLDEN, name + ".json")))
     p = AxParams()
     for k, v in fx["params"].items():
@@ -222,6 +233,126 @@ def test_gpu_matches_reference_fixture(name):
             lat.step(1)
         cur, _ = lat.levels()
         assert sha(cur) == fx["final_sha256"]
+
+
+TAILS = {
+    # z tails of 1..31 sites: the last chunk marches with its other lanes idle
+    "ah_tail1_fp64": (lambda: ax.ah_params(10, ny=9, nz=65), 12),
+    "ah_tail2_fp32": (lambda: ax.ah_params(9, ny=10, nz=66, precision=32), 12),
+    "ah_tail17_damped": (lambda: ax.ah_params(11, ny=8, nz=81, alpha=2.0, damped_nt=4, dx=0.6, dt=0.11), 12),
+    "ah_tail31_fp32_aniso": (lambda: ax.ah_params(8, ny=12, nz=95, dx=0.7, dy=0.6, dz=0.55, dt=0.13,
+                                                  precision=32), 12),
+    "ah_nz17": (lambda: ax.ah_params(12, ny=10, nz=17), 12),
+    "moore_tail5": (lambda: ax.moore_params(9, ny=8, nz=69, dt=0.2), 10),
+    "moore_tail20_fp32": (lambda: ax.moore_params(8, ny=9, nz=20, dt=0.2, precision=32), 10),
+}
+
+
+@pytest.mark.parametrize("name", list(TAILS))
+def test_short_z_tails_march_bitexact_vs_oracle(name, monkeypatch):
+    """The march against the oracle (FP32 Moore, which the oracle does not
+    restate, against the site kernel that defines it)."""
+    build, nsteps = TAILS[name]
+    p = build()
+    say(f"[tails] {name} {p.nx}x{p.ny}x{p.nz} fp{p.precision}")
+    fp32 = p.precision == 32
+    st = orc.init_random_phase(p, 42, 0)
+    if fp32:
+        st = st.astype(np.float32)
+    with ax.Lattice(p) as lat:
+        assert lat.tile()[1], "every z tail marches"
+        lat.init_random_phase(42, 0)
+        lat.step(nsteps)
+        if fp32 and p.model == AX_MODEL_MOORE:
+            g = lat.get_state()
+            monkeypatch.setenv("AX_FORCE_SITE_KERNEL", "1")
+            with ax.Lattice(p) as site:
+                assert not site.tile()[1]
+                site.init_random_phase(42, 0)
+                site.step(nsteps)
+                assert bits_equal(g, site.get_state())
+        elif fp32:
+            orc.step_r(p, st, 0, nsteps, NT)
+            g = lat.get_state().astype(np.float32)
+            assert np.array_equal(g.view(np.uint32), st.view(np.uint32))
+        else:
+            orc.step(p, st, 0, nsteps, 0, NT)
+            assert bits_equal(lat.get_state(), st)
+
+
+@pytest.mark.parametrize("name", ["ah_plain_64", "tiled_aniso_damped", "ah_odd_sizes", "tiled_rect_damped"])
+def test_big_plane_addressing_bitexact_vs_oracle(name):
+    """The 64-bit plane addressing of lattices whose field planes reach 4 GiB,
+    forced on small lattices (AX_BIG_PLANES): the same bits as the oracle."""
+    import os
+    build, nsteps = CASES[name]
+    p = build()
+    say(f"[big planes] {name} {p.nx}x{p.ny}x{p.nz}")
+    st = orc.init_random_phase(p, 42, 0)
+    os.environ["AX_BIG_PLANES"] = "1"
+    try:
+        with ax.Lattice(p) as lat:
+            lat.init_random_phase(42, 0)
+            lat.step(nsteps)
+            g = lat.get_state()
+            d = lat.diagnostics(winding=True)
+    finally:
+        os.environ.pop("AX_BIG_PLANES", None)
+    orc.step(p, st, 0, nsteps, 0, NT)
+    assert bits_equal(g, st)
+    o = orc.diag(p, st, nsteps, 0)
+    assert rel(d["energy"], o["energy"]) < 1e-12
+
+
+LONG = {
+    # periodic Moore on the march (the band-flag division path after the
+    # first step), 800 TimeSteps as the 256^3 run
+    "moore_march_800": (lambda: ax.moore_params(24, ny=20, nz=64, dt=0.2), 800, (1, 100, 400, 800)),
+    # Abelian-Higgs with non-power-of-two spacings (true divisions), 1000 TimeSteps
+    "ah_aniso_march_1000": (lambda: ax.ah_params(24, ny=20, nz=64, dx=0.7, dy=0.6, dz=0.55, dt=0.13, lam=1.5,
+                                                 eta=0.9, g=0.8), 1000, (1, 100, 500, 1000)),
+}
+
+
+@pytest.mark.parametrize("name", list(LONG))
+def test_long_run_bitexact_vs_oracle(name):
+    """Long trajectories, so that late-time branches (the small-angle sine
+    vote once plaquette angles grow, the plain-mode overflow guard, the Moore
+    band flags) are exercised: the GPU march equals the oracle bit for bit at
+    every checkpoint."""
+    build, nsteps, checks = LONG[name]
+    p = build()
+    say(f"[long] {name} {p.nx}x{p.ny}x{p.nz} {nsteps} steps")
+    st = orc.init_random_phase(p, 42, 0)
+    with ax.Lattice(p) as lat:
+        assert lat.tile()[1]
+        lat.init_random_phase(42, 0)
+        done = 0
+        for c in checks:
+            lat.step(c - done)
+            orc.step(p, st, done, c - done, 0, NT)
+            done = c
+            g = lat.get_state()
+            assert np.all(np.isfinite(g))
+            if not bits_equal(g, st):
+                d = np.nonzero(g != st)[0]
+                pytest.fail(f"{name}: {d.size} values differ after {done} steps, first flat index {d[0]}")
+
+
+@pytest.mark.parametrize("precision", [32])
+def test_fp32_long_run_bitexact_vs_oracle(precision):
+    """The FP32 march over 1000 TimeSteps against the oracle's binary32
+    instantiation of the same step."""
+    p = ax.ah_params(24, ny=20, nz=64, precision=precision)
+    say("[long] fp32 24x20x64 1000 steps")
+    st = orc.init_random_phase(p, 42, 0).astype(np.float32)
+    with ax.Lattice(p) as lat:
+        lat.init_random_phase(42, 0)
+        lat.step(1000)
+        orc.step_r(p, st, 0, 1000, NT)
+        g = lat.get_state().astype(np.float32)
+        assert np.all(np.isfinite(g))
+        assert np.array_equal(g.view(np.uint32), st.view(np.uint32))
 
 
 @pytest.mark.parametrize("shape,precision,dx,g,energy_form", [
@@ -290,3 +421,25 @@ def test_work_orders_give_the_same_state(shape, precision):
         assert np.array_equal(w, w0), order
         assert d["wound_plaquettes"] == d0["wound_plaquettes"]
         assert rel(d["energy"], d0["energy"]) < 1e-13 and rel(d["gauss_mean"], d0["gauss_mean"]) < 1e-13
+
+
+@pytest.mark.parametrize("shape,xseg", [((16, 12, 128), 1), ((16, 12, 128), 3), ((40, 8, 64), 32),
+                                        ((9, 10, 192), 0)])
+def test_prefetch_march_equals_plain_march(shape, xseg, monkeypatch):
+    """The FP64 march that prefetches each plane's previous level and next
+    plane into LDS (plain runs, power-of-two spacings, whole chunks) gives the
+    bits of the march without it, for segments of 1, 3, 32 planes and whole
+    rows."""
+    nx, ny, nz = shape
+    p = ax.ah_params(nx, ny=ny, nz=nz)
+    say(f"[prefetch] {shape} xseg={xseg}")
+    outs = []
+    for off in (False, True):
+        if off:
+            monkeypatch.setenv("AX_NO_PREFETCH", "1")
+        with ax.Lattice(p) as lat:
+            lat.set_tile(xseg)
+            lat.init_random_phase(8)
+            lat.step(11)
+            outs.append(lat.get_state())
+    assert bits_equal(outs[0], outs[1])
