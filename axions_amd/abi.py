@@ -190,6 +190,7 @@ class AxPoolStats(ctypes.Structure):
         ("site_updates", ctypes.c_double),
         ("elapsed", ctypes.c_double),
         ("budget", ctypes.c_size_t), ("in_use", ctypes.c_size_t), ("peak_in_use", ctypes.c_size_t),
+        ("step_ms", ctypes.c_double),
     ]
 
 

@@ -247,6 +247,7 @@ struct ChunkRec {
   int64_t seq = 0;
   int buf = 0;
   hipEvent_t done = nullptr;
+  hipEvent_t s0 = nullptr, s1 = nullptr;   /* around the chunk's step launches */
   struct Item {
     int64_t id;
     int t0, steps, pre;
@@ -278,8 +279,12 @@ struct ax_pool {
   char *h_tab[kBufs] = {};          /* pinned: header + entries */
   unsigned long long *h_cnt[kBufs] = {};
   int grid_step = 0, grid_wind = 0, grid_phase = 0;
-  hipGraphExec_t gexec = nullptr;
-  hipGraph_t graph = nullptr;
+  /* a chunk is three parts: zero counters + windings of new lattices, the
+   * K step launches, windings after the steps; one captured graph each, so
+   * that events between them time the step launches alone */
+  static constexpr int kParts = 3;
+  hipGraphExec_t gexec[kParts] = {};
+  hipGraph_t graph[kParts] = {};
 
   std::mutex mu;
   std::condition_variable cv_sched, cv_ic, cv_done;
@@ -302,7 +307,7 @@ struct ax_pool {
   std::thread sched;
   std::vector<std::thread> icw;
 
-  int launch_chunk_kernels(hipStream_t s);
+  int launch_chunk_kernels(hipStream_t s, int part);
   void launch_winding(hipStream_t s, int which);
   int capture_graph();
   void scheduler();
@@ -344,11 +349,13 @@ void ax_pool::launch_winding(hipStream_t s, int which) {
   }
 }
 
-int ax_pool::launch_chunk_kernels(hipStream_t s) {
+int ax_pool::launch_chunk_kernels(hipStream_t s, int part) {
   const int K = cfg.chunk_steps;
-  hipLaunchKernelGGL(pool_zero_kernel, dim3(1), dim3(256), 0, s, d_hdr, d_cnt);
-  launch_winding(s, 0);
-  for (int j = 0; j < K; ++j) {
+  if (part == 0) {
+    hipLaunchKernelGGL(pool_zero_kernel, dim3(1), dim3(256), 0, s, d_hdr, d_cnt);
+    launch_winding(s, 0);
+  }
+  for (int j = 0; part == 1 && j < K; ++j) {
     if (esize == 8) {
       if (pow2) hipLaunchKernelGGL((pool_step_kernel<double, true>), dim3(grid_step), dim3(64), 0, s, p, lim64, d_hdr, d_ent, j);
       else hipLaunchKernelGGL((pool_step_kernel<double, false>), dim3(grid_step), dim3(64), 0, s, p, lim64, d_hdr, d_ent, j);
@@ -357,20 +364,22 @@ int ax_pool::launch_chunk_kernels(hipStream_t s) {
       else hipLaunchKernelGGL((pool_step_kernel<float, false>), dim3(grid_step), dim3(64), 0, s, p, lim32, d_hdr, d_ent, j);
     }
   }
-  launch_winding(s, 1);
+  if (part == 2) launch_winding(s, 1);
   hipError_t e = hipGetLastError();
   return e == hipSuccess ? AX_OK : hip_fail(e, "pool chunk launch");
 }
 
 int ax_pool::capture_graph() {
-  hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
-  if (e != hipSuccess) return hip_fail(e, "begin capture");
-  int rc = launch_chunk_kernels(cs);
-  e = hipStreamEndCapture(cs, &graph);
-  if (rc) return rc;
-  if (e != hipSuccess) return hip_fail(e, "end capture");
-  e = hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0);
-  if (e != hipSuccess) return hip_fail(e, "graph instantiate");
+  for (int part = 0; part < kParts; ++part) {
+    hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) return hip_fail(e, "begin capture");
+    int rc = launch_chunk_kernels(cs, part);
+    e = hipStreamEndCapture(cs, &graph[part]);
+    if (rc) return rc;
+    if (e != hipSuccess) return hip_fail(e, "end capture");
+    e = hipGraphInstantiate(&gexec[part], graph[part], nullptr, nullptr, 0);
+    if (e != hipSuccess) return hip_fail(e, "graph instantiate");
+  }
   return AX_OK;
 }
 
@@ -452,6 +461,10 @@ void ax_pool::harvest(ChunkRec &c) {
     }
   }
   st.site_updates += c.site_updates;
+  float ms = 0;
+  if (hipEventElapsedTime(&ms, c.s0, c.s1) == hipSuccess) st.step_ms += ms;
+  (void)hipEventDestroy(c.s0);
+  (void)hipEventDestroy(c.s1);
   (void)hipEventDestroy(c.done);
   cv_done.notify_all();
 }
@@ -546,13 +559,34 @@ int ax_pool::enqueue_chunk() {
   h->pad = 0;
   hipError_t e = hipMemcpyAsync(d_hdr, h_tab[b], 64 + (size_t)count * sizeof(BatchEntry), hipMemcpyHostToDevice, cs);
   if (e != hipSuccess) return hip_fail(e, "table upload");
-  int rc = cfg.use_graph ? (hipGraphLaunch(gexec, cs) == hipSuccess ? AX_OK : hip_fail(hipGetLastError(), "graph launch"))
-                         : launch_chunk_kernels(cs);
-  if (rc) return rc;
+  e = hipEventCreate(&c.s0);
+  if (e == hipSuccess) e = hipEventCreate(&c.s1);
+  if (e != hipSuccess) {
+    if (c.s0) (void)hipEventDestroy(c.s0);
+    return hip_fail(e, "chunk timing events");
+  }
+  for (int part = 0; part < kParts; ++part) {
+    if (part == 1) e = hipEventRecord(c.s0, cs);
+    if (part == 2) e = hipEventRecord(c.s1, cs);
+    int rc = e != hipSuccess ? hip_fail(e, "chunk timing event")
+             : cfg.use_graph ? (hipGraphLaunch(gexec[part], cs) == hipSuccess ? AX_OK
+                                                                              : hip_fail(hipGetLastError(), "graph launch"))
+                             : launch_chunk_kernels(cs, part);
+    if (rc) {
+      (void)hipEventDestroy(c.s0);
+      (void)hipEventDestroy(c.s1);
+      return rc;
+    }
+  }
   e = hipMemcpyAsync(h_cnt[b], d_cnt, (size_t)count * kCnt * sizeof(unsigned long long), hipMemcpyDeviceToHost, cs);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&c.done, hipEventDisableTiming);
   if (e == hipSuccess) e = hipEventRecord(c.done, cs);
-  if (e != hipSuccess) return hip_fail(e, "chunk record");
+  if (e != hipSuccess) {
+    (void)hipEventDestroy(c.s0);
+    (void)hipEventDestroy(c.s1);
+    if (c.done) (void)hipEventDestroy(c.done);
+    return hip_fail(e, "chunk record");
+  }
   st.chunks++;
   next_seq++;
   inflight.push_back(std::move(c));
@@ -895,16 +929,21 @@ void ax_pool_destroy(ax_pool *P) {
     if (t.joinable()) t.join();
   if (P->cs) (void)hipStreamSynchronize(P->cs);
   if (P->us) (void)hipStreamSynchronize(P->us);
-  for (auto &c : P->inflight)
+  for (auto &c : P->inflight) {
     if (c.done) (void)hipEventDestroy(c.done);
+    if (c.s0) (void)hipEventDestroy(c.s0);
+    if (c.s1) (void)hipEventDestroy(c.s1);
+  }
   for (auto &kv : P->reqs)
     if (kv.second->upload) (void)hipEventDestroy(kv.second->upload);
   for (auto &sg : P->staging) {
     if (sg.done) (void)hipEventDestroy(sg.done);
     if (sg.host) (void)hipHostFree(sg.host);
   }
-  if (P->gexec) (void)hipGraphExecDestroy(P->gexec);
-  if (P->graph) (void)hipGraphDestroy(P->graph);
+  for (int part = 0; part < ax_pool::kParts; ++part) {
+    if (P->gexec[part]) (void)hipGraphExecDestroy(P->gexec[part]);
+    if (P->graph[part]) (void)hipGraphDestroy(P->graph[part]);
+  }
   for (int b = 0; b < ax_pool::kBufs; ++b) {
     if (P->h_tab[b]) (void)hipHostFree(P->h_tab[b]);
     if (P->h_cnt[b]) (void)hipHostFree(P->h_cnt[b]);

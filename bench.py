@@ -16,12 +16,16 @@ Prints ONE JSON line:
                 kernel's mean duration from HIP events on its stream, against
                 8 TB/s; traffic = HBM bytes per launch measured by rocprofv3
                 (committed summary under profiles/), null if none
-  cpu_baseline  the oracle (C++/OpenMP restatement of the reference step, glibc
-                sin/cos like the reference) timed on this host over a bounded
-                sample of the same lattice
+  cpu_baseline  the reference program as shipped (evolution.cpp, glibc sin/cos,
+                OpenMP; built from its sources into oracle/_ref/) timed on this
+                host's CPU share over 20 TimeSteps of a 128^3 lattice of the
+                same workload, with the oracle (the C++/OpenMP restatement of
+                the fused mpi_evolution.cpp step) on the same sample beside it
   fp32          the same run in FP32 storage/arithmetic (FP64 reductions), its
                 throughput and roofline, and its deviation from the FP64 run in
-                energy, Gauss residual and xi(t)
+                energy, Gauss residual and xi(t) over TimeSteps 0..1000
+                (--series-steps; both runs continue untimed past the timed
+                window)
   moore         the Moore model (two scalars, charges 2 and 1, one U(1) field)
                 at 256^3 FP64, mooreEvol.cpp couplings, 800 steps with winding
                 counts every 20: site-updates/s and the step kernel's roofline
@@ -83,8 +87,75 @@ def cpu_model():
     return None
 
 
-def cpu_baseline(n, steps, threads, log):
-    """Oracle step (reference arithmetic, libm trig) at n^3 for `steps` steps."""
+def logical_cpus():
+    try:
+        return len(os.sched_getaffinity(0))
+    except Exception:
+        return os.cpu_count() or 1
+
+
+def cpu_threads():
+    """Threads for the CPU baseline: OMP_NUM_THREADS when the host sets it
+    (a share of a shared host), else every physical core."""
+    v = os.environ.get("OMP_NUM_THREADS", "")
+    return int(v) if v.isdigit() and int(v) > 0 else physical_cores()
+
+
+REF_CPU_N = 128          # lattice of the reference CPU sample
+REF_CPU_STEPS = (2, 22)  # its two builds' TimeStep counts
+
+
+def reference_cpu_baseline(threads, log):
+    """The reference program as shipped (evolution.cpp, glibc sin/cos,
+    OpenMP, compiled from its own sources into oracle/_ref/ by
+    oracle/ref/build_ref.py) on the seeded random-phase 128^3 lattice with
+    the benchmark's couplings.  Its two builds differ only in nt; the
+    difference of their wall times is 20 TimeSteps of its loop (the update,
+    its energy/Gauss sums and the valsPerLoop line) without the ic.txt read.
+    None when the builds are absent."""
+    import subprocess
+    import numpy as np
+    from axions_amd.abi import ah_params
+    from oracle import oracle_py as orc
+    dirs = [os.path.join(REPO, "oracle", "_ref", f"cpu_bench_nt{k}") for k in REF_CPU_STEPS]
+    if not all(os.path.exists(os.path.join(d, "prog")) for d in dirs):
+        log("cpu baseline: reference builds absent")
+        return None
+    p = ah_params(REF_CPU_N)
+    st = orc.init_random_phase(p, 42, 0)
+    cur, _ = orc.levels(p, st, 0)
+    log(f"cpu baseline: writing the {REF_CPU_N}^3 Data/ic.txt")
+    for d in dirs:
+        os.makedirs(os.path.join(d, "Data"), exist_ok=True)
+    np.savetxt(os.path.join(dirs[0], "Data", "ic.txt"), cur.reshape(5, -1).T, fmt="%.17g")
+    ic1 = os.path.join(dirs[1], "Data", "ic.txt")
+    if os.path.exists(ic1):
+        os.remove(ic1)
+    os.link(os.path.join(dirs[0], "Data", "ic.txt"), ic1)
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads))
+    wall = []
+    for d, k in zip(dirs, REF_CPU_STEPS):
+        log(f"cpu baseline: reference evolution.cpp {REF_CPU_N}^3 nt={k}, {threads} threads")
+        t0 = time.perf_counter()
+        subprocess.run(["./prog"], cwd=d, input="bench\n", text=True, env=env, check=True,
+                       stdout=subprocess.DEVNULL)
+        wall.append(time.perf_counter() - t0)
+    for d in dirs:
+        for f in os.listdir(os.path.join(d, "Data")):
+            os.remove(os.path.join(d, "Data", f))
+    nsteps = REF_CPU_STEPS[1] - REF_CPU_STEPS[0]
+    dt = wall[1] - wall[0]
+    return {"value": REF_CPU_N ** 3 * nsteps / dt, "unit": "site-updates/s", "cores": threads,
+            "kind": "reference",
+            "sample": f"reference evolution.cpp as shipped (glibc sin/cos, g++ -O3 -fopenmp), "
+                      f"{REF_CPU_N}^3 periodic FP64, seeded random-phase IC, benchmark couplings; "
+                      f"{nsteps} TimeSteps = run(nt={REF_CPU_STEPS[1]}) {wall[1]:.2f} s - "
+                      f"run(nt={REF_CPU_STEPS[0]}) {wall[0]:.2f} s = {dt:.2f} s"}
+
+
+def port_cpu_baseline(n, steps, threads, log):
+    """The oracle's step (the reference's arithmetic in its fused
+    mpi_evolution.cpp form, glibc sin/cos, OpenMP) at n^3 for `steps` steps."""
     from axions_amd.abi import ah_params
     from oracle import oracle_py as orc
     p = ah_params(n)
@@ -96,9 +167,23 @@ def cpu_baseline(n, steps, threads, log):
     dt = time.perf_counter() - t0
     del st
     return {"value": n ** 3 * steps / dt, "unit": "site-updates/s", "cores": threads, "kind": "port",
-            "cpu": cpu_model(),
             "sample": f"oracle (C++/OpenMP restatement of mpi_evolution.cpp:381-501, glibc sin/cos) "
                       f"AH {n}^3 FP64, {steps} steps after 1 untimed, {dt:.2f} s"}
+
+
+def cpu_baseline(threads, log):
+    """The reference program timed on this host (the reported baseline),
+    with the oracle on the same lattice beside it."""
+    host = {"cpu": cpu_model(), "physical_cores": physical_cores(), "logical_cpus": logical_cpus(),
+            "omp_num_threads_env": os.environ.get("OMP_NUM_THREADS")}
+    port = port_cpu_baseline(REF_CPU_N, REF_CPU_STEPS[1] - REF_CPU_STEPS[0], threads, log)
+    ref = reference_cpu_baseline(threads, log)
+    out = dict(ref if ref is not None else port)
+    out.update(host)
+    if ref is not None:
+        out["port"] = port
+        out["port_over_reference"] = port["value"] / ref["value"]
+    return out
 
 
 def copy_calibration(nbytes, reps):
@@ -118,9 +203,10 @@ def copy_calibration(nbytes, reps):
     print(json.dumps({"copy_calibration_bytes": nbytes, "reps": reps}))
 
 
-def run_lattice(ax, n, precision, steps, warmup, diag_every, seed, log, moore=False, moore_dt=0.3):
+def run_lattice(ax, n, precision, steps, warmup, diag_every, seed, log, moore=False, moore_dt=0.3, series_to=0):
     """The headline loop on one lattice: warmup, then `steps` timed steps with
-    diagnostics every diag_every steps; returns throughput, the step kernel's
+    diagnostics every diag_every steps, then (untimed) on to TimeStep
+    series_to if that is further; returns throughput, the step kernel's
     mean time and the diagnostic series."""
     p = ax.moore_params(n, dt=moore_dt, precision=precision) if moore else ax.ah_params(n, precision=precision)
     log(f"lattice {n}^3 FP{precision}{' Moore' if moore else ''}: IC seed {seed}")
@@ -130,7 +216,7 @@ def run_lattice(ax, n, precision, steps, warmup, diag_every, seed, log, moore=Fa
 
         def diag():
             d = lat.diagnostics(winding=True)
-            if not d["finite"] and not moore:
+            if not d["finite"]:
                 raise RuntimeError(f"non-finite state at step {d['timestep']}")
             series.append((d["timestep"], d["energy"], d["gauss_mean"], d["wound_plaquettes"], d["xi"], d["finite"]))
 
@@ -154,6 +240,9 @@ def run_lattice(ax, n, precision, steps, warmup, diag_every, seed, log, moore=Fa
         t1 = time.perf_counter()
         kms, ksteps = lat.kernel_time()
         lat.kernel_timer(False)
+        if lat.timestep < series_to:
+            log(f"untimed steps {lat.timestep} -> {series_to} for the diagnostic series")
+            run(series_to - lat.timestep)
         diag()
     elapsed = t1 - t0
     sites = n ** 3
@@ -261,25 +350,21 @@ def compare_series(a, b):
     }
 
 
-def ensemble(ax, count, rate, seed, budget_gb, chunk, fp64_compare, log):
+def serve(ax, Ns, seeds, steps, arrivals, budget_gb, chunk, threads, log, tuned=None):
+    """One pass of the request stream through the FP32 pool: submissions at
+    the given arrival times, then wait for all.  Returns the load point's
+    figures, the requests' xi series and the tuned tiles."""
     import numpy as np
-    rng = np.random.default_rng(seed)
-    dx, dt = 0.5, 0.1
-    Ns = [int(v) for v in rng.choice(ENSEMBLE_NS, size=count)]
-    seeds = [int(v) for v in rng.integers(1, 2 ** 31, size=count)]
-    steps = [int(round(N * dx / (2 * dt))) for N in Ns]
-    gaps = rng.exponential(1.0 / rate, size=count)
-    arrivals = np.cumsum(gaps) - gaps[0]
+    count = len(Ns)
     total_su = float(sum(N ** 3 * s for N, s in zip(Ns, steps)))
-    threads = max(2, min(12, physical_cores() - 2))
-    log(f"ensemble: {count} requests, Poisson rate {rate}/s, {total_su:.3e} site-updates, {threads} IC threads")
-    out = {"requests": count, "n_choices": list(ENSEMBLE_NS), "arrival_rate_per_s": rate, "chunk_steps": chunk,
-           "site_updates_total": total_su, "dtype": "f32", "reductions": "f64"}
-    with ax.Pool(ax.ah_params(8, dx=dx, dt=dt, precision=32), hbm_budget=int(budget_gb * (1 << 30)),
+    with ax.Pool(ax.ah_params(8, dx=0.5, dt=0.1, precision=32), hbm_budget=int(budget_gb * (1 << 30)),
                  chunk_steps=chunk, autotune=1, ic_threads=threads) as pool:
-        tuned = {N: pool.tune(N) for N in ENSEMBLE_NS}
-        out["tuned_xseg"] = {str(N): x for N, (x, _) in tuned.items()}
-        log(f"tuned tiles: {out['tuned_xseg']}")
+        if tuned is None:
+            tuned = {N: pool.tune(N)[0] for N in ENSEMBLE_NS}
+            log(f"tuned tiles: {tuned}")
+        else:
+            for N in ENSEMBLE_NS:
+                pool.tune(N)
         ids = []
         t0 = time.perf_counter()
         for i in range(count):
@@ -296,19 +381,61 @@ def ensemble(ax, count, rate, seed, budget_gb, chunk, fp64_compare, log):
         st = pool.stats()
     lat = np.array([r["t_done"] - r["t_submit"] for r in infos])
     failed = sum(1 for r in infos if r["status"] != 2)
-    out.update({
+    # the batched step kernel against the HBM roofline: 60 algorithmic bytes
+    # per site-update (FP32), over the GPU time of the chunks' step launches
+    step_s = st["step_ms"] / 1e3
+    achieved = 60.0 * st["site_updates"] / step_s / 1e9 if step_s > 0 else None
+    out = {
+        "arrival_rate_per_s": float(count / arrivals[-1]) if arrivals[-1] > 0 else None,
         "failed": failed,
         "wall_s": wall,
         "realizations_per_hour": count / wall * 3600,
         "site_updates_per_s": total_su / wall,
         "latency_p50_s": float(np.percentile(lat, 50)), "latency_p99_s": float(np.percentile(lat, 99)),
         "latency_mean_s": float(lat.mean()),
+        "service_p50_s": float(np.percentile([r["t_done"] - r["t_admit"] for r in infos], 50)),
+        "queue_wait_p50_s": float(np.percentile([r["t_admit"] - r["t_submit"] for r in infos], 50)),
         "max_resident": st["max_resident"], "peak_hbm_bytes": st["peak_in_use"], "graph_chunks": st["chunks"],
-    })
-    log(f"pool: {out['realizations_per_hour']:.0f} realizations/h, {out['site_updates_per_s']:.3e} SU/s, "
-        f"p50 {out['latency_p50_s']:.2f} s, p99 {out['latency_p99_s']:.2f} s")
+        "step_kernel_share_of_wall": step_s / wall,
+        "pool_step_roofline": None if achieved is None else {
+            "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "bytes_per_site": 60,
+            "step_kernel_s": step_s, "site_updates": st["site_updates"]},
+    }
+    log(f"pool @ {out['arrival_rate_per_s']:.1f}/s: {out['realizations_per_hour']:.0f} realizations/h, "
+        f"{out['site_updates_per_s']:.3e} SU/s, p50 {out['latency_p50_s']:.2f} s, p99 {out['latency_p99_s']:.2f} s")
+    return out, xi32, tuned
+
+
+def ensemble(ax, count, rate, seed, budget_gb, chunk, fp64_compare, log, light_frac=0.7):
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    dx, dt = 0.5, 0.1
+    Ns = [int(v) for v in rng.choice(ENSEMBLE_NS, size=count)]
+    seeds = [int(v) for v in rng.integers(1, 2 ** 31, size=count)]
+    steps = [int(round(N * dx / (2 * dt))) for N in Ns]
+    gaps = rng.exponential(1.0, size=count)          # unit-rate Poisson process, scaled per load point
+    unit = np.cumsum(gaps) - gaps[0]
+    total_su = float(sum(N ** 3 * s for N, s in zip(Ns, steps)))
+    threads = max(2, min(12, cpu_threads() - 2))
+    log(f"ensemble: {count} requests, Poisson rate {rate}/s, {total_su:.3e} site-updates, {threads} IC threads")
+    out = {"requests": count, "n_choices": list(ENSEMBLE_NS), "chunk_steps": chunk,
+           "site_updates_total": total_su, "dtype": "f32", "reductions": "f64"}
+    sat, xi32, tuned = serve(ax, Ns, seeds, steps, unit / rate, budget_gb, chunk, threads, log)
+    out["tuned_xseg"] = {str(N): x for N, x in tuned.items()}
+    out.update(sat)
+    # the same stream below saturation: arrivals at light_frac of the
+    # capacity just measured (completions per second of the saturated pass)
+    capacity = count / sat["wall_s"]
+    light_rate = light_frac * capacity
+    log(f"ensemble at {light_frac:.0%} of capacity: {light_rate:.2f} arrivals/s")
+    light, _, _ = serve(ax, Ns, seeds, steps, unit / light_rate, budget_gb, chunk, threads, log, tuned=tuned)
+    light["offered_load_frac_of_capacity"] = light_frac
+    out["below_saturation"] = light
+    out["capacity_per_s"] = capacity
     if not fp64_compare:
         return out
+    arrivals = unit / rate
     # the same stream, one realization at a time through the FP64 plain path
     log("fp64 plain path: the same requests one by one")
     xi64 = []
@@ -382,10 +509,11 @@ def main():
     ap.add_argument("--n", type=int, default=512)
     ap.add_argument("--precision", type=int, default=64, choices=(64, 32))
     ap.add_argument("--diag-every", type=int, default=20)
+    ap.add_argument("--series-steps", type=int, default=1000,
+                    help="the FP64 and FP32 runs continue (untimed) to this TimeStep for their diagnostic "
+                         "series and the FP32-vs-FP64 comparison")
     ap.add_argument("--seed", type=int, default=42)
     ap.add_argument("--cpu-baseline", choices=("on", "off"), default="on")
-    ap.add_argument("--cpu-steps", type=int, default=2)
-    ap.add_argument("--cpu-n", type=int, default=0, help="lattice for the CPU sample (default: --n)")
     ap.add_argument("--fp32", choices=("on", "off"), default="on", help="FP32 run of the same workload")
     ap.add_argument("--moore", choices=("on", "off"), default="on", help="Moore-model 256^3 run")
     ap.add_argument("--moore-n", type=int, default=256)
@@ -397,7 +525,8 @@ def main():
     ap.add_argument("--ensemble-rate", type=float, default=50.0, help="Poisson arrivals per second")
     ap.add_argument("--ensemble-seed", type=int, default=2024)
     ap.add_argument("--ensemble-budget-gb", type=float, default=96.0)
-    ap.add_argument("--ensemble-chunk", type=int, default=10)
+    ap.add_argument("--ensemble-chunk", type=int, default=20,
+                    help="TimeSteps per graph chunk = xi(t) cadence (the 512^3 run's diagnostic cadence)")
     ap.add_argument("--ensemble-fp64", choices=("on", "off"), default="on")
     ap.add_argument("--boundary-leg", choices=("on", "off"), default="on",
                     help="evolution.cpp schedule with absorbing x/y faces at --boundary-n")
@@ -419,10 +548,11 @@ def main():
         sys.exit(2)
 
     import axions_amd as ax
-    main_run = run_lattice(ax, a.n, a.precision, a.steps, a.warmup, a.diag_every, a.seed, log)
+    main_run = run_lattice(ax, a.n, a.precision, a.steps, a.warmup, a.diag_every, a.seed, log,
+                           series_to=a.series_steps)
     fp32 = None
     if a.fp32 == "on" and a.precision == 64:
-        r32 = run_lattice(ax, a.n, 32, a.steps, a.warmup, a.diag_every, a.seed, log)
+        r32 = run_lattice(ax, a.n, 32, a.steps, a.warmup, a.diag_every, a.seed, log, series_to=a.series_steps)
         fp32 = {"value": r32["value"], "unit": "site-updates/s", "ms_per_step": r32["ms_per_step"],
                 "roofline": r32["roofline"], "vs_fp64": compare_series(main_run["series"], r32["series"])}
     moore = None
@@ -433,7 +563,9 @@ def main():
                              f"dt={a.moore_dt}, {a.moore_steps} steps, energy+Gauss+winding every 20",
                  "value": rm["value"], "unit": "site-updates/s", "ms_per_step": rm["ms_per_step"],
                  "roofline": rm["roofline"], "final_wound_plaquettes": last_m[3],
-                 "finite_to_end": bool(all(r[5] for r in rm["series"]))}
+                 "finite_to_end": bool(all(r[5] for r in rm["series"])),
+                 "dt_note": "mooreEvol.cpp's own dt = 0.3 overflows from this random-phase IC within "
+                            "16-22 steps (oracle, 32^3-96^3; tests/test_moore_timestep.py)"}
     ens = None
     if a.ensemble == "on":
         ens = ensemble(ax, a.ensemble_count, a.ensemble_rate, a.ensemble_seed, a.ensemble_budget_gb,
@@ -442,7 +574,7 @@ def main():
     sor = sor_leg(ax, a.sor_cpu_sweeps if a.cpu_baseline == "on" else 0, log) if a.sor == "on" else None
     cpu = None
     if a.cpu_baseline == "on":
-        cpu = cpu_baseline(a.cpu_n or a.n, a.cpu_steps, min(16, physical_cores()), log)
+        cpu = cpu_baseline(cpu_threads(), log)
 
     last = main_run["series"][-1]
     out = {

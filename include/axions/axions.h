@@ -320,6 +320,8 @@ typedef struct ax_pool_stats {
   double site_updates;           /* sum over completed steps of N^3 */
   double elapsed;                /* seconds since pool creation */
   size_t budget, in_use, peak_in_use;
+  double step_ms;                /* GPU milliseconds of the step launches of those steps
+                                    (HIP events around each chunk's steps) */
 } ax_pool_stats;
 
 void ax_pool_default_config(ax_pool_config *cfg);

@@ -198,3 +198,39 @@ def test_straight_string_pipeline(tmp_path):
     orc.step(p, st, 0, 6, 0, 16)
     cur, _ = orc.levels(p, st, 6)
     assert np.array_equal(fin.T, cur.reshape(5, -1))
+
+
+@pytest.mark.parametrize("program", ["evolution", "mpi_evolution"])
+def test_two_level_restart(tmp_path, program):
+    """Restart from a two-level Data/ic.txt -- the level-0 block, then the
+    level-1 block, 5 columns per site in x-major order (evolution.cpp:122-134
+    with stationary_ic = false; mpi_evolution.cpp:229-285 with ic_type data):
+    the run continues the trajectory the two levels were taken from, as the
+    oracle continues it (energies every step to 1e-12, final field bit for
+    bit)."""
+    d = str(tmp_path)
+    os.makedirs(os.path.join(d, "Data"), exist_ok=True)
+    nx, ny, nz, k, nt = 12, 10, 14, 6, 10
+    mpi = program == "mpi_evolution"
+    p = ah_params(nx, ny=ny, nz=nz, g=1.0, theta_damp_form=0 if mpi else 1, energy_form=1 if mpi else 0)
+    st = orc.init_random_phase(p, 5)
+    orc.step(p, st, 0, k, 0, 16)        # k even: level 1 is the current level of TimeStep 0 again
+    V = nx * ny * nz
+    s5 = st.reshape(5, 2, V)
+    np.savetxt(os.path.join(d, "Data", "ic.txt"), np.concatenate([s5[:, 0].T, s5[:, 1].T]), fmt="%.17g")
+    common = [f"nx={nx}", f"ny={ny}", f"nz={nz}", f"nt={nt}", "dx=0.5", "dy=0.5", "dz=0.5", "dt=0.1", "lambda=2",
+              "g=1", "damped_nt=0", "alpha=0", "stringDetect=false"]
+    if mpi:
+        run(["--program", "mpi_evolution", *common, "ic_type=data", "seed=3", "calcEnergy=true", "outTag=r"], d)
+        e = np.loadtxt(os.path.join(d, "Data", f"valsPerLoop_r_nx{nx}_seed3.txt"))
+    else:
+        run(["--program", "evolution", "--tag", "r", *common, "stationary_ic=false", "finalOut=true"], d)
+        e = np.loadtxt(os.path.join(d, "Data", "valsPerLoop_r.txt"))[:, 0]
+    assert e.shape == (nt,)
+    for t in range(nt):
+        assert rel(e[t], orc.diag(p, st, t, 0)["energy"]) < 1e-12, t
+        orc.step(p, st, t, 1, 0, 16)
+    if not mpi:
+        fin = np.loadtxt(os.path.join(d, "Data", "finalField.txt")).T
+        cur, _ = orc.levels(p, st, nt)
+        assert np.array_equal(fin, cur.reshape(5, -1))
