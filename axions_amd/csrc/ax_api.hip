@@ -24,7 +24,9 @@
 #include <vector>
 
 #include "ax_device.h"
+#include "ax_energy.h"
 #include "ax_internal.h"
+#include "ax_march.h"
 
 namespace ax {
 thread_local std::string g_last_error;
@@ -115,6 +117,12 @@ struct ax_lattice {
   double *cxyz = nullptr;
   size_t cxyz_cap = 0;
   int *band = nullptr;      /* Moore: per-level band flags of the FP64 march (ax_march_moore.h), TimeStep mod 3 */
+  double *divb = nullptr;   /* fused diagnostics: divergence of the links of level div_for - 1 [V] */
+  int div_for = -1;         /* the TimeStep whose diagnostics divb serves (-1: none) */
+  double *fpart = nullptr;  /* fused diagnostics: partial sums per step work item */
+  size_t fpart_n = 0;
+  double *fres = nullptr;   /* fused diagnostics: 8 doubles per diagnostic point */
+  size_t fres_n = 0;
   int band_steps = 0;       /* marching steps since the state or TimeStep was last set from the host */
   bool timing = false;                         /* ax_kernel_timer */
   std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
@@ -148,6 +156,7 @@ int lattice_timestep(const ax_lattice *L) { return L->timestep; }
 void lattice_reset_timestep(ax_lattice *L) {
   L->timestep = 0;
   L->band_steps = 0;
+  L->div_for = -1;
 }
 
 }  // namespace ax
@@ -255,6 +264,9 @@ void ax_destroy(ax_lattice *L) {
   if (L->part) (void)hipFree(L->part);
   if (L->dres) (void)hipFree(L->dres);
   if (L->band) (void)hipFree(L->band);
+  if (L->divb) (void)hipFree(L->divb);
+  if (L->fpart) (void)hipFree(L->fpart);
+  if (L->fres) (void)hipFree(L->fres);
   if (L->hres) (void)hipHostFree(L->hres);
   if (L->wind) (void)hipFree(L->wind);
   if (L->phase) (void)hipFree(L->phase);
@@ -276,6 +288,7 @@ int ax_set_timestep(ax_lattice *L, int32_t ts) {
   if (!L || ts < 0) return fail(AX_EINVAL, "bad timestep");
   L->timestep = ts;
   L->band_steps = 0;
+  L->div_for = -1;
   return AX_OK;
 }
 
@@ -319,8 +332,10 @@ int ax_set_state(ax_lattice *L, const double *host, size_t n) {
   if (!L || !host) return fail(AX_EINVAL, "NULL argument");
   const size_t want = (size_t)host_planes(L) * L->V;
   if (n != want) return fail(AX_EINVAL, "state length %zu != %zu", n, want);
-  /* no band flag describes the state from here on, even if an upload fails */
+  /* no band flag (nor stored divergence) describes the state from here on,
+   * even if an upload fails */
   L->band_steps = 0;
+  L->div_for = -1;
   std::vector<float> tmp;
   for (int m = 0; m < host_planes(L); ++m) {
     int rc = upload_plane(L, device_plane(L, m), host + (size_t)m * L->V, tmp);
@@ -410,6 +425,7 @@ int ax_init_random_phase(ax_lattice *L, uint32_t seed, int32_t mode) {
   }
   L->timestep = 0;
   L->band_steps = 0;
+  L->div_for = -1;
   return AX_OK;
 }
 
@@ -488,6 +504,34 @@ int ax_kernel_time(ax_lattice *L, double *ms, int64_t *steps) {
   return AX_OK;
 }
 
+}  // extern "C"
+
+/* ax_diag from the device results h (0: energy, 1: sum |G|, 2: non-finite
+ * count, 4..6: wound plaquettes xy, xz, yz) of TimeStep ts */
+static void fill_diag(const ax_lattice *L, const double *h, int ts, bool want_winding, ax_diag *out) {
+  const ax_params &p = L->p;
+  const bool glob = p.model == AX_MODEL_GLOBAL;
+  memset(out, 0, sizeof *out);
+  out->timestep = ts;
+  out->energy = (glob && ts == 0) ? NAN : h[0];
+  double dev = h[1];
+  if (p.model != AX_MODEL_MOORE || p.boundary == AX_BC_PERIODIC) {
+    /* evolution.cpp:1404-1408 (Moore with fixed edges divides per site) */
+    dev = p.boundary == AX_BC_PERIODIC ? dev / (p.nx * p.ny) : dev / ((p.nx - 2) * (p.ny - 2));
+    dev = p.boundary_z == AX_BC_PERIODIC ? dev / p.nz : dev / (p.nz - 2);
+  }
+  out->gauss_mean = ((p.model == AX_MODEL_AH && p.energy_form == 1) || glob) ? 0.0 : dev;
+  out->finite = h[2] == 0 ? 1 : 0;
+  if (want_winding) {
+    const double wxy = h[4], wxz = h[5], wyz = h[6];
+    out->wound_plaquettes = (int64_t)(wxy + wxz + wyz);
+    out->string_length = (2.0 / 3.0) * (wxy * p.dz + wxz * p.dy + wyz * p.dx);
+    out->xi = out->string_length / ((double)L->V * p.dx * p.dy * p.dz);
+  }
+}
+
+extern "C" {
+
 int ax_diagnostics(ax_lattice *L, int32_t want_winding, ax_diag *out) {
   if (!L || !out) return fail(AX_EINVAL, "NULL argument");
   const ax_params &p = L->p;
@@ -530,23 +574,161 @@ int ax_diagnostics(ax_lattice *L, int32_t want_winding, ax_diag *out) {
   e = hipMemcpyAsync(L->hres, L->dres, 8 * sizeof(double), hipMemcpyDeviceToHost, L->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(L->stream);
   if (e != hipSuccess) return hip_fail(e, "diag readback");
-  memset(out, 0, sizeof *out);
-  out->timestep = L->timestep;
-  out->energy = (glob && L->timestep == 0) ? NAN : L->hres[0];
-  double dev = L->hres[1];
-  if (p.model != AX_MODEL_MOORE || p.boundary == AX_BC_PERIODIC) {
-    /* evolution.cpp:1404-1408 (Moore with fixed edges divides per site) */
-    dev = p.boundary == AX_BC_PERIODIC ? dev / (p.nx * p.ny) : dev / ((p.nx - 2) * (p.ny - 2));
-    dev = p.boundary_z == AX_BC_PERIODIC ? dev / p.nz : dev / (p.nz - 2);
+  fill_diag(L, L->hres, L->timestep, want_winding != 0, out);
+  return AX_OK;
+}
+
+}  // extern "C"
+
+/* The fused diagnostics apply: FP64 periodic Abelian-Higgs with the
+ * evolution.cpp energy form and g != 0, on the marching kernels (planes
+ * below 4 GiB). */
+static bool fused_ok(const ax_lattice *L) {
+  const ax_params &p = L->p;
+  return p.model == AX_MODEL_AH && p.precision == 64 && p.boundary == AX_BC_PERIODIC &&
+         p.boundary_z == AX_BC_PERIODIC && p.energy_form == 0 && p.g != 0 && use_tiled_ah(p) && use_diag_march(p) &&
+         (double)L->V * 8.0 < 4294967296.0 && getenv("AX_NO_FUSED_DIAG") == nullptr;
+}
+
+/* device buffers of the fused path for up to npoints diagnostic points */
+static int ensure_fused(ax_lattice *L, size_t npoints, bool winding) {
+  const size_t items = (size_t)fused_work_items(L->p.nx, L->p.ny, L->p.nz, L->xseg);
+  auto grow = [&](double **buf, size_t *have, size_t want, const char *what) -> int {
+    if (*have >= want) return AX_OK;
+    if (*buf) (void)hipFree(*buf);
+    *buf = nullptr;
+    *have = 0;
+    if (hipMalloc((void **)buf, want * sizeof(double)) != hipSuccess) {
+      (void)hipGetLastError();
+      return fail(AX_ENOMEM, "%s of %zu doubles", what, want);
+    }
+    *have = want;
+    return AX_OK;
+  };
+  size_t have_div = L->divb ? L->V : 0;
+  int rc = grow(&L->divb, &have_div, L->V, "divergence buffer");
+  if (!rc) rc = grow(&L->fpart, &L->fpart_n, 3 * items, "partial sums");
+  if (!rc) rc = grow(&L->fres, &L->fres_n, 8 * (npoints ? npoints : 1), "diagnostic results");
+  if (!rc) rc = ensure_phase_buffer(L);
+  if (!rc && winding) rc = ensure_winding_buffer(L);
+  return rc;
+}
+
+/* one fused step of mode 1 (store the divergence of the current level) or 2
+ * (the current level's diagnostics into fres[8 k ...]) */
+static int launch_fused(ax_lattice *L, int mode, size_t k, bool winding) {
+  const int n = L->timestep;
+  const ax_params &p = L->p;
+  FusedDiag fd;
+  fd.div = L->divb;
+  fd.phase = L->phase;
+  fd.part = L->fpart;
+  fd.ek = energy_consts(p.dx, p.dy, p.dz, p.dt, p.g, p.lambda, p.eta);
+  const StepConsts sk = step_consts(&p, n);
+  const Levels<double> lv = levels_of<double>(L, n);
+  hipError_t e = hipSuccess;
+  if (mode == 2 && L->div_for != n) {
+    /* no fused step stored level n-1's divergence: take it from the
+     * previous level (intact until this step overwrites it) */
+    const bool pow2 = is_pow2(sk.dx2) && is_pow2(sk.dy2) && is_pow2(sk.dz2);
+    e = launch_level_div(p.nx, p.ny, p.nz, lv.prv[2], lv.prv[3], lv.prv[4], fd.ek, pow2, L->divb, L->stream);
   }
-  out->gauss_mean = ((p.model == AX_MODEL_AH && p.energy_form == 1) || glob) ? 0.0 : dev;
-  out->finite = L->hres[2] == 0 ? 1 : 0;
-  if (want_winding) {
-    const double wxy = L->hres[4], wxz = L->hres[5], wyz = L->hres[6];
-    out->wound_plaquettes = (int64_t)(wxy + wxz + wyz);
-    out->string_length = (2.0 / 3.0) * (wxy * p.dz + wxz * p.dy + wyz * p.dx);
-    out->xi = out->string_length / ((double)L->V * p.dx * p.dy * p.dz);
+  if (e == hipSuccess)
+    e = launch_step_fused(sk, lv, L->xseg, L->stream, mode, diag_pow2(p.dx, p.dy, p.dz, p.g), fd);
+  if (e == hipSuccess && mode == 2) {
+    double *res = L->fres + 8 * k;
+    e = launch_sum3(L->fpart, fused_work_items(p.nx, p.ny, p.nz, L->xseg), res, L->stream);
+    /* windings of level n: its links are intact until the next step */
+    if (e == hipSuccess && winding) e = launch_phase_winding(L, L->wind, res + 4);
   }
+  if (e != hipSuccess) return hip_fail(e, "fused step launch");
+  L->div_for = mode == 1 ? n + 1 : -1;
+  L->band_steps = 0;
+  return AX_OK;
+}
+
+/* plain steps, bracketed by the kernel timer's events when it is on */
+static int plain_steps(ax_lattice *L, int nsteps) {
+  if (nsteps <= 0) return AX_OK;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (L->timing) {
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+      if (e0) (void)hipEventDestroy(e0);
+      if (e1) (void)hipEventDestroy(e1);
+      return fail(AX_EHIP, "event");
+    }
+    (void)hipEventRecord(e0, L->stream);
+  }
+  for (int s = 0; s < nsteps; ++s) {
+    int rc = launch_step(L, L->timestep);
+    if (rc) {
+      if (e0) (void)hipEventDestroy(e0);
+      if (e1) (void)hipEventDestroy(e1);
+      return rc;
+    }
+    L->timestep++;
+  }
+  if (e0) {
+    (void)hipEventRecord(e1, L->stream);
+    L->ev.emplace_back(e0, e1);
+    L->timed_steps += nsteps;
+  }
+  return AX_OK;
+}
+
+extern "C" {
+
+int ax_step_diag(ax_lattice *L, int32_t nsteps, int32_t every, int32_t want_winding, ax_diag *out, size_t cap,
+                 size_t *count) {
+  if (!L || nsteps < 0 || every <= 0 || !count || (cap && !out)) return fail(AX_EINVAL, "bad step_diag request");
+  *count = 0;
+  std::vector<int> at;   /* the TimeSteps whose diagnostics are taken */
+  for (int s = 0; s < nsteps && at.size() < cap; ++s)
+    if ((L->timestep + s) % every == 0) at.push_back(L->timestep + s);
+  if (!fused_ok(L)) {
+    /* the separate diagnostics pass before each such step */
+    size_t k = 0;
+    for (int s = 0; s < nsteps; ++s) {
+      if (k < at.size() && L->timestep == at[k]) {
+        int rc = ax_diagnostics(L, want_winding, &out[k]);
+        if (rc) return rc;
+        *count = ++k;
+      }
+      int rc = plain_steps(L, 1);
+      if (rc) return rc;
+    }
+    return AX_OK;
+  }
+  int rc = ensure_fused(L, at.size(), want_winding != 0);
+  if (rc) return rc;
+  size_t k = 0;
+  int run = 0;   /* plain steps not yet launched */
+  for (int s = 0; s < nsteps; ++s) {
+    const int t = L->timestep + run;
+    const bool diag_here = k < at.size() && t == at[k];
+    const bool pre = !diag_here && (t + 1) % every == 0;
+    if (!diag_here && !pre) {
+      ++run;
+      continue;
+    }
+    rc = plain_steps(L, run);
+    run = 0;
+    if (rc) return rc;
+    rc = launch_fused(L, diag_here ? 2 : 1, k, want_winding != 0);
+    if (rc) return rc;
+    if (diag_here) ++k;
+    L->timestep++;
+  }
+  rc = plain_steps(L, run);
+  if (rc) return rc;
+  if (k) {
+    std::vector<double> h(8 * k);
+    hipError_t e = hipMemcpyAsync(h.data(), L->fres, 8 * k * sizeof(double), hipMemcpyDeviceToHost, L->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(L->stream);
+    if (e != hipSuccess) return hip_fail(e, "diagnostic readback");
+    for (size_t j = 0; j < k; ++j) fill_diag(L, &h[8 * j], at[j], want_winding != 0, &out[j]);
+  }
+  *count = k;
   return AX_OK;
 }
 
@@ -724,6 +906,7 @@ int launch_winding_field(ax_lattice *L, int field, int8_t *out, double *part, do
 
 int launch_step(ax_lattice *L, int n) {
   hipError_t e;
+  L->div_for = -1;
   if (L->p.model == AX_MODEL_GLOBAL) {
     e = L->esize == 8 ? launch_global_step<double>(L->p, L->state, L->pitch, n, L->stream)
                       : launch_global_step<float>(L->p, L->state, L->pitch, n, L->stream);

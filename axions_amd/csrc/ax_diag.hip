@@ -17,6 +17,7 @@
  * evolution.cpp:390-391), so they run as a separate pass before the in-place
  * update of the same TimeStep.
  */
+#include "ax_energy.h"
 #include "ax_internal.h"
 #include "ax_march.h"
 #include "ax_wind.h"
@@ -419,14 +420,7 @@ __global__ void __launch_bounds__(256) cross_write_kernel(int nx, int ny, int nz
  * a plaquette needs four edge phases of three sites; computing each edge
  * once here instead of nine per site in winding_kernel is the saving. */
 
-/* link_phase with the link's sin/cos already at hand (the same arithmetic) */
-__device__ __forceinline__ double link_phase_sc(double cr, double ci, double nr, double ni, double sn, double c) {
-  double wr = c * nr + sn * ni;
-  double wi = c * ni - sn * nr;
-  double re = cr * wr + ci * wi;
-  double im = cr * wi - ci * wr;
-  return ax_atan2(im, re);
-}
+/* link_phase_sc (ax_energy.h): link_phase with the link's sin/cos already at hand */
 
 /* P2: dx, dy, dz and g (when g != 0) are powers of two, so every division by
  * dx, dx*dx, g or (g*dx*dy)^2 is the multiplication by its exact reciprocal
@@ -694,6 +688,44 @@ hipError_t launch_diag_march(const DiagConsts &k, const Levels<T> &L, int xseg, 
 #undef AX_DM
   hipLaunchKernelGGL(sum3_kernel, dim3(1), dim3(1024), 0, st, part, g.work, out3);
   return hipGetLastError();
+}
+
+hipError_t launch_sum3(const double *part, int nb, double *out3, hipStream_t st) {
+  hipLaunchKernelGGL(sum3_kernel, dim3(1), dim3(1024), 0, st, part, nb, out3);
+  return hipGetLastError();
+}
+
+/* div[s] = sum_a (theta_a(s) - theta_a(s-a))/dx_a^2 of one level (periodic),
+ * the arithmetic of the fused step's stored divergence (ax_march.h). */
+template <bool P2>
+__global__ void __launch_bounds__(256) level_div_kernel(int nx, int ny, int nz, const double *tx, const double *ty,
+                                                        const double *tz, EnergyConsts ek, double *div) {
+  const size_t V = (size_t)nx * ny * nz;
+  const size_t s = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (s >= V) return;
+  const int kk = (int)(s % nz), r = (int)(s / nz), j = r % ny, i = r / ny;
+  auto id = [&](int a, int b, int c) { return ((size_t)a * ny + b) * nz + c; };
+  div[s] = link_div<P2>(ek, tx[s], tx[id(wrapm(i, nx), j, kk)], ty[s], ty[id(i, wrapm(j, ny), kk)], tz[s],
+                        tz[id(i, j, wrapm(kk, nz))]);
+}
+
+hipError_t launch_level_div(int nx, int ny, int nz, const double *tx, const double *ty, const double *tz,
+                            const EnergyConsts &ek, bool pow2, double *div, hipStream_t st) {
+  const size_t V = (size_t)nx * ny * nz;
+  const unsigned nb = (unsigned)((V + 255) / 256);
+  if (pow2) hipLaunchKernelGGL((level_div_kernel<true>), dim3(nb), dim3(256), 0, st, nx, ny, nz, tx, ty, tz, ek, div);
+  else hipLaunchKernelGGL((level_div_kernel<false>), dim3(nb), dim3(256), 0, st, nx, ny, nz, tx, ty, tz, ek, div);
+  return hipGetLastError();
+}
+
+/* the P2 condition of the diagnostics march: dx, dy, dz, g and the derived
+ * divisors are powers of two with finite reciprocals */
+bool diag_pow2(double dx, double dy, double dz, double gg) {
+  const double dvs[10] = {dx, dy, dz, dx * dx, dy * dy, dz * dz, gg, (gg * dx * dy) * (gg * dx * dy),
+                          (gg * dx * dz) * (gg * dx * dz), (gg * dy * dz) * (gg * dy * dz)};
+  bool p2 = gg != 0;
+  for (double d : dvs) p2 = p2 && is_pow2(fabs(d)) && std::isfinite(1 / d);
+  return p2;
 }
 
 template <typename T>

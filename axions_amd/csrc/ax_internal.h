@@ -142,6 +142,17 @@ hipError_t launch_crossings(int nx, int ny, int nz, double dx, double dy, double
                             int ex, int ez, int *bcount, long long *boff, long long *total, double *xyz,
                             long long cap, hipStream_t st, bool write);
 
+/* Fused diagnostics (ax_march.h MODE 1/2, FP64 periodic Abelian-Higgs) */
+struct FusedDiag;
+struct EnergyConsts;
+int fused_work_items(int nx, int ny, int nz, int xseg);
+hipError_t launch_step_fused(const StepConsts &k, const Levels<double> &L, int xseg, hipStream_t st, int mode,
+                             bool p2d, const FusedDiag &fd);
+hipError_t launch_sum3(const double *part, int nb, double *out3, hipStream_t st);
+hipError_t launch_level_div(int nx, int ny, int nz, const double *tx, const double *ty, const double *tz,
+                            const EnergyConsts &ek, bool pow2, double *div, hipStream_t st);
+bool diag_pow2(double dx, double dy, double dz, double gg);
+
 /* Boundary work of one TimeStep on a lattice that is not periodic
  * (ax_bound.hip).  launch_neumann_faces sets the faces of level tN from
  * their inner neighbours; launch_absorbing steps the x/y faces and corners

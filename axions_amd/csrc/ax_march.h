@@ -38,6 +38,7 @@
 #define AXIONS_AX_MARCH_H
 
 #include "ax_device.h"
+#include "ax_energy.h"
 
 namespace ax {
 
@@ -156,6 +157,7 @@ struct MarchCarry {
   R PRm, PIm;          /* phi(s-x) */
   R cxm, sxm;          /* cos/sin theta_x(s-x) */
   R Qyx_m, Qzx_m;      /* plaquette sines at s-x needed by F_yx, F_zx */
+  R TXm;               /* theta_x(s-x) (read only by the divergence of the fused diagnostics) */
 };
 
 /* What the site before a chunk (z0 - 1) contributes to the chunk's first
@@ -165,6 +167,16 @@ struct MarchCarry {
 template <typename R>
 struct ZEdge {
   R s, c, qxz, qyz;
+  R tz;                /* theta_z there (the fused diagnostics' divergence) */
+};
+
+/* The link trigonometry and plaquette angles of one site update, handed to
+ * the diagnostics a fused step takes of its current level. */
+template <typename R>
+struct SiteTrig {
+  R cx, sx, cy, sy, cz, sz;    /* cos/sin theta_a(s) */
+  R cym, sym, czm, szm;        /* cos/sin theta_y(s-y), theta_z(s-z) */
+  R a_xy, a_xz, a_yz;          /* plaquette angles at s */
 };
 
 /* The new previous-level values (level n+1) of one site. */
@@ -241,9 +253,9 @@ __device__ __forceinline__ uint64_t big_lanes(R d, R lim) {
  * chunk-edge values ze (used by lane 0 only).  Every lane of the wave must
  * call it (it votes across the wave with ballots and shifts values between
  * lanes). */
-template <typename R, bool POW2, typename G>
+template <typename R, bool POW2, typename G, bool TR = false>
 __device__ __forceinline__ void ah_site_update(const MarchConsts<R> &k, bool plain, const G &g, MarchCarry<R> &m,
-                                               const ZEdge<R> &ze, SiteOut<R> &o) {
+                                               const ZEdge<R> &ze, SiteOut<R> &o, SiteTrig<R> *tr = nullptr) {
   typedef TrigR<R> T;
   const R dx2 = k.dx2, dy2 = k.dy2, dz2 = k.dz2, dt = k.dt, dt2 = k.dt2;
   const R hl = k.hl, eta2 = k.eta2, m2gg = k.m2gg, fric = k.fric, damp = k.damp;
@@ -278,6 +290,11 @@ __device__ __forceinline__ void ah_site_update(const MarchConsts<R> &k, bool pla
     Qxy = T::sn(a_xy); Qxz = T::sn(a_xz); Qyx = T::sn(a_yx);
     Qyz = T::sn(a_yz); Qzx = T::sn(a_zx); Qzy = T::sn(a_zy);
     Qxy_m = T::sn(b_xy); Qzy_m = T::sn(b_zy);
+  }
+  if constexpr (TR) {
+    tr->cx = cx; tr->sx = sx; tr->cy = cy; tr->sy = sy; tr->cz = cz; tr->sz = sz;
+    tr->cym = cym; tr->sym = sym_; tr->czm = czm; tr->szm = szm;
+    tr->a_xy = a_xy; tr->a_xz = a_xz; tr->a_yz = a_yz;
   }
   /* the second sines of F_xz, F_yz (arguments TXzm + TZxpzm - TXc - TZzm and
    * TYzm + TZypzm - TYc - TZzm) are lane - 1's first ones */
@@ -343,6 +360,7 @@ __device__ __forceinline__ void ah_site_update(const MarchConsts<R> &k, bool pla
   /* carry to x+1 */
   m.PRm = PRc; m.PIm = PIc;
   m.cxm = cx; m.sxm = sx; m.Qyx_m = Qyx; m.Qzx_m = Qzx;
+  m.TXm = TXc;
 }
 
 /* The prologue of a march starting at plane x0: phi, cos/sin theta_x and
@@ -356,6 +374,7 @@ __device__ __forceinline__ void march_prologue(Get G, int xm0, int x0, MarchCarr
   m.PIm = G(1, xm0, 0, 0);
   const R X = G(2, xm0, 0, 0), Y = G(3, xm0, 0, 0), Z = G(4, xm0, 0, 0);
   T::sc(X, &m.sxm, &m.cxm);
+  m.TXm = X;
   const R ayx = Y + G(2, xm0, 1, 0) - G(3, x0, 0, 0) - X;
   const R azx = Z + G(2, xm0, 0, 1) - G(4, x0, 0, 0) - X;
   m.Qyx_m = T::sn(ayx);
@@ -380,10 +399,34 @@ typedef __attribute__((address_space(3))) void lds_void;
  * current theta_z at x+1), (current phi_r, phi_i at x+1), (current theta_x,
  * theta_y at x+1) each lie within 4 GiB of their lower plane, whose buffer
  * resource the pair's load uses.  Measured at 512^3: 3.747 -> 3.713 ms. */
-template <typename R, bool POW2, int PLAIN, bool BIG = false, bool PF = false>
+/* MODE (FP64, periodic): 0 the step alone; 1 the step also stores the
+ * divergence of its current level's links (FusedDiag::div), which the next
+ * step's diagnostics need for level n-1 once that level is overwritten; 2
+ * the step also takes the diagnostics of its current level n -- energy
+ * density, Gauss residual (from the stored div of level n-1), finiteness,
+ * forward edge phases -- with the arithmetic of the separate diagnostics
+ * pass (ax_energy.h), from the link trigonometry the update computes
+ * anyway.  P2D: dx, dy, dz and g are powers of two (divisions by them as
+ * multiplications by the exact reciprocals). */
+struct FusedDiag {
+  double *div;     /* [V] divergence of the links of a level */
+  double *phase;   /* [3][V] forward edge phases */
+  double *part;    /* [3] per work item: energy, sum |G|, non-finite count */
+  EnergyConsts ek;
+};
+
+__device__ __forceinline__ double march_wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+template <typename R, bool POW2, int PLAIN, bool BIG = false, bool PF = false, int MODE = 0, bool P2D = false>
 __device__ __forceinline__ void ah_march(const MarchConsts<R> &k, const Levels<R> &L, int NX, int NY, int NZ,
-                                         int y, int zc, int x0, int x1, bool zfix = false) {
+                                         int y, int zc, int x0, int x1, bool zfix = false,
+                                         FusedDiag fd = FusedDiag{}, int item = 0) {
   static_assert(!PF || (sizeof(R) == 8 && !BIG), "the prefetch path is FP64 with 32-bit plane offsets");
+  static_assert(MODE == 0 || (sizeof(R) == 8 && !BIG && !PF), "the fused diagnostics are FP64");
   const int lane = (int)(threadIdx.x & 63);
   const int zr = zc * 64 + lane;
   const bool inz = zr < NZ;
@@ -530,6 +573,7 @@ __device__ __forceinline__ void ah_march(const MarchConsts<R> &k, const Levels<R
     TrigR<R>::sc(TZe, &e.s, &e.c);
     e.qxz = TrigR<R>::sn(TXe + TZxe - TX0 - TZe);   /* a_xz at (xj, y, z0 - 1) */
     e.qyz = TrigR<R>::sn(TYe + TZye - TY0 - TZe);   /* a_yz there */
+    e.tz = TZe;
     __syncthreads();   /* the previous block's values have been read */
     if (lane < xe - xb) zedge[lane] = e;
     __syncthreads();
@@ -563,6 +607,7 @@ __device__ __forceinline__ void ah_march(const MarchConsts<R> &k, const Levels<R
     __builtin_amdgcn_raw_ptr_buffer_load_lds(lc23 ? c[2] : c[3], (lds_void *)&pf[buf][4][0], 16, d4 + sn_, 0, 0, 0);
   };
   if constexpr (PF) prefetch(x0, x0 & 1);
+  double rho_sum = 0, g_sum = 0, bad = 0;   /* MODE 2 */
 
   for (int x = x0; x < x1; ++x) {
     const int i = (x - x0) % EB;
@@ -586,7 +631,41 @@ __device__ __forceinline__ void ah_march(const MarchConsts<R> &k, const Levels<R
       prefetch(x + 1 < x1 ? x + 1 : x, (x + 1) & 1);
     }
     SiteOut<R> out;
-    ah_site_update<R, POW2>(k, plain, RegOps<R>{D, PRc, PIc, TXc, TYc, TZc}, m, ze, out);
+    if constexpr (MODE == 0) {
+      ah_site_update<R, POW2>(k, plain, RegOps<R>{D, PRc, PIc, TXc, TYc, TZc}, m, ze, out);
+    } else {
+      const MarchCarry<R> mb = m;   /* the carry from s-x, before the update advances it */
+      SiteTrig<R> tr;
+      ah_site_update<R, POW2, RegOps<R>, true>(k, plain, RegOps<R>{D, PRc, PIc, TXc, TYc, TZc}, m, ze, out, &tr);
+      const size_t si = ((size_t)x * NY + y) * NZ + z;
+      const double TZzm = lane_prev((double)TZc, (double)ze.tz);   /* theta_z(s-z): lane - 1's own */
+      const double divN = link_div<POW2>(fd.ek, TXc, mb.TXm, TYc, D.TYym, TZc, TZzm);
+      if constexpr (MODE == 1) {
+        if (act) fd.div[si] = divN;
+      } else {
+        const double divP = fd.div[si];
+        double cxy, cxz, cyz;
+        plaquette_cos(tr.a_xy, tr.a_xz, tr.a_yz, cxy, cxz, cyz);
+        double rho, G;
+        energy_gauss<P2D, false>(fd.ek, PRc, PIc, D.oPR, D.oPI, TXc, TYc, TZc, D.oX, D.oY, D.oZ, mb.PRm, mb.PIm,
+                                 mb.cxm, mb.sxm, D.PRym, D.PIym, tr.cym, tr.sym, D.PRzm, D.PIzm, tr.czm, tr.szm,
+                                 divN, divP, cxy, cxz, cyz, rho, G);
+        const double dX = link_phase_sc(PRc, PIc, D.PRn, D.PIn, tr.sx, tr.cx);
+        const double dY = link_phase_sc(PRc, PIc, D.PRyp, D.PIyp, tr.sy, tr.cy);
+        const double dZ = link_phase_sc(PRc, PIc, D.PRzp, D.PIzp, tr.sz, tr.cz);
+        if (act) {
+          const size_t V = (size_t)NX * NY * NZ;
+          fd.phase[si] = dX;
+          fd.phase[V + si] = dY;
+          fd.phase[2 * V + si] = dZ;
+          rho_sum += rho;
+          g_sum += fabs(G);
+          const double vals[10] = {PRc, PIc, TXc, TYc, TZc, D.oPR, D.oPI, D.oX, D.oY, D.oZ};
+#pragma unroll
+          for (int q = 0; q < 10; ++q) bad += isfinite(vals[q]) ? 0.0 : 1.0;
+        }
+      }
+    }
     if (act) {
       SP(2, x, R0, oz, out.X);
       SP(3, x, R0, oz, out.Y);
@@ -595,6 +674,14 @@ __device__ __forceinline__ void ah_march(const MarchConsts<R> &k, const Levels<R
       SP(1, x, R0, oz, out.PI);
     }
     PRc = D.PRn; PIc = D.PIn; TXc = D.TXn; TYc = D.TYn; TZc = D.TZn;
+  }
+  if constexpr (MODE == 2) {
+    const double a = march_wave_sum(rho_sum), b = march_wave_sum(g_sum), c = march_wave_sum(bad);
+    if (lane == 0) {
+      fd.part[3 * (size_t)item] = a;
+      fd.part[3 * (size_t)item + 1] = b;
+      fd.part[3 * (size_t)item + 2] = c;
+    }
   }
 }
 

@@ -165,6 +165,42 @@ hipError_t launch_step_tiled_ah(const StepConsts &k, const Levels<R> &L, int xse
   return hipGetLastError();
 }
 
+/* The FP64 periodic Abelian-Higgs step with its current level's divergence
+ * stored (MODE 1) or its current level's diagnostics taken (MODE 2), see
+ * ah_march.  One partial-sum triple per work item (MODE 2). */
+template <bool POW2, bool P2D, int MODE>
+__global__ void __launch_bounds__(64) ah_fused_kernel(MarchConsts<double> k, Levels<double> L, int nx, int ny, int nz,
+                                                      TileGrid g, FusedDiag fd) {
+  const int w = (int)blockIdx.x;
+  if (w >= g.work) return;
+  int zc, y, x0, x1;
+  if (!tile_range(g, nx, ny, 0, w, zc, y, x0, x1)) return;
+  ah_march<double, POW2, 2, false, false, MODE, P2D>(k, L, nx, ny, nz, y, zc, x0, x1, false, fd, w);
+}
+
+int fused_work_items(int nx, int ny, int nz, int xseg) { return make_tile_grid(nx, ny, nz, xseg).work; }
+
+hipError_t launch_step_fused(const StepConsts &k, const Levels<double> &L, int xseg, hipStream_t st, int mode,
+                             bool p2d, const FusedDiag &fd) {
+  const TileGrid g = make_tile_grid(k.nx, k.ny, k.nz, xseg);
+  MarchConsts<double> m;
+  march_consts_from<double>(k, rate_limit<double>(k.dt), m);
+  const bool pow2 = is_pow2(k.dx2) && is_pow2(k.dy2) && is_pow2(k.dz2);
+  dim3 grid(g.work), block(64);
+#define AX_FL(P2, PD, MD) \
+  hipLaunchKernelGGL((ah_fused_kernel<P2, PD, MD>), grid, block, 0, st, m, L, k.nx, k.ny, k.nz, g, fd)
+  if (mode == 1) {
+    if (pow2) AX_FL(true, false, 1);
+    else AX_FL(false, false, 1);
+  } else {
+    if (pow2 && p2d) AX_FL(true, true, 2);
+    else if (pow2) AX_FL(true, false, 2);
+    else AX_FL(false, false, 2);
+  }
+#undef AX_FL
+  return hipGetLastError();
+}
+
 template hipError_t launch_step_tiled_ah<double>(const StepConsts &, const Levels<double> &, int, hipStream_t);
 template hipError_t launch_step_tiled_ah<float>(const StepConsts &, const Levels<float> &, int, hipStream_t);
 

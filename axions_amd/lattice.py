@@ -46,6 +46,7 @@ def _load():
         "ax_step": (ctypes.c_int, [vp, i32]),
         "ax_sync": (ctypes.c_int, [vp]),
         "ax_diagnostics": (ctypes.c_int, [vp, i32, ctypes.POINTER(AxDiag)]),
+        "ax_step_diag": (ctypes.c_int, [vp, i32, i32, i32, vp, sz, ctypes.POINTER(sz)]),
         "ax_winding": (ctypes.c_int, [vp, vp, sz]),
         "ax_winding_field": (ctypes.c_int, [vp, i32, vp, sz]),
         "ax_crossings": (ctypes.c_int, [vp, vp, sz, ctypes.POINTER(sz)]),
@@ -213,6 +214,17 @@ class Lattice:
         d = AxDiag()
         _check(lib.ax_diagnostics(self._h, 1 if winding else 0, ctypes.byref(d)))
         return {f: getattr(d, f) for f, _ in d._fields_}
+
+    def step_diag(self, n, every, winding=True):
+        """n TimeSteps with the diagnostics of every TimeStep t % every == 0
+        taken before its step (fused into the step where the library can):
+        a list of diagnostics dicts."""
+        cap = n // max(every, 1) + 2
+        arr = (AxDiag * cap)()
+        cnt = ctypes.c_size_t()
+        _check(lib.ax_step_diag(self._h, n, every, 1 if winding else 0, ctypes.cast(arr, ctypes.c_void_p), cap,
+                                ctypes.byref(cnt)))
+        return [{f: getattr(arr[i], f) for f, _ in AxDiag._fields_} for i in range(cnt.value)]
 
     def winding(self, field=0):
         p = self.params

@@ -203,7 +203,8 @@ def copy_calibration(nbytes, reps):
     print(json.dumps({"copy_calibration_bytes": nbytes, "reps": reps}))
 
 
-def run_lattice(ax, n, precision, steps, warmup, diag_every, seed, log, moore=False, moore_dt=0.3, series_to=0):
+def run_lattice(ax, n, precision, steps, warmup, diag_every, seed, log, moore=False, moore_dt=0.3, series_to=0,
+                fused=True):
     """The headline loop on one lattice: warmup, then `steps` timed steps with
     diagnostics every diag_every steps, then (untimed) on to TimeStep
     series_to if that is further; returns throughput, the step kernel's
@@ -221,6 +222,15 @@ def run_lattice(ax, n, precision, steps, warmup, diag_every, seed, log, moore=Fa
             series.append((d["timestep"], d["energy"], d["gauss_mean"], d["wound_plaquettes"], d["xi"], d["finite"]))
 
         def run(nsteps):
+            if fused:
+                # the diagnostics of every diag_every-th TimeStep taken by its
+                # step (ax_step_diag; the separate pass where that does not apply)
+                for d in lat.step_diag(nsteps, diag_every, winding=True):
+                    if not d["finite"]:
+                        raise RuntimeError(f"non-finite state at step {d['timestep']}")
+                    series.append((d["timestep"], d["energy"], d["gauss_mean"], d["wound_plaquettes"], d["xi"],
+                                   d["finite"]))
+                return
             done = 0
             while done < nsteps:
                 k = min(diag_every - (lat.timestep % diag_every), nsteps - done)
@@ -513,6 +523,8 @@ def main():
                     help="the FP64 and FP32 runs continue (untimed) to this TimeStep for their diagnostic "
                          "series and the FP32-vs-FP64 comparison")
     ap.add_argument("--seed", type=int, default=42)
+    ap.add_argument("--fused-diag", choices=("on", "off"), default="on",
+                    help="diagnostics taken by the step kernels (ax_step_diag) or by the separate pass")
     ap.add_argument("--cpu-baseline", choices=("on", "off"), default="on")
     ap.add_argument("--fp32", choices=("on", "off"), default="on", help="FP32 run of the same workload")
     ap.add_argument("--moore", choices=("on", "off"), default="on", help="Moore-model 256^3 run")
@@ -548,11 +560,13 @@ def main():
         sys.exit(2)
 
     import axions_amd as ax
+    fused = a.fused_diag == "on"
     main_run = run_lattice(ax, a.n, a.precision, a.steps, a.warmup, a.diag_every, a.seed, log,
-                           series_to=a.series_steps)
+                           series_to=a.series_steps, fused=fused)
     fp32 = None
     if a.fp32 == "on" and a.precision == 64:
-        r32 = run_lattice(ax, a.n, 32, a.steps, a.warmup, a.diag_every, a.seed, log, series_to=a.series_steps)
+        r32 = run_lattice(ax, a.n, 32, a.steps, a.warmup, a.diag_every, a.seed, log, series_to=a.series_steps,
+                          fused=fused)
         fp32 = {"value": r32["value"], "unit": "site-updates/s", "ms_per_step": r32["ms_per_step"],
                 "roofline": r32["roofline"], "vs_fp64": compare_series(main_run["series"], r32["series"])}
     moore = None
@@ -592,7 +606,8 @@ def main():
         "data": f"synthetic: seeded random-phase IC (std::mt19937 seed {a.seed}), pi=0, A=E=0",
         "config": {"workload": f"Abelian-Higgs leapfrog {a.n}^3 periodic FP{a.precision}, dx=0.5 dt=0.1 "
                                f"lambda=2 g=eta=1, energy+Gauss+winding every {a.diag_every} steps",
-                   "n": a.n, "precision": a.precision, "diag_every": a.diag_every},
+                   "n": a.n, "precision": a.precision, "diag_every": a.diag_every,
+                   "diagnostics": "fused into the step" if fused else "separate pass"},
         "roofline": main_run["roofline"],
         "cpu_baseline": cpu,
         "fp32": fp32,

@@ -183,6 +183,20 @@ int ax_kernel_time(ax_lattice *lat, double *ms, int64_t *steps);
  * Synchronous.  want_winding = 0 skips the plaquette pass. */
 int ax_diagnostics(ax_lattice *lat, int32_t want_winding, ax_diag *out);
 
+/* nsteps TimeSteps as ax_step, taking at every TimeStep t of the call with
+ * t % every == 0 the diagnostics ax_diagnostics would return just before
+ * that step (the reference's per-step energy/Gauss sums, evolution.cpp:
+ * 386-427, 1404-1408, at its output cadence): out[k] for the k-th such t,
+ * at most cap of them, *count = how many.  On FP64 periodic Abelian-Higgs
+ * lattices (evolution.cpp energy form, g != 0) the step at t computes them
+ * from the values and link trigonometry it uses for its own update, with
+ * the arithmetic of the separate pass, the step before it storing the
+ * divergence of its links; other lattices take the separate pass.  Same
+ * fields, the same energy and Gauss sums to 1e-13 and the same windings as
+ * ax_diagnostics + ax_step.  Waits for the device at the end. */
+int ax_step_diag(ax_lattice *lat, int32_t nsteps, int32_t every, int32_t want_winding, ax_diag *out, size_t cap,
+                 size_t *count);
+
 /* Per-plaquette winding numbers of the current state, 3*nx*ny*nz int8 in
  * [plane][x][y][z] order, plane 0 = xy, 1 = xz, 2 = yz (plaquette spanned
  * from site s along the two axes).  No counterpart in the reference; see
