@@ -581,15 +581,18 @@ __device__ __forceinline__ void ah_march(const MarchConsts<R> &k, const Levels<R
 
   __shared__ double pf[PF ? 2 : 1][5][128];
   /* per lane: the byte distance of its field of each pair from the pair's
-   * lower plane (lanes 32-63 take the second field) */
+   * lower plane (lanes 32-63 take the second field).  Within a level phi_r
+   * lies below phi_i and theta_x below theta_y (the [comp][level] plane
+   * order; the launcher checks it), so those pairs use the resources of
+   * phi_r and theta_x directly -- resources chosen at run time are held in
+   * scalar registers of their own, which the loop then spills */
   const bool hi = lane >= 32;
   const uint32_t lz = (uint32_t)(zc * 64) * 8u + 16u * (uint32_t)(lane & 31);
   auto pdelta = [&](const void *F, const void *G) -> uint32_t {
     const char *pF = (const char *)F, *pG = (const char *)G;
     return pF < pG ? (hi ? (uint32_t)(pG - pF) : 0u) : (hi ? 0u : (uint32_t)(pF - pG));
   };
-  const bool lo01 = L.prv[0] < L.prv[1], lo23 = L.prv[2] < L.prv[3], lo4 = L.prv[4] < L.cur[4];
-  const bool lc01 = L.cur[0] < L.cur[1], lc23 = L.cur[2] < L.cur[3];
+  const bool lo4 = L.prv[4] < L.cur[4];
   const uint32_t d0 = pdelta(L.prv[0], L.prv[1]) + lz, d1 = pdelta(L.prv[2], L.prv[3]) + lz;
   const uint32_t d2 = pdelta(L.prv[4], L.cur[4]) + lz, d3 = pdelta(L.cur[0], L.cur[1]) + lz;
   const uint32_t d4 = pdelta(L.cur[2], L.cur[3]) + lz;
@@ -598,16 +601,30 @@ __device__ __forceinline__ void ah_march(const MarchConsts<R> &k, const Levels<R
     const int xp = x == NX - 1 ? 0 : x + 1;
     const uint32_t so_ = (uint32_t)x * PLB + R0, sn_ = (uint32_t)xp * PLB + R0;
     const uint32_t s2 = hi ? sn_ : so_;   /* pair 2: plane x (previous level), x+1 (current) */
-    __builtin_amdgcn_raw_ptr_buffer_load_lds(lo01 ? o[0] : o[1], (lds_void *)&pf[buf][0][0], 16, d0 + so_, 0, 0,
-                                             kStream);
-    __builtin_amdgcn_raw_ptr_buffer_load_lds(lo23 ? o[2] : o[3], (lds_void *)&pf[buf][1][0], 16, d1 + so_, 0, 0,
-                                             kStream);
+    __builtin_amdgcn_raw_ptr_buffer_load_lds(o[0], (lds_void *)&pf[buf][0][0], 16, d0 + so_, 0, 0, kStream);
+    __builtin_amdgcn_raw_ptr_buffer_load_lds(o[2], (lds_void *)&pf[buf][1][0], 16, d1 + so_, 0, 0, kStream);
     __builtin_amdgcn_raw_ptr_buffer_load_lds(lo4 ? o[4] : c[4], (lds_void *)&pf[buf][2][0], 16, d2 + s2, 0, 0, 0);
-    __builtin_amdgcn_raw_ptr_buffer_load_lds(lc01 ? c[0] : c[1], (lds_void *)&pf[buf][3][0], 16, d3 + sn_, 0, 0, 0);
-    __builtin_amdgcn_raw_ptr_buffer_load_lds(lc23 ? c[2] : c[3], (lds_void *)&pf[buf][4][0], 16, d4 + sn_, 0, 0, 0);
+    __builtin_amdgcn_raw_ptr_buffer_load_lds(c[0], (lds_void *)&pf[buf][3][0], 16, d3 + sn_, 0, 0, 0);
+    __builtin_amdgcn_raw_ptr_buffer_load_lds(c[2], (lds_void *)&pf[buf][4][0], 16, d4 + sn_, 0, 0, 0);
   };
   if constexpr (PF) prefetch(x0, x0 & 1);
   double rho_sum = 0, g_sum = 0, bad = 0;   /* MODE 2 */
+  /* DEFER (the FP64 step alone): plane x's five stores are issued at the top
+   * of plane x+1 (after the prefetched values, where there are any, are
+   * read).  Issued at the end of plane x they sit in the memory counter
+   * behind the prefetch, and the wait for the prefetched values (which the
+   * compiler makes a full drain) also waits for them.  It also shortens the live ranges the compiler
+   * keeps across the loop: 64-bit plane addressing 170 -> 160 VGPRs, 3
+   * waves per SIMD instead of 2.  FP32 is slower with it (+0.5 %). */
+  constexpr bool DEFER = sizeof(R) == 8 && MODE == 0;
+  SiteOut<R> pend{};
+  auto store_out = [&](int x, const SiteOut<R> &v) {
+    SP(2, x, R0, oz, v.X);
+    SP(3, x, R0, oz, v.Y);
+    SP(4, x, R0, oz, v.Z);
+    SP(0, x, R0, oz, v.PR);
+    SP(1, x, R0, oz, v.PI);
+  };
 
   for (int x = x0; x < x1; ++x) {
     const int i = (x - x0) % EB;
@@ -621,6 +638,9 @@ __device__ __forceinline__ void ah_march(const MarchConsts<R> &k, const Levels<R
       D.oZ = pf[buf][2][lane]; D.TZn = pf[buf][2][64 + lane];
       D.PRn = pf[buf][3][lane]; D.PIn = pf[buf][3][64 + lane];
       D.TXn = pf[buf][4][lane]; D.TYn = pf[buf][4][64 + lane];
+    }
+    if constexpr (DEFER) {
+      if (x > x0 && act) store_out(x - 1, pend);
     }
     load_plane(x, D, PRc, PIc, TXc, TYc, TZc);
     if constexpr (PF) {
@@ -666,14 +686,12 @@ __device__ __forceinline__ void ah_march(const MarchConsts<R> &k, const Levels<R
         }
       }
     }
-    if (act) {
-      SP(2, x, R0, oz, out.X);
-      SP(3, x, R0, oz, out.Y);
-      SP(4, x, R0, oz, out.Z);
-      SP(0, x, R0, oz, out.PR);
-      SP(1, x, R0, oz, out.PI);
-    }
+    if constexpr (DEFER) pend = out;
+    else if (act) store_out(x, out);
     PRc = D.PRn; PIc = D.PIn; TXc = D.TXn; TYc = D.TYn; TZc = D.TZn;
+  }
+  if constexpr (DEFER) {
+    if (x1 > x0 && act) store_out(x1 - 1, pend);
   }
   if constexpr (MODE == 2) {
     const double a = march_wave_sum(rho_sum), b = march_wave_sum(g_sum), c = march_wave_sum(bad);

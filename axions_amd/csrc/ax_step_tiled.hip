@@ -142,8 +142,10 @@ hipError_t launch_step_tiled_ah(const StepConsts &k, const Levels<R> &L, int xse
   /* FP64 plain runs with power-of-two spacings (the variant that keeps 3
    * waves per SIMD with it: 166 VGPRs; the others reach 169-179 and drop to
    * 2), whole chunks and three planes within 32-bit offsets: the previous
-   * level and the next plane are prefetched into LDS (ah_march) */
-  const bool pf = sizeof(R) == 8 && !big && pow2 && m.plain && k.nz % 64 == 0 &&
+   * level and the next plane are prefetched into LDS (ah_march; its phi and
+   * theta_x/theta_y pairs rely on the [comp][level] plane order) */
+  const bool order = L.cur[0] < L.cur[1] && L.cur[2] < L.cur[3] && L.prv[0] < L.prv[1] && L.prv[2] < L.prv[3];
+  const bool pf = sizeof(R) == 8 && !big && pow2 && m.plain && k.nz % 64 == 0 && order &&
                   (size_t)k.nx * k.ny * k.nz * sizeof(R) * 3 + ((size_t)1 << 20) < ((size_t)1 << 32) &&
                   getenv("AX_NO_PREFETCH") == nullptr;
   if (pf) {
