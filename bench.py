@@ -36,6 +36,18 @@ Prints ONE JSON line:
                 hipGraph chunks); realizations/hour, site-updates/s, p50/p99
                 latency; the same stream run one by one through the FP64
                 single-lattice path for comparison (realizations/hour and xi)
+                and a second load point offered at 0.7x the pool's capacity;
+                the pool's step kernel against the same roofline
+  boundaries    evolution.cpp's own schedule (absorbing x/y faces, PRS
+                friction, damping window) at 512^3
+  sor_profile   the straight-string SOR profile on the GPU and the oracle's
+                one-thread loop beside it
+
+Two auxiliary modes print their own line instead: --copy-calibration (HIP
+runtime device-to-device copies of 1 GiB, run under rocprofv3 --pmc to turn
+FETCH_SIZE / WRITE_SIZE into bytes) and --stream-probe (what plain streaming
+kernels -- torch's sum, copy and add over 1 GiB FP64 tensors -- reach on this
+GPU's HBM).
 """
 import argparse
 import json
@@ -508,9 +520,40 @@ def ensemble(ax, count, rate, seed, budget_gb, chunk, fp64_compare, log, light_f
     return out
 
 
+def stream_probe(nbytes, reps):
+    """What plain streaming kernels reach on this GPU's HBM: torch's own
+    sum (read only), copy and add (two reads, one write: the step's ratio of
+    read to written bytes) over FP64 tensors of nbytes each, best of reps
+    runs timed with events on the current stream; one JSON line of GB/s."""
+    import torch
+    n = nbytes // 8
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.ones(n, dtype=torch.float64, device="cuda")
+    c = torch.empty(n, dtype=torch.float64, device="cuda")
+    acc = torch.empty((), dtype=torch.float64, device="cuda")
+    kinds = {"read": (lambda: torch.sum(a, 0, out=acc), 1), "copy": (lambda: c.copy_(a), 2),
+             "add_2r1w": (lambda: torch.add(a, b, out=c), 3)}
+    out = {"bytes_per_tensor": nbytes}
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    for name, (fn, moved) in kinds.items():
+        fn()
+        best = float("inf")
+        for _ in range(reps):
+            e0.record()
+            fn()
+            e1.record()
+            e1.synchronize()
+            best = min(best, e0.elapsed_time(e1))
+        out[name + "_GBps"] = moved * nbytes / (best * 1e-3) / 1e9
+    print(json.dumps(out))
+
+
 def main():
     if "--copy-calibration" in sys.argv:
         copy_calibration(1 << 30, 3)
+        return
+    if "--stream-probe" in sys.argv:
+        stream_probe(1 << 30, 10)
         return
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
