@@ -718,19 +718,27 @@ void ax_pool::ic_worker() {
 }
 
 int ax_pool::tune(int n, int *xseg, double *ms) {
-  /* scratch lattice inside the arena, random-phase IC, each candidate timed
-   * with the batched kernel over a one-lattice table on its own stream */
-  int64_t off;
+  /* m scratch lattices of edge n (about 2^24 sites in all, as many as the
+   * arena has room for), random-phase IC, each candidate timed with the
+   * batched kernel over an m-lattice table on its own stream.  The pool
+   * serves many lattices at once: timed alone, a small lattice needs short x
+   * segments to fill the GPU (64^3 chose 4 planes per wave), while among many
+   * resident lattices longer segments pay the march's prologue less often. */
+  constexpr size_t kTuneSites = (size_t)1 << 24;
+  const size_t V = (size_t)n * n * n;
+  const size_t LB = lattice_bytes(n, esize, pad_bytes);
+  int m = (int)std::max<size_t>(1, std::min<size_t>(64, kTuneSites / V));
+  int64_t off = -1;
   {
     std::lock_guard<std::mutex> g(mu);
-    off = arena.alloc(lattice_bytes(n, esize, pad_bytes));
+    while (m > 1 && (off = arena.alloc(LB * (size_t)m)) < 0) m /= 2;
+    if (off < 0) off = arena.alloc(LB);   /* m == 1 */
   }
   if (off < 0) return fail(AX_EBUDGET, "no room to tune n=%d", n);
   hipStream_t s = nullptr;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   BatchHeader *dh = nullptr;
   int rc = AX_OK;
-  const size_t V = (size_t)n * n * n;
   const size_t pitch = lattice_pitch(V, esize, pad_bytes);
   char *base = arena_base + off;
   std::vector<char> ic(2 * V * esize);
@@ -741,32 +749,41 @@ int ax_pool::tune(int n, int *xseg, double *ms) {
     if (c <= n && std::find(cand.begin(), cand.end(), c) == cand.end()) cand.push_back(c);
   double best = 1e300;
   int bestx = cfg.xseg_default;
+  const size_t tab_bytes = 64 + (size_t)m * sizeof(BatchEntry);
   hipError_t e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipEventCreate(&e0);
   if (e == hipSuccess) e = hipEventCreate(&e1);
-  if (e == hipSuccess) e = hipMalloc((void **)&dh, 64 + sizeof(BatchEntry));
+  if (e == hipSuccess) e = hipMalloc((void **)&dh, tab_bytes);
   const size_t plane = V * esize, pb = pitch * esize;
   if (e == hipSuccess) e = hipMemcpyAsync(base, ic.data(), plane, hipMemcpyHostToDevice, s);
   if (e == hipSuccess) e = hipMemcpyAsync(base + 2 * pb, ic.data() + plane, plane, hipMemcpyHostToDevice, s);
   if (e == hipSuccess) e = hipMemcpyAsync(base + pb, base, plane, hipMemcpyDeviceToDevice, s);
   if (e == hipSuccess) e = hipMemcpyAsync(base + 3 * pb, base + 2 * pb, plane, hipMemcpyDeviceToDevice, s);
   if (e == hipSuccess) e = hipMemsetAsync(base + 4 * pb, 0, 6 * pb, s);
+  /* the other lattices start from the same state (ten planes) */
+  for (int k = 1; k < m && e == hipSuccess; ++k)
+    e = hipMemcpyAsync(base + (size_t)k * LB, base, 10 * pb, hipMemcpyDeviceToDevice, s);
+  std::vector<char> tab(tab_bytes);
   const int reps = 2, warm = 2, timed = 6;
   for (size_t ci = 0; ci < cand.size() && e == hipSuccess; ++ci) {
-    char tab[64 + sizeof(BatchEntry)];
-    memset(tab, 0, sizeof tab);
-    BatchHeader *h = (BatchHeader *)tab;
-    BatchEntry *E = (BatchEntry *)(tab + 64);
-    E->base = (uint64_t)(uintptr_t)base;
-    E->pitch = pitch;
-    E->n = n;
-    E->nw = (n + 63) / 64;
-    E->xseg = cand[ci];
-    E->nseg = (n + E->xseg - 1) / E->xseg;
-    E->steps = 1 << 30;
-    h->count = 1;
-    h->work = E->nw * n * E->nseg;
-    e = hipMemcpyAsync(dh, tab, sizeof tab, hipMemcpyHostToDevice, s);
+    std::fill(tab.begin(), tab.end(), 0);
+    BatchHeader *h = (BatchHeader *)tab.data();
+    BatchEntry *ent = (BatchEntry *)(tab.data() + 64);
+    const int nw = (n + 63) / 64, nseg = (n + cand[ci] - 1) / cand[ci];
+    for (int k = 0; k < m; ++k) {
+      BatchEntry *E = ent + k;
+      E->base = (uint64_t)(uintptr_t)(base + (size_t)k * LB);
+      E->pitch = pitch;
+      E->n = n;
+      E->nw = nw;
+      E->xseg = cand[ci];
+      E->nseg = nseg;
+      E->work_begin = k * nw * n * nseg;
+      E->steps = 1 << 30;
+    }
+    h->count = m;
+    h->work = m * nw * n * nseg;
+    e = hipMemcpyAsync(dh, tab.data(), tab_bytes, hipMemcpyHostToDevice, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     for (int rep = 0; rep < reps && e == hipSuccess; ++rep) {
       for (int j = 0; j < warm + timed; ++j) {
