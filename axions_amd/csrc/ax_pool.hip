@@ -107,11 +107,22 @@ template <typename R, bool POW2>
 __global__ void __launch_bounds__(64) pool_step_kernel(ax_params p, R lim, const BatchHeader *__restrict__ hdr,
                                                        const BatchEntry *__restrict__ ent, int j) {
   const int count = hdr->count, work = hdr->work;
-  for (int w = (int)blockIdx.x; w < work; w += (int)gridDim.x) {
-    const int e = find_entry(ent, count, w, true);
+  /* XCD-aware order: the grid is resident at once and its workgroups are
+   * handed to the eight XCDs round-robin, so workgroup b strides through
+   * the w of one residue mod 8 on XCD b mod 8 (the grid is a multiple of
+   * 8).  Item it = (w mod 8) * q + w / 8 gives XCD k the k-th eighth of the
+   * items -- consecutive chunks and rows of a lattice's segment, whose shared
+   * neighbour rows and chunk-edge lines then sit in one L2 -- instead of
+   * every eighth item.  The map is a bijection of [0, 8q), so every item is
+   * stepped once whatever the grid. */
+  const int q = (work + 7) >> 3;
+  for (int w = (int)blockIdx.x; w < 8 * q; w += (int)gridDim.x) {
+    const int it = (w & 7) * q + (w >> 3);
+    if (it >= work) continue;
+    const int e = find_entry(ent, count, it, true);
     const BatchEntry E = ent[e];
     if (j >= E.steps) continue;
-    const int local = w - E.work_begin;
+    const int local = it - E.work_begin;
     const int zc = local % E.nw, r = local / E.nw;
     const int y = r % E.n, seg = r / E.n;
     const int x0 = seg * E.xseg, x1 = min(E.n, x0 + E.xseg);
@@ -553,6 +564,26 @@ int ax_pool::enqueue_chunk() {
     ++count;
   }
   if (count == 0) return AX_OK;
+  /* The tuned segments are chosen for a full pool.  With few lattices
+   * resident they leave workgroups of the persistent grid idle: halve the
+   * longest segments until the chunk has two items per workgroup or every
+   * segment is 4 planes (the partition of a lattice does not change its
+   * results). */
+  while (work < 2 * grid_step) {
+    int longest = 0;
+    for (int k = 0; k < count; ++k) longest = std::max(longest, (int)ent[k].xseg);
+    if (longest <= 4) break;
+    work = 0;
+    for (int k = 0; k < count; ++k) {
+      BatchEntry &E = ent[k];
+      if (E.xseg == longest) {
+        E.xseg = std::max(4, longest / 2);
+        E.nseg = (E.n + E.xseg - 1) / E.xseg;
+      }
+      E.work_begin = work;
+      work += E.nw * E.n * E.nseg;
+    }
+  }
   h->count = count;
   h->work = work;
   h->wblocks = wblk;
@@ -718,16 +749,18 @@ void ax_pool::ic_worker() {
 }
 
 int ax_pool::tune(int n, int *xseg, double *ms) {
-  /* m scratch lattices of edge n (about 2^24 sites in all, as many as the
-   * arena has room for), random-phase IC, each candidate timed with the
-   * batched kernel over an m-lattice table on its own stream.  The pool
+  /* m scratch lattices of edge n, random-phase IC, each candidate timed with
+   * the batched kernel over an m-lattice table on its own stream.  The pool
    * serves many lattices at once: timed alone, a small lattice needs short x
    * segments to fill the GPU (64^3 chose 4 planes per wave), while among many
-   * resident lattices longer segments pay the march's prologue less often. */
-  constexpr size_t kTuneSites = (size_t)1 << 24;
+   * resident lattices longer segments pay the march's prologue less often.
+   * So the table holds enough sites for four 32-plane work items per
+   * workgroup of the persistent grid (enqueue_chunk shortens the segments of
+   * a pool that holds fewer), as many lattices as the arena has room for. */
   const size_t V = (size_t)n * n * n;
   const size_t LB = lattice_bytes(n, esize, pad_bytes);
-  int m = (int)std::max<size_t>(1, std::min<size_t>(64, kTuneSites / V));
+  const size_t target = std::max<size_t>((size_t)1 << 24, (size_t)4 * grid_step * 64 * 32);
+  int m = (int)std::max<size_t>(1, std::min<size_t>(256, (target + V - 1) / V));
   int64_t off = -1;
   {
     std::lock_guard<std::mutex> g(mu);
@@ -744,9 +777,12 @@ int ax_pool::tune(int n, int *xseg, double *ms) {
   std::vector<char> ic(2 * V * esize);
   if (esize == 8) build_ic<double>(12345u, V, p.eta, (double *)ic.data(), (double *)ic.data() + V);
   else build_ic<float>(12345u, V, p.eta, (float *)ic.data(), (float *)ic.data() + V);
+  /* at most 32 planes per work item: a launch lasts as long as its longest
+   * item, so longer segments (which the table's timing can favour) hold the
+   * other lattices' workgroups idle at the end of each launch */
   std::vector<int> cand;
-  for (int c : {4, 8, 16, 32, 64, 128, n})
-    if (c <= n && std::find(cand.begin(), cand.end(), c) == cand.end()) cand.push_back(c);
+  for (int c : {4, 8, 16, 32, n})
+    if (c <= n && c <= 32 && std::find(cand.begin(), cand.end(), c) == cand.end()) cand.push_back(c);
   double best = 1e300;
   int bestx = cfg.xseg_default;
   const size_t tab_bytes = 64 + (size_t)m * sizeof(BatchEntry);
@@ -916,7 +952,7 @@ int ax_pool_create(const ax_params *tmpl, const ax_pool_config *cfg_in, ax_pool 
     e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&bp, P->esize == 8 ? (const void *)pool_phase_kernel<double>
                                                                          : (const void *)pool_phase_kernel<float>, 256, 0);
   if (e != hipSuccess) return cleanup(hip_fail(e, "occupancy"));
-  P->grid_step = std::max(1, ncu) * std::max(1, bs);
+  P->grid_step = (std::max(1, ncu) * std::max(1, bs) + 7) / 8 * 8;   /* a multiple of the 8 XCDs */
   P->grid_wind = std::max(1, ncu) * std::max(1, bw);
   P->grid_phase = std::max(1, ncu) * std::max(1, bp);
   if (cfg.use_graph) {

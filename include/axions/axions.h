@@ -362,9 +362,11 @@ int ax_pool_get_state(ax_pool *pool, int64_t id, double *host, size_t n);
 /* Forget a finished request (frees a kept state). */
 int ax_pool_release(ax_pool *pool, int64_t id);
 int ax_pool_stats_get(ax_pool *pool, ax_pool_stats *out);
-/* Autotuner: time each candidate tile x extent with the batched kernel on a
- * table of scratch n^3 lattices (about 2^24 sites in all, up to 64 lattices,
- * fewer when the arena lacks room: the pool serves many lattices at once) and
+/* Autotuner: time each candidate tile x extent (4, 8, 16 or 32 planes) with
+ * the batched kernel on a table of scratch n^3 lattices (enough sites for
+ * four 32-plane work items per workgroup of the persistent grid, up to 256
+ * lattices, fewer when the arena lacks room: the pool serves many lattices
+ * at once) and
  * keep the fastest for requests of that N.  Runs on its own stream (the
  * scheduler tunes a new N with its chunks drained; a direct call times
  * alongside whatever the pool is running); returns the chosen xseg and the
