@@ -613,9 +613,10 @@ __device__ __forceinline__ void ah_march(const MarchConsts<R> &k, const Levels<R
    * of plane x+1 (after the prefetched values, where there are any, are
    * read).  Issued at the end of plane x they sit in the memory counter
    * behind the prefetch, and the wait for the prefetched values (which the
-   * compiler makes a full drain) also waits for them.  It also shortens the live ranges the compiler
-   * keeps across the loop: 64-bit plane addressing 170 -> 160 VGPRs, 3
-   * waves per SIMD instead of 2.  FP32 is slower with it (+0.5 %). */
+   * compiler makes a full drain) also waits for them.  It also shortens the
+   * live ranges the compiler keeps across the loop: every FP64 variant fits
+   * 161 VGPRs with the prefetch, 64-bit plane addressing 170 -> 160, so all
+   * keep 3 waves per SIMD.  FP32 is slower with it (+0.5 %). */
   constexpr bool DEFER = sizeof(R) == 8 && MODE == 0;
   SiteOut<R> pend{};
   auto store_out = [&](int x, const SiteOut<R> &v) {

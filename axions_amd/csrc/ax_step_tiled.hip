@@ -139,19 +139,25 @@ hipError_t launch_step_tiled_ah(const StepConsts &k, const Levels<R> &L, int xse
   dim3 grid(g.work), block(64);
 #define AX_AHL(P2, PL, BG) \
   hipLaunchKernelGGL((ah_tiled_kernel<R, P2, PL, BG>), grid, block, 0, st, m, L, k.nx, k.ny, k.nz, g, k.ex, k.ez)
-  /* FP64 plain runs with power-of-two spacings (the variant that keeps 3
-   * waves per SIMD with it: 166 VGPRs; the others reach 169-179 and drop to
-   * 2), whole chunks and three planes within 32-bit offsets: the previous
-   * level and the next plane are prefetched into LDS (ah_march; its phi and
-   * theta_x/theta_y pairs rely on the [comp][level] plane order) */
+  /* FP64, whole chunks and three planes within 32-bit offsets: the
+   * previous level and the next plane are prefetched into LDS (ah_march; its
+   * phi and theta_x/theta_y pairs rely on the [comp][level] plane order).
+   * Every variant keeps 3 waves per SIMD with it (161 VGPRs, with the stores
+   * issued one plane late). */
   const bool order = L.cur[0] < L.cur[1] && L.cur[2] < L.cur[3] && L.prv[0] < L.prv[1] && L.prv[2] < L.prv[3];
-  const bool pf = sizeof(R) == 8 && !big && pow2 && m.plain && k.nz % 64 == 0 && order &&
+  const bool pf = sizeof(R) == 8 && !big && k.nz % 64 == 0 && order &&
                   (size_t)k.nx * k.ny * k.nz * sizeof(R) * 3 + ((size_t)1 << 20) < ((size_t)1 << 32) &&
                   getenv("AX_NO_PREFETCH") == nullptr;
+#define AX_AHPF(P2, PL)                                                                                     \
+  hipLaunchKernelGGL((ah_tiled_kernel<double, P2, PL, false, true>), grid, block, 0, st, m, L, k.nx, k.ny, k.nz, g, \
+                     k.ex, k.ez)
   if (pf) {
-    if constexpr (sizeof(R) == 8)
-      hipLaunchKernelGGL((ah_tiled_kernel<double, true, true, false, true>), grid, block, 0, st, m, L, k.nx, k.ny,
-                         k.nz, g, k.ex, k.ez);
+    if constexpr (sizeof(R) == 8) {
+      if (pow2 && m.plain) AX_AHPF(true, true);
+      else if (pow2) AX_AHPF(true, false);
+      else if (m.plain) AX_AHPF(false, true);
+      else AX_AHPF(false, false);
+    }
   } else if (big) {
     if (pow2 && m.plain) AX_AHL(true, true, true);
     else if (pow2) AX_AHL(true, false, true);
@@ -164,6 +170,7 @@ hipError_t launch_step_tiled_ah(const StepConsts &k, const Levels<R> &L, int xse
     else AX_AHL(false, false, false);
   }
 #undef AX_AHL
+#undef AX_AHPF
   return hipGetLastError();
 }
 

@@ -443,3 +443,45 @@ def test_prefetch_march_equals_plain_march(shape, xseg, monkeypatch):
             lat.step(11)
             outs.append(lat.get_state())
     assert bits_equal(outs[0], outs[1])
+
+
+PREFETCH_VARIANTS = {
+    # PRS friction and the damping window (theta damping, both forms)
+    "friction": lambda: ax.ah_params(16, ny=12, nz=128, alpha=3.0, damped_nt=6, dampFac=0.5),
+    "friction_form1": lambda: ax.ah_params(16, ny=12, nz=128, alpha=3.0, damped_nt=6, theta_damp_form=1),
+    # anisotropic spacings that are not powers of two: true divisions
+    "aniso": lambda: ax.ah_params(16, ny=12, nz=128, dx=0.7, dy=0.6, dz=0.55, dt=0.1),
+    # evolution.cpp's own schedule: absorbing x/y faces, g = 0, friction
+    "absorbing": lambda: ax.ah_params(18, ny=14, nz=64, g=0.0, lam=1.0, dt=0.05, alpha=3.0, damped_nt=4,
+                                      boundary=AX_BC_ABSORBING),
+    # Neumann x/y with fixed z: the march skips boundary planes and rows, the
+    # z-face lanes do not store
+    "neumann_zfixed": lambda: ax.ah_params(14, ny=10, nz=128, boundary=AX_BC_NEUMANN, boundary_z=AX_BC_FIXED),
+}
+
+
+@pytest.mark.parametrize("name", sorted(PREFETCH_VARIANTS))
+def test_prefetch_march_equals_plain_march_variants(name, monkeypatch):
+    """Every FP64 variant of the march takes the LDS prefetch when its chunks
+    are whole: friction products, true divisions and the non-periodic faces
+    give the bits of the march without the prefetch (itself bit-exact against
+    the oracle in the other tests), over segments of 3 planes and whole rows;
+    the prefetched run also matches the oracle directly."""
+    p = PREFETCH_VARIANTS[name]()
+    outs = []
+    for xseg in (3, 0):
+        for off in (False, True):
+            say(f"[prefetch] {name} xseg={xseg} prefetch={'off' if off else 'on'}")
+            if off:
+                monkeypatch.setenv("AX_NO_PREFETCH", "1")
+            else:
+                monkeypatch.delenv("AX_NO_PREFETCH", raising=False)
+            with ax.Lattice(p) as lat:
+                lat.set_tile(xseg)
+                lat.init_random_phase(5)
+                lat.step(9)
+                outs.append(lat.get_state())
+        assert bits_equal(outs[-2], outs[-1])
+    st = orc.init_random_phase(p, 5, 0)
+    orc.step(p, st, 0, 9, 0, NT)
+    assert bits_equal(outs[0], st)
