@@ -426,7 +426,7 @@ __device__ __forceinline__ void ah_march(const MarchConsts<R> &k, const Levels<R
                                          int y, int zc, int x0, int x1, bool zfix = false,
                                          FusedDiag fd = FusedDiag{}, int item = 0) {
   static_assert(!PF || (sizeof(R) == 8 && !BIG), "the prefetch path is FP64 with 32-bit plane offsets");
-  static_assert(MODE == 0 || (sizeof(R) == 8 && !BIG && !PF), "the fused diagnostics are FP64");
+  static_assert(MODE == 0 || (sizeof(R) == 8 && !BIG), "the fused diagnostics are FP64");
   const int lane = (int)(threadIdx.x & 63);
   const int zr = zc * 64 + lane;
   const bool inz = zr < NZ;
@@ -609,7 +609,7 @@ __device__ __forceinline__ void ah_march(const MarchConsts<R> &k, const Levels<R
   };
   if constexpr (PF) prefetch(x0, x0 & 1);
   double rho_sum = 0, g_sum = 0, bad = 0;   /* MODE 2 */
-  /* DEFER (the FP64 step alone): plane x's five stores are issued at the top
+  /* DEFER (FP64): plane x's five stores are issued at the top
    * of plane x+1 (after the prefetched values, where there are any, are
    * read).  Issued at the end of plane x they sit in the memory counter
    * behind the prefetch, and the wait for the prefetched values (which the
@@ -617,7 +617,7 @@ __device__ __forceinline__ void ah_march(const MarchConsts<R> &k, const Levels<R
    * live ranges the compiler keeps across the loop: every FP64 variant fits
    * 161 VGPRs with the prefetch, 64-bit plane addressing 170 -> 160, so all
    * keep 3 waves per SIMD.  FP32 is slower with it (+0.5 %). */
-  constexpr bool DEFER = sizeof(R) == 8 && MODE == 0;
+  constexpr bool DEFER = sizeof(R) == 8;
   SiteOut<R> pend{};
   auto store_out = [&](int x, const SiteOut<R> &v) {
     SP(2, x, R0, oz, v.X);

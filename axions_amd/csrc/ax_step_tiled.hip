@@ -127,6 +127,17 @@ int default_xseg(const ax_params &p) {
   return p.precision == 32 ? 16 : 32;
 }
 
+/* The LDS prefetch of the FP64 march applies: whole 64-site chunks, three
+ * field planes within 32-bit offsets, and the [comp][level] plane order its
+ * phi and theta_x/theta_y pairs rely on (AX_NO_PREFETCH turns it off). */
+template <typename R>
+static bool prefetch_ok(const StepConsts &k, const Levels<R> &L) {
+  const bool order = L.cur[0] < L.cur[1] && L.cur[2] < L.cur[3] && L.prv[0] < L.prv[1] && L.prv[2] < L.prv[3];
+  return k.nz % 64 == 0 && order &&
+         (size_t)k.nx * k.ny * k.nz * sizeof(double) * 3 + ((size_t)1 << 20) < ((size_t)1 << 32) &&
+         getenv("AX_NO_PREFETCH") == nullptr;
+}
+
 template <typename R>
 hipError_t launch_step_tiled_ah(const StepConsts &k, const Levels<R> &L, int xseg, hipStream_t st) {
   const TileGrid g = make_tile_grid(k.nx, k.ny, k.nz, xseg);
@@ -144,10 +155,7 @@ hipError_t launch_step_tiled_ah(const StepConsts &k, const Levels<R> &L, int xse
    * phi and theta_x/theta_y pairs rely on the [comp][level] plane order).
    * Every variant keeps 3 waves per SIMD with it (161 VGPRs, with the stores
    * issued one plane late). */
-  const bool order = L.cur[0] < L.cur[1] && L.cur[2] < L.cur[3] && L.prv[0] < L.prv[1] && L.prv[2] < L.prv[3];
-  const bool pf = sizeof(R) == 8 && !big && k.nz % 64 == 0 && order &&
-                  (size_t)k.nx * k.ny * k.nz * sizeof(R) * 3 + ((size_t)1 << 20) < ((size_t)1 << 32) &&
-                  getenv("AX_NO_PREFETCH") == nullptr;
+  const bool pf = sizeof(R) == 8 && !big && prefetch_ok(k, L);
 #define AX_AHPF(P2, PL)                                                                                     \
   hipLaunchKernelGGL((ah_tiled_kernel<double, P2, PL, false, true>), grid, block, 0, st, m, L, k.nx, k.ny, k.nz, g, \
                      k.ex, k.ez)
@@ -176,15 +184,17 @@ hipError_t launch_step_tiled_ah(const StepConsts &k, const Levels<R> &L, int xse
 
 /* The FP64 periodic Abelian-Higgs step with its current level's divergence
  * stored (MODE 1) or its current level's diagnostics taken (MODE 2), see
- * ah_march.  One partial-sum triple per work item (MODE 2). */
-template <bool POW2, bool P2D, int MODE>
+ * ah_march.  One partial-sum triple per work item (MODE 2).  PF: the LDS
+ * prefetch of the plain step (MODE 1 keeps 3 waves per SIMD with it, 167
+ * VGPRs; MODE 2 runs at 2 either way). */
+template <bool POW2, bool P2D, int MODE, bool PF>
 __global__ void __launch_bounds__(64) ah_fused_kernel(MarchConsts<double> k, Levels<double> L, int nx, int ny, int nz,
                                                       TileGrid g, FusedDiag fd) {
   const int w = (int)blockIdx.x;
   if (w >= g.work) return;
   int zc, y, x0, x1;
   if (!tile_range(g, nx, ny, 0, w, zc, y, x0, x1)) return;
-  ah_march<double, POW2, 2, false, false, MODE, P2D>(k, L, nx, ny, nz, y, zc, x0, x1, false, fd, w);
+  ah_march<double, POW2, 2, false, PF, MODE, P2D>(k, L, nx, ny, nz, y, zc, x0, x1, false, fd, w);
 }
 
 int fused_work_items(int nx, int ny, int nz, int xseg) { return make_tile_grid(nx, ny, nz, xseg).work; }
@@ -195,9 +205,14 @@ hipError_t launch_step_fused(const StepConsts &k, const Levels<double> &L, int x
   MarchConsts<double> m;
   march_consts_from<double>(k, rate_limit<double>(k.dt), m);
   const bool pow2 = is_pow2(k.dx2) && is_pow2(k.dy2) && is_pow2(k.dz2);
+  /* the plain step's prefetch conditions (launch_step_tiled_ah) */
+  const bool pf = prefetch_ok(k, L);
   dim3 grid(g.work), block(64);
-#define AX_FL(P2, PD, MD) \
-  hipLaunchKernelGGL((ah_fused_kernel<P2, PD, MD>), grid, block, 0, st, m, L, k.nx, k.ny, k.nz, g, fd)
+#define AX_FL(P2, PD, MD)                                                                                      \
+  do {                                                                                                         \
+    if (pf) hipLaunchKernelGGL((ah_fused_kernel<P2, PD, MD, true>), grid, block, 0, st, m, L, k.nx, k.ny, k.nz, g, fd); \
+    else hipLaunchKernelGGL((ah_fused_kernel<P2, PD, MD, false>), grid, block, 0, st, m, L, k.nx, k.ny, k.nz, g, fd); \
+  } while (0)
   if (mode == 1) {
     if (pow2) AX_FL(true, false, 1);
     else AX_FL(false, false, 1);

@@ -74,6 +74,11 @@ CASES = {
                                             g=0.8, alpha=2.0, damped_nt=6, dampFac=0.4), 4, [9, 12], 3, 8),
     # the benchmark's couplings, first call starts off-cadence
     "c2_couplings_64": (lambda: ax.ah_params(64), 7, [20, 20, 13], 20, None),
+    # whole z chunks (the LDS prefetch) with true divisions, friction and
+    # damping, 3-plane segments
+    "aniso_friction_128": (lambda: ax.ah_params(9, ny=10, nz=128, dx=0.7, dy=0.6, dz=0.55, dt=0.13, lam=1.5,
+                                                eta=0.9, g=0.8, alpha=2.0, damped_nt=6, dampFac=0.4,
+                                                theta_damp_form=1), 3, [10, 8], 3, 3),
 }
 
 
