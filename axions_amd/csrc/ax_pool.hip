@@ -117,7 +117,7 @@ __global__ void __launch_bounds__(64) pool_step_kernel(ax_params p, R lim, const
    * stepped once whatever the grid. */
   const int q = (work + 7) >> 3;
   for (int w = (int)blockIdx.x; w < 8 * q; w += (int)gridDim.x) {
-    const int it = (w & 7) * q + (w >> 3);
+    const int it = xcd_item(w, q);
     if (it >= work) continue;
     const int e = find_entry(ent, count, it, true);
     const BatchEntry E = ent[e];
@@ -564,22 +564,20 @@ int ax_pool::enqueue_chunk() {
     ++count;
   }
   if (count == 0) return AX_OK;
-  /* The tuned segments are chosen for a full pool.  With few lattices
-   * resident they leave workgroups of the persistent grid idle: halve the
-   * longest segments until the chunk has two items per workgroup or every
-   * segment is 4 planes (the partition of a lattice does not change its
-   * results). */
-  while (work < 2 * grid_step) {
-    int longest = 0;
-    for (int k = 0; k < count; ++k) longest = std::max(longest, (int)ent[k].xseg);
-    if (longest <= 4) break;
+  /* two items per workgroup of the persistent grid (fill_segments) */
+  if (work < 2 * grid_step) {
+    std::vector<int> nn(count), nw(count), xs(count);
+    for (int k = 0; k < count; ++k) {
+      nn[k] = ent[k].n;
+      nw[k] = ent[k].nw;
+      xs[k] = ent[k].xseg;
+    }
+    fill_segments(count, nn.data(), nw.data(), xs.data(), 2LL * grid_step);
     work = 0;
     for (int k = 0; k < count; ++k) {
       BatchEntry &E = ent[k];
-      if (E.xseg == longest) {
-        E.xseg = std::max(4, longest / 2);
-        E.nseg = (E.n + E.xseg - 1) / E.xseg;
-      }
+      E.xseg = xs[k];
+      E.nseg = (E.n + E.xseg - 1) / E.xseg;
       E.work_begin = work;
       work += E.nw * E.n * E.nseg;
     }

@@ -9,6 +9,9 @@
  *                 the edge phases of its winding pass).
  *  xi_of       string-length density from the three plaquette-plane counts,
  *              the expression ax_diagnostics uses.
+ *  fill_segments  the chunk fill: shortens the tuned x segments of a chunk
+ *              that holds too few work items for the persistent grid.
+ *  xcd_item    the XCD-aware order of the batched step kernel's work items.
  */
 #ifndef AXIONS_AX_SCHED_H
 #define AXIONS_AX_SCHED_H
@@ -116,6 +119,42 @@ inline double xi_of(double wxy, double wxz, double wyz, double sites, double dx,
   const double len = (2.0 / 3.0) * (wxy * dz + wxz * dy + wyz * dx);
   return len / (sites * dx * dy * dz);
 }
+
+/* The chunk fill.  The tuned segments are chosen for a full pool; with few
+ * lattices resident they leave workgroups of the persistent grid idle.  For
+ * count lattices of edge n[k] with nw[k] z chunks of 64 and segments of
+ * xseg[k] planes, halve the longest segments (never below floor planes)
+ * until the chunk has at least want work items or every segment is at
+ * floor; xseg is updated in place and the chunk's work items (nw n nseg
+ * summed) returned.  The partition of a lattice does not change its
+ * results. */
+inline long long fill_segments(int count, const int *n, const int *nw, int *xseg, long long want, int floor = 4) {
+  auto items = [&]() {
+    long long w = 0;
+    for (int k = 0; k < count; ++k) w += (long long)nw[k] * n[k] * ((n[k] + xseg[k] - 1) / xseg[k]);
+    return w;
+  };
+  long long work = items();
+  while (work < want) {
+    int longest = 0;
+    for (int k = 0; k < count; ++k) longest = xseg[k] > longest ? xseg[k] : longest;
+    if (longest <= floor) break;
+    const int half = longest / 2 > floor ? longest / 2 : floor;
+    for (int k = 0; k < count; ++k)
+      if (xseg[k] == longest) xseg[k] = half;
+    work = items();
+  }
+  return work;
+}
+
+/* The XCD-aware order of a grid of 8q workgroups (a multiple of the eight
+ * XCDs, dispatched to them round-robin): the w-th of the grid's items is
+ * item (w mod 8) q + w / 8, so XCD k takes the k-th eighth of the items.  A
+ * bijection of [0, 8q). */
+#if defined(__HIPCC__) || defined(__HIP__)
+__host__ __device__
+#endif
+inline int xcd_item(int w, int q) { return (w & 7) * q + (w >> 3); }
 
 }  // namespace ax
 

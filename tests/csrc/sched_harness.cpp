@@ -16,4 +16,8 @@ size_t lattice_bytes(int n, size_t esize, size_t pad) { return ax::lattice_bytes
 double xi_of(double a, double b, double c, double V, double dx, double dy, double dz) {
   return ax::xi_of(a, b, c, V, dx, dy, dz);
 }
+long long fill_segments(int count, const int *n, const int *nw, int *xseg, long long want, int floor) {
+  return ax::fill_segments(count, n, nw, xseg, want, floor);
+}
+int xcd_item(int w, int q) { return ax::xcd_item(w, q); }
 }

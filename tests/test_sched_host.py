@@ -35,6 +35,11 @@ def h():
     L.lattice_bytes.argtypes = [ctypes.c_int, sz, sz]
     L.xi_of.restype = ctypes.c_double
     L.xi_of.argtypes = [ctypes.c_double] * 7
+    ip = ctypes.POINTER(ctypes.c_int)
+    L.fill_segments.restype = ctypes.c_longlong
+    L.fill_segments.argtypes = [ctypes.c_int, ip, ip, ip, ctypes.c_longlong, ctypes.c_int]
+    L.xcd_item.restype = ctypes.c_int
+    L.xcd_item.argtypes = [ctypes.c_int, ctypes.c_int]
     return L
 
 
@@ -108,3 +113,54 @@ def test_xi_expression(h):
         V, dx, dy, dz = 64.0 ** 3, 0.5, 0.6, 0.7
         ref = (2.0 / 3.0) * (w[0] * dz + w[1] * dy + w[2] * dx) / (V * dx * dy * dz)
         assert h.xi_of(*w, V, dx, dy, dz) == ref
+
+
+def items_of(n, xseg):
+    """Work items of a chunk: z chunks of 64 x rows x x segments per lattice."""
+    return sum(((m + 63) // 64) * m * -(-m // x) for m, x in zip(n, xseg))
+
+
+def fill_model(n, xseg, want, floor=4):
+    """The chunk fill restated: while short of want items, halve every
+    segment of the longest length (not below floor)."""
+    xs = list(xseg)
+    while items_of(n, xs) < want:
+        longest = max(xs)
+        if longest <= floor:
+            break
+        xs = [max(longest // 2, floor) if x == longest else x for x in xs]
+    return xs, items_of(n, xs)
+
+
+def test_fill_segments(h):
+    rng = np.random.default_rng(3)
+    arr = lambda v: (ctypes.c_int * len(v))(*v)
+    for case in range(300):
+        count = int(rng.integers(1, 12))
+        n = [int(rng.choice([64, 96, 128, 192, 256])) for _ in range(count)]
+        xseg = [int(rng.choice([4, 8, 16, 32])) for _ in range(count)]
+        want = int(rng.integers(1, 40000))
+        xs = arr(xseg)
+        work = h.fill_segments(count, arr(n), arr([(m + 63) // 64 for m in n]), xs, want, 4)
+        got = list(xs)
+        ref, ref_work = fill_model(n, xseg, want)
+        assert got == ref and work == ref_work == items_of(n, got), (n, xseg, want, got, ref)
+        # segments only shorten; the chunk is full or every segment is at the floor
+        assert all(g <= x for g, x in zip(got, xseg))
+        assert work >= want or all(g == 4 for g in got) or work >= items_of(n, xseg)
+    # a full chunk is left as it is
+    xs = arr([32, 32])
+    assert h.fill_segments(2, arr([256, 256]), arr([4, 4]), xs, 100, 4) == 2 * 4 * 256 * 8
+    assert list(xs) == [32, 32]
+
+
+@pytest.mark.parametrize("q", [1, 2, 7, 64, 1000, 12345])
+def test_xcd_item_order(h, q):
+    """w -> (w mod 8) q + w / 8 is a bijection of [0, 8q) that gives the
+    workgroups of XCD k (w = k mod 8) the contiguous items [k q, (k+1) q),
+    in order."""
+    its = np.array([h.xcd_item(w, q) for w in range(8 * q)])
+    assert np.array_equal(np.sort(its), np.arange(8 * q))
+    for k in range(8):
+        mine = its[k::8]
+        assert np.array_equal(mine, np.arange(k * q, (k + 1) * q))
