@@ -49,7 +49,7 @@ CONFIGS = {
                                            **{"lambda": 1.5}, eta=0.9, g=0.8,
                                            damped_nt=10, dampFac=0.3, alpha=2,
                                            scaling=1), True),
-    # plain AH 64^3, dx=0.5 dt=0.1 lambda=2 g=eta=1, 200 steps
+    # the plain AH workload at 64^3 (dx=0.5 dt=0.1 lambda=2 g=eta=1), 200 steps
     "evo_c1_shared": ("evolution.cpp", dict(nx=64, ny=64, nz=64, nt=200, dx=0.5, dy=0.5, dz=0.5,
                                             dt=0.1, **{"lambda": 2}, eta=1, g=1, damped_nt=0,
                                             dampFac=0.5, alpha=0, scaling=1, saveFreq=10,
