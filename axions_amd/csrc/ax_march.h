@@ -22,9 +22,11 @@
  *     the short k = 0 form of the sine when every lane's argument qualifies,
  *     which holds almost always because plaquette angles stay below pi/4
  *     while link angles drift.
- * Each field value is read from HBM once per step: plane x+1 (current
- * level) is loaded as the next iteration's centre and carried; the y/z
- * neighbour rows are cache hits.  The loop's loads and stores are raw
+ * Each field value is loaded once per step by the wave that owns it: plane
+ * x+1 (current level) is loaded as the next iteration's centre and carried;
+ * the y/z neighbour rows are meant to be cache hits (at 512^3 FP64 the L2 ->
+ * fabric reads are 1.25x the algorithmic bytes without the LDS prefetch
+ * below and 1.6x with it, part of them served by the Infinity Cache).  The loop's loads and stores are raw
  * buffer accesses (bld/bst below) whose addresses cost no vector
  * instructions; the previous level's reads and the new level's writes carry
  * the streaming hint so that they do not push those rows out of L2.
