@@ -550,8 +550,10 @@ __device__ __forceinline__ void ah_march(const MarchConsts<R> &k, const Levels<R
    * per plane.  The pass reads its planes' lines ahead of the march; over
    * longer blocks (16 planes: +0.4 GB FP64, +1.7 GB FP32 per 512^3 step)
    * they are evicted from L2 before the march reaches them and are read
-   * twice, so the blocks are short (measured best: FP64 8, FP32 4). */
-  constexpr int EB = sizeof(R) == 8 ? 8 : 4;
+   * twice, so the blocks are short.  Measured best: 4 planes in FP32, and
+   * in FP64 with the LDS prefetch (512^3: 2 planes 3.697 ms, 3: 3.671, 4:
+   * 3.663, 6: 3.668, 8: 3.674, 16: 3.706); 8 in FP64 before it. */
+  constexpr int EB = 4;
   __shared__ ZEdge<R> zedge[EB];
   const uint32_t em = R0 + oe0, emP = RP + oe0, ec = R0 + (uint32_t)z0 * E;
   auto edge_block = [&](int xb, int xe) {
