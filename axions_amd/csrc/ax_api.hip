@@ -561,13 +561,13 @@ int ax_diagnostics(ax_lattice *L, int32_t want_winding, ax_diag *out) {
   }
   if (e != hipSuccess) return hip_fail(e, "diag launch");
   if (want_winding) {
-    int rc = ensure_winding_buffer(L);
-    if (rc) return rc;
+    /* only the count is returned: the march stores no winding field */
     if (march) {
-      e = launch_phase_winding(L, L->wind, L->dres + 4);
+      e = launch_phase_winding(L, nullptr, L->dres + 4);
       if (e != hipSuccess) return hip_fail(e, "winding launch");
     } else {
-      rc = launch_winding_field(L, 0, L->wind, L->part, L->dres + 4);
+      int rc = ensure_winding_buffer(L);
+      if (!rc) rc = launch_winding_field(L, 0, L->wind, L->part, L->dres + 4);
       if (rc) return rc;
     }
   }
@@ -610,7 +610,6 @@ static int ensure_fused(ax_lattice *L, size_t npoints, bool winding) {
   if (!rc) rc = grow(&L->fpart, &L->fpart_n, 3 * items, "partial sums");
   if (!rc) rc = grow(&L->fres, &L->fres_n, 8 * (npoints ? npoints : 1), "diagnostic results");
   if (!rc) rc = ensure_phase_buffer(L);
-  if (!rc && winding) rc = ensure_winding_buffer(L);
   return rc;
 }
 
@@ -639,7 +638,7 @@ static int launch_fused(ax_lattice *L, int mode, size_t k, bool winding) {
     double *res = L->fres + 8 * k;
     e = launch_sum3(L->fpart, fused_work_items(p.nx, p.ny, p.nz, L->xseg), res, L->stream);
     /* windings of level n: its links are intact until the next step */
-    if (e == hipSuccess && winding) e = launch_phase_winding(L, L->wind, res + 4);
+    if (e == hipSuccess && winding) e = launch_phase_winding(L, nullptr, res + 4);
   }
   if (e != hipSuccess) return hip_fail(e, "fused step launch");
   L->div_for = mode == 1 ? n + 1 : -1;

@@ -124,7 +124,8 @@ hipError_t launch_global_energy(const ax_params &p, const void *state, size_t pi
 template <typename T>
 hipError_t launch_diag(const DiagConsts &k, const Levels<T> &L, double *part, double *out3, hipStream_t st);
 /* periodic Abelian-Higgs: energy/Gauss march, with phase != NULL also the
- * forward edge phases [3][V]; launch_wind_phase turns them into windings */
+ * forward edge phases [3][V]; launch_wind_phase turns them into windings
+ * (the field into out when out != NULL, the counts into out3) */
 constexpr int kDiagXseg = 32;
 bool use_diag_march(const ax_params &p);
 size_t diag_part_blocks(const ax_params &p);

@@ -13,12 +13,15 @@
  *                      each chunk, so the launch itself never changes and a
  *                      chunk of K steps is one captured hipGraph replayed
  *                      chunk after chunk.
- *   pool_phase_kernel  the forward edge phases of every resident lattice
- *                      into its slot, plus a non-finite flag;
- *   pool_wind_kernel   wound-plaquette counts from those phases (ax_wind.h,
- *                      the arithmetic of the single-lattice diagnostics),
- *                      summed per lattice with integer atomics
- *                      (order-independent, so deterministic).
+ *   pool_wind_pass_kernel
+ *                      the winding passes over the same work items, each
+ *                      item a march along x (ax_wind.h): first the forward
+ *                      edge phases of every resident lattice into its slot,
+ *                      plus a non-finite count, then the wound-plaquette
+ *                      counts from those phases (the arithmetic of the
+ *                      single-lattice diagnostics), summed per lattice
+ *                      with integer atomics (order-independent, so
+ *                      deterministic).
  *   pool_zero_kernel   clears those counters (a kernel, not a memset, so the
  *                      graph holds kernel nodes only).
  *
@@ -59,31 +62,30 @@ struct BatchEntry {
   uint64_t pitch;       /* elements from one plane to the next */
   int32_t n;            /* lattice edge (n^3 sites) */
   int32_t nw, xseg, nseg;
-  int32_t work_begin;   /* first step work item of this lattice */
-  int32_t wblk_begin;   /* first 256-site winding block of this lattice */
+  int32_t work_begin;   /* first work item of this lattice */
   int32_t t0;           /* TimeStep at the start of the chunk */
   int32_t steps;        /* TimeSteps to take in this chunk (<= chunk length) */
   int32_t pre;          /* 1: count windings before the first step (t0 == 0) */
-  int32_t pad;
+  int32_t pad[2];
   uint64_t phase;       /* device address of the lattice's edge phases [3][n^3] */
 };
 static_assert(sizeof(BatchEntry) == 64, "entry layout");
 
 struct BatchHeader {
   int32_t count;        /* lattices in the table */
-  int32_t work;         /* total step work items */
-  int32_t wblocks;      /* total winding blocks */
-  int32_t pad;
+  int32_t work;         /* total work items */
+  int32_t pad[2];
 };
 
 /* counters per table entry: [phase 0: xy, xz, yz, bad][phase 1: xy, xz, yz, bad] */
 constexpr int kCnt = 8;
 
-__device__ __forceinline__ int find_entry(const BatchEntry *ent, int count, int w, bool by_work) {
+/* the table entry whose work items include item w */
+__device__ __forceinline__ int find_entry(const BatchEntry *ent, int count, int w) {
   int lo = 0, hi = count - 1;
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
-    const int b = by_work ? ent[mid].work_begin : ent[mid].wblk_begin;
+    const int b = ent[mid].work_begin;
     if (b <= w) lo = mid;
     else hi = mid - 1;
   }
@@ -119,7 +121,7 @@ __global__ void __launch_bounds__(64) pool_step_kernel(ax_params p, R lim, const
   for (int w = (int)blockIdx.x; w < 8 * q; w += (int)gridDim.x) {
     const int it = xcd_item(w, q);
     if (it >= work) continue;
-    const int e = find_entry(ent, count, it, true);
+    const int e = find_entry(ent, count, it);
     const BatchEntry E = ent[e];
     if (j >= E.steps) continue;
     const int local = it - E.work_begin;
@@ -139,76 +141,67 @@ __device__ __forceinline__ int wave_isum(int v) {
   return v;
 }
 
-/* Block-sum of four per-thread counts, added to the entry's counters. */
-__device__ __forceinline__ void add_counts(int v[4], int (*sh)[4], unsigned long long *dst) {
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+/* Adds the wave's per-lane counts v[0..nc) to counters dst[0..nc) (lane 0,
+ * integer atomics: the totals do not depend on the order). */
+template <int NC>
+__device__ __forceinline__ void flush_counts(const int *v, unsigned long long *dst) {
 #pragma unroll
-  for (int c = 0; c < 4; ++c) v[c] = wave_isum(v[c]);
-  __syncthreads();                    /* sh free from the previous block */
-  if (lane == 0)
-    for (int c = 0; c < 4; ++c) sh[wave][c] = v[c];
-  __syncthreads();
-  if (threadIdx.x < 4) {
-    const int c = threadIdx.x;
-    const int tot = sh[0][c] + sh[1][c] + sh[2][c] + sh[3][c];
-    if (tot) atomicAdd(&dst[c], (unsigned long long)tot);
+  for (int c = 0; c < NC; ++c) {
+    const int tot = wave_isum(v[c]);
+    if ((threadIdx.x & 63) == 0 && tot) atomicAdd(&dst[c], (unsigned long long)tot);
   }
 }
 
-/* which = 0: the state before the chunk's first step (lattices with pre
- * set), 1: the state after the chunk's steps. */
-template <typename T>
-__global__ void __launch_bounds__(256) pool_phase_kernel(const BatchHeader *__restrict__ hdr,
-                                                         const BatchEntry *__restrict__ ent,
-                                                         unsigned long long *counts, int which) {
-  __shared__ int sh[4][4];
-  const int count = hdr->count, nb = hdr->wblocks;
-  for (int b = (int)blockIdx.x; b < nb; b += (int)gridDim.x) {
-    const int e = find_entry(ent, count, b, false);
+/* The winding passes march over the step kernel's work items (same table,
+ * same XCD-aware order): item -> lattice, z chunk, row, x segment.  A
+ * workgroup keeps its counts in registers while consecutive items belong to
+ * one lattice and adds them to that lattice's counters when the lattice
+ * changes.  which = 0: the state before the chunk's first step (lattices
+ * with pre set), 1: the state after the chunk's steps. */
+template <typename T, bool WIND>
+__global__ void __launch_bounds__(64) pool_wind_pass_kernel(const BatchHeader *__restrict__ hdr,
+                                                            const BatchEntry *__restrict__ ent,
+                                                            unsigned long long *counts, int which) {
+  const int count = hdr->count, work = hdr->work;
+  const int q = (work + 7) >> 3;
+  int ce = -1;
+  int v[3] = {0, 0, 0};
+  /* edge phases: v[0] counts sites with a non-finite field (counter 3);
+   * windings: v[0..2] sum |W| of the xy, xz, yz plaquettes (counters 0..2) */
+  auto flush = [&]() {
+    if (ce < 0) return;
+    unsigned long long *dst = counts + (size_t)ce * kCnt + which * 4;
+    if (WIND) flush_counts<3>(v, dst);
+    else flush_counts<1>(v, dst + 3);
+    v[0] = v[1] = v[2] = 0;
+  };
+  for (int w = (int)blockIdx.x; w < 8 * q; w += (int)gridDim.x) {
+    const int it = xcd_item(w, q);
+    if (it >= work) continue;
+    const int e = find_entry(ent, count, it);
     const BatchEntry E = ent[e];
     if (which == 0 && !E.pre) continue;
-    const int n = E.n;
-    const int V = n * n * n;
-    const int s = (b - E.wblk_begin) * 256 + (int)threadIdx.x;
-    const int t = E.t0 + (which == 0 ? 0 : E.steps);   /* state after t TimeSteps */
-    int v[4] = {0, 0, 0, 0};
-    if (s < V) {
-      const Levels<T> L = entry_levels<T>(E, t);
-      const int kk = s % n, rr = s / n, jj = rr % n, ii = rr / n;
-      double d[3];
-      site_edge_phases<T>(n, n, n, L.cur[0], L.cur[1], L.cur[2], L.cur[3], L.cur[4], ii, jj, kk, d);
-      double *ph = (double *)E.phase;
-      for (int a = 0; a < 3; ++a) ph[(size_t)a * V + s] = d[a];
-      for (int c = 0; c < 5; ++c) v[3] |= !isfinite((double)L.cur[c][s]);
+    if (e != ce) {
+      flush();
+      ce = e;
     }
-    add_counts(v, sh, counts + (size_t)e * kCnt + which * 4);
-  }
-}
-
-template <typename T>
-__global__ void __launch_bounds__(256) pool_wind_kernel(const BatchHeader *__restrict__ hdr,
-                                                        const BatchEntry *__restrict__ ent,
-                                                        unsigned long long *counts, int which) {
-  __shared__ int sh[4][4];
-  const int count = hdr->count, nb = hdr->wblocks;
-  for (int b = (int)blockIdx.x; b < nb; b += (int)gridDim.x) {
-    const int e = find_entry(ent, count, b, false);
-    const BatchEntry E = ent[e];
-    if (which == 0 && !E.pre) continue;
-    const int n = E.n;
-    const int V = n * n * n;
-    const int s = (b - E.wblk_begin) * 256 + (int)threadIdx.x;
-    const int t = E.t0 + (which == 0 ? 0 : E.steps);
-    int v[4] = {0, 0, 0, 0};
-    if (s < V) {
-      const Levels<T> L = entry_levels<T>(E, t);
-      const int kk = s % n, rr = s / n, jj = rr % n, ii = rr / n;
-      int w[3];
-      windings_from_phases<T>(n, n, n, L.cur[2], L.cur[3], L.cur[4], (const double *)E.phase, ii, jj, kk, w);
-      for (int c = 0; c < 3; ++c) v[c] = abs(w[c]);
+    const int local = it - E.work_begin;
+    const int zc = local % E.nw, r = local / E.nw;
+    const int y = r % E.n, seg = r / E.n;
+    const int x0 = seg * E.xseg, x1 = min(E.n, x0 + E.xseg);
+    const Levels<T> L = entry_levels<T>(E, E.t0 + (which == 0 ? 0 : E.steps));   /* state after t TimeSteps */
+    const WindRow row(E.n, E.n, E.n, y, zc);
+    if (WIND) {
+      wind_march<T>(row, L.cur[2], L.cur[3], L.cur[4], (const double *)E.phase, x0, x1,
+                    [&](size_t, const int *wn) {
+#pragma unroll
+                      for (int c = 0; c < 3; ++c) v[c] += abs(wn[c]);
+                    });
+    } else {
+      v[0] += phase_march<T>(row, L.cur[0], L.cur[1], L.cur[2], L.cur[3], L.cur[4], (double *)E.phase, x0, x1);
     }
-    add_counts(v, sh, counts + (size_t)e * kCnt + which * 4);
   }
+  flush();
 }
 
 __global__ void __launch_bounds__(256) pool_zero_kernel(const BatchHeader *__restrict__ hdr,
@@ -289,7 +282,7 @@ struct ax_pool {
   static constexpr int kBufs = 3;
   char *h_tab[kBufs] = {};          /* pinned: header + entries */
   unsigned long long *h_cnt[kBufs] = {};
-  int grid_step = 0, grid_wind = 0, grid_phase = 0;
+  int grid_step = 0, grid_wind = 0;
   /* a chunk is three parts: zero counters + windings of new lattices, the
    * K step launches, windings after the steps; one captured graph each, so
    * that events between them time the step launches alone */
@@ -352,11 +345,11 @@ void build_ic(uint32_t seed, size_t V, double eta, R *re, R *im) {
 /* edge phases, then windings from them (which: 0 before the steps, 1 after) */
 void ax_pool::launch_winding(hipStream_t s, int which) {
   if (esize == 8) {
-    hipLaunchKernelGGL((pool_phase_kernel<double>), dim3(grid_phase), dim3(256), 0, s, d_hdr, d_ent, d_cnt, which);
-    hipLaunchKernelGGL((pool_wind_kernel<double>), dim3(grid_wind), dim3(256), 0, s, d_hdr, d_ent, d_cnt, which);
+    hipLaunchKernelGGL((pool_wind_pass_kernel<double, false>), dim3(grid_wind), dim3(64), 0, s, d_hdr, d_ent, d_cnt, which);
+    hipLaunchKernelGGL((pool_wind_pass_kernel<double, true>), dim3(grid_wind), dim3(64), 0, s, d_hdr, d_ent, d_cnt, which);
   } else {
-    hipLaunchKernelGGL((pool_phase_kernel<float>), dim3(grid_phase), dim3(256), 0, s, d_hdr, d_ent, d_cnt, which);
-    hipLaunchKernelGGL((pool_wind_kernel<float>), dim3(grid_wind), dim3(256), 0, s, d_hdr, d_ent, d_cnt, which);
+    hipLaunchKernelGGL((pool_wind_pass_kernel<float, false>), dim3(grid_wind), dim3(64), 0, s, d_hdr, d_ent, d_cnt, which);
+    hipLaunchKernelGGL((pool_wind_pass_kernel<float, true>), dim3(grid_wind), dim3(64), 0, s, d_hdr, d_ent, d_cnt, which);
   }
 }
 
@@ -531,7 +524,7 @@ int ax_pool::enqueue_chunk() {
   ChunkRec c;
   c.buf = b;
   c.seq = next_seq;
-  int count = 0, work = 0, wblk = 0;
+  int count = 0, work = 0;
   for (int64_t id : resident) {
     Req &r = *reqs[id];
     if (r.status != AX_REQ_RUNNING || r.t >= r.steps) continue;
@@ -544,14 +537,12 @@ int ax_pool::enqueue_chunk() {
     E.xseg = (r.xseg <= 0 || r.xseg > r.n) ? r.n : r.xseg;
     E.nseg = (r.n + E.xseg - 1) / E.xseg;
     E.work_begin = work;
-    E.wblk_begin = wblk;
     E.t0 = r.t;
     E.steps = std::min(K, r.steps - r.t);
     E.pre = r.pre_done ? 0 : 1;
-    E.pad = 0;
+    E.pad[0] = E.pad[1] = 0;
     E.phase = (uint64_t)(uintptr_t)(arena_base + r.off + lattice_phase_offset(r.n, esize, pad_bytes));
     work += E.nw * r.n * E.nseg;
-    wblk += (int)((V + 255) / 256);
     c.items.push_back({id, E.t0, E.steps, E.pre});
     c.site_updates += (double)V * E.steps;
     if (r.wait_upload) {
@@ -584,8 +575,7 @@ int ax_pool::enqueue_chunk() {
   }
   h->count = count;
   h->work = work;
-  h->wblocks = wblk;
-  h->pad = 0;
+  h->pad[0] = h->pad[1] = 0;
   hipError_t e = hipMemcpyAsync(d_hdr, h_tab[b], 64 + (size_t)count * sizeof(BatchEntry), hipMemcpyHostToDevice, cs);
   if (e != hipSuccess) return hip_fail(e, "table upload");
   e = hipEventCreate(&c.s0);
@@ -942,17 +932,18 @@ int ax_pool_create(const ax_params *tmpl, const ax_pool_config *cfg_in, ax_pool 
   else
     e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&bs, P->pow2 ? (const void *)pool_step_kernel<float, true>
                                                                   : (const void *)pool_step_kernel<float, false>, 64, 0);
+  /* the two winding passes share a grid: the smaller of their occupancies */
   int bp = 0;
   if (e == hipSuccess)
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&bw, P->esize == 8 ? (const void *)pool_wind_kernel<double>
-                                                                         : (const void *)pool_wind_kernel<float>, 256, 0);
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&bw, P->esize == 8 ? (const void *)pool_wind_pass_kernel<double, true>
+                                                                         : (const void *)pool_wind_pass_kernel<float, true>, 64, 0);
   if (e == hipSuccess)
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&bp, P->esize == 8 ? (const void *)pool_phase_kernel<double>
-                                                                         : (const void *)pool_phase_kernel<float>, 256, 0);
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&bp, P->esize == 8 ? (const void *)pool_wind_pass_kernel<double, false>
+                                                                         : (const void *)pool_wind_pass_kernel<float, false>, 64, 0);
   if (e != hipSuccess) return cleanup(hip_fail(e, "occupancy"));
-  P->grid_step = (std::max(1, ncu) * std::max(1, bs) + 7) / 8 * 8;   /* a multiple of the 8 XCDs */
-  P->grid_wind = std::max(1, ncu) * std::max(1, bw);
-  P->grid_phase = std::max(1, ncu) * std::max(1, bp);
+  /* multiples of the 8 XCDs (the XCD-aware item order) */
+  P->grid_step = (std::max(1, ncu) * std::max(1, bs) + 7) / 8 * 8;
+  P->grid_wind = (std::max(1, ncu) * std::max(1, std::min(bw, bp)) + 7) / 8 * 8;
   if (cfg.use_graph) {
     int rc = P->capture_graph();
     if (rc) return cleanup(rc);

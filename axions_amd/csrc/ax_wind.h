@@ -17,6 +17,7 @@
 #define AXIONS_AX_WIND_H
 
 #include "ax_device.h"
+#include "ax_march.h"   /* ldo */
 
 namespace ax {
 
@@ -77,43 +78,100 @@ __device__ __forceinline__ void site_winding(int nx, int ny, int nz, const T *fr
   }
 }
 
-/* The three forward edge phases delta_a(s), a = x, y, z, of site (i,j,kk)
- * (charge 1): the edges that windings_from_phases combines.  Same
- * arithmetic as link_phase, and as the marching diagnostics that store them
- * with the link's sin/cos computed once. */
+/* One 64-lane wave per (64-site z chunk zc, row y), marching along x over
+ * the planes [x0, x1): lane l holds z = 64 zc + l (lanes past the row end
+ * load the row's last site and emit nothing).  Plane x+1's values are loaded
+ * once, as the next plane's centre, and carried; the y+1 and z+1 operands
+ * are rows that neighbouring waves load too.  Plane addresses are formed on
+ * the scalar unit (64-bit), row and lane offsets are 32-bit bytes. */
+struct WindRow {
+  int nx, ny, nz, y, z;
+  bool act;
+  size_t nynz, V;
+  uint32_t r0, rp, z0, zp;   /* element offsets: row y, row y+1, lane z, z+1 */
+  __device__ WindRow(int nx_, int ny_, int nz_, int y_, int zc) : nx(nx_), ny(ny_), nz(nz_), y(y_) {
+    const int zr = zc * 64 + (int)(threadIdx.x & 63);
+    act = zr < nz;
+    z = act ? zr : nz - 1;
+    nynz = (size_t)ny * nz;
+    V = (size_t)nx * nynz;
+    r0 = (uint32_t)(y * nz);
+    rp = (uint32_t)((y == ny - 1 ? 0 : y + 1) * nz);
+    z0 = (uint32_t)z;
+    zp = (uint32_t)(z == nz - 1 ? 0 : z + 1);
+  }
+  /* field b (element type E) in x plane x at element offsets row, lane */
+  template <typename E>
+  __device__ double at(const E *b, int x, uint32_t row, uint32_t lane) const {
+    return (double)ldo(b + (size_t)x * nynz, row * (uint32_t)sizeof(E), lane * (uint32_t)sizeof(E));
+  }
+  __device__ size_t site(int x) const { return ((size_t)x * ny + y) * nz + z; }
+};
+
+/* The forward edge phases delta_a(s) = link_phase(phi(s), phi(s+a),
+ * theta_a(s)), a = x, y, z (charge 1), of the sites of one wave's march
+ * into phase[a V + s]: the edges that wind_march combines.  The arithmetic
+ * of link_phase, as the marching diagnostics that store them with the
+ * link's sin/cos computed once.  Returns the number of the lane's sites
+ * holding a non-finite value in one of the five fields. */
 template <typename T>
-__device__ __forceinline__ void site_edge_phases(int nx, int ny, int nz, const T *fr, const T *fi, const T *tx,
-                                                 const T *ty, const T *tz, int i, int j, int kk, double d[3]) {
-  auto id = [&](int a, int b, int c) { return (a * ny + b) * nz + c; };
-  const int s = id(i, j, kk);
-  const int sa[3] = {id(wrapp(i, nx), j, kk), id(i, wrapp(j, ny), kk), id(i, j, wrapp(kk, nz))};
-  const T *th[3] = {tx, ty, tz};
-  const double cr = ld(fr, s), ci = ld(fi, s);
-#pragma unroll
-  for (int a = 0; a < 3; ++a) d[a] = link_phase(cr, ci, ld(fr, sa[a]), ld(fi, sa[a]), 1.0 * ld(th[a], s));
+__device__ __forceinline__ int phase_march(const WindRow &w, const T *fr, const T *fi, const T *tx, const T *ty,
+                                           const T *tz, double *phase, int x0, int x1) {
+  double cr = w.at(fr, x0, w.r0, w.z0), ci = w.at(fi, x0, w.r0, w.z0);
+  int bad = 0;
+  for (int x = x0; x < x1; ++x) {
+    const int xp = x == w.nx - 1 ? 0 : x + 1;
+    const double nr = w.at(fr, xp, w.r0, w.z0), ni = w.at(fi, xp, w.r0, w.z0);
+    const double yr = w.at(fr, x, w.rp, w.z0), yi = w.at(fi, x, w.rp, w.z0);
+    const double zr = w.at(fr, x, w.r0, w.zp), zi = w.at(fi, x, w.r0, w.zp);
+    const double X = w.at(tx, x, w.r0, w.z0), Y = w.at(ty, x, w.r0, w.z0), Z = w.at(tz, x, w.r0, w.z0);
+    const double dX = link_phase(cr, ci, nr, ni, 1.0 * X);
+    const double dY = link_phase(cr, ci, yr, yi, 1.0 * Y);
+    const double dZ = link_phase(cr, ci, zr, zi, 1.0 * Z);
+    if (w.act) {
+      const size_t s = w.site(x);
+      phase[s] = dX;
+      phase[w.V + s] = dY;
+      phase[2 * w.V + s] = dZ;
+      bad += (!isfinite(cr) || !isfinite(ci) || !isfinite(X) || !isfinite(Y) || !isfinite(Z)) ? 1 : 0;
+    }
+    cr = nr;
+    ci = ni;
+  }
+  return bad;
 }
 
-/* Windings of the xy, xz, yz plaquettes at site s = (i,j,kk) from stored
- * forward edge phases D[a] (V doubles each): the arithmetic of site_winding
- * with q = 1, phases and angles in its order, so the results are its bits. */
-template <typename T>
-__device__ __forceinline__ void windings_from_phases(int nx, int ny, int nz, const T *tx, const T *ty, const T *tz,
-                                                     const double *phase, int i, int j, int kk, int w[3]) {
-  auto id = [&](int a, int b, int c) { return (a * ny + b) * nz + c; };
-  const size_t V = (size_t)nx * ny * nz;
-  const int s = id(i, j, kk);
-  const int sa[3] = {id(wrapp(i, nx), j, kk), id(i, wrapp(j, ny), kk), id(i, j, wrapp(kk, nz))};
-  const T *th[3] = {tx, ty, tz};
-  const double *D[3] = {phase, phase + V, phase + 2 * V};
-  const int pa[3] = {0, 0, 1}, pb[3] = {1, 2, 2};
-#pragma unroll
-  for (int pl = 0; pl < 3; ++pl) {
-    const int a = pa[pl], b = pb[pl];
-    const double th0a = 1.0 * ld(th[a], s), th0b = 1.0 * ld(th[b], s);
-    const double th1ab = 1.0 * ld(th[b], sa[a]), th1ba = 1.0 * ld(th[a], sa[b]);
-    const double P = th0a + th1ab - th1ba - th0b;
-    const double sum = D[a][s] + D[b][sa[a]] - D[a][sa[b]] - D[b][s] + P;
-    w[pl] = (int)rint(sum * 0.15915494309189535);
+/* Windings of the xy, xz, yz plaquettes of one wave's march from the stored
+ * forward edge phases D[a] = phase + a V of the same level: for axes (a, b)
+ *   W = rint( (D_a(s) + D_b(s+a) - D_a(s+b) - D_b(s) + P) / 2pi ),
+ *   P = theta_a(s) + theta_b(s+a) - theta_a(s+b) - theta_b(s),
+ * the arithmetic of site_winding with q = 1 (phases and angles in its
+ * order, so its bits).  emit(s, w) is called for each of the lane's sites. */
+template <typename T, typename Emit>
+__device__ __forceinline__ void wind_march(const WindRow &w, const T *tx, const T *ty, const T *tz,
+                                           const double *phase, int x0, int x1, Emit emit) {
+  const double *DX = phase, *DY = phase + w.V, *DZ = phase + 2 * w.V;
+  double cX = 1.0 * w.at(tx, x0, w.r0, w.z0), cY = 1.0 * w.at(ty, x0, w.r0, w.z0);
+  double cZ = 1.0 * w.at(tz, x0, w.r0, w.z0);
+  double dX = w.at(DX, x0, w.r0, w.z0), dY = w.at(DY, x0, w.r0, w.z0), dZ = w.at(DZ, x0, w.r0, w.z0);
+  for (int x = x0; x < x1; ++x) {
+    const int xp = x == w.nx - 1 ? 0 : x + 1;
+    /* plane x+1 (next centre), row y+1, site z+1 */
+    const double nX = 1.0 * w.at(tx, xp, w.r0, w.z0), nY = 1.0 * w.at(ty, xp, w.r0, w.z0);
+    const double nZ = 1.0 * w.at(tz, xp, w.r0, w.z0);
+    const double ndX = w.at(DX, xp, w.r0, w.z0), ndY = w.at(DY, xp, w.r0, w.z0), ndZ = w.at(DZ, xp, w.r0, w.z0);
+    const double yX = 1.0 * w.at(tx, x, w.rp, w.z0), yZ = 1.0 * w.at(tz, x, w.rp, w.z0);
+    const double ydX = w.at(DX, x, w.rp, w.z0), ydZ = w.at(DZ, x, w.rp, w.z0);
+    const double zX = 1.0 * w.at(tx, x, w.r0, w.zp), zY = 1.0 * w.at(ty, x, w.r0, w.zp);
+    const double zdX = w.at(DX, x, w.r0, w.zp), zdY = w.at(DY, x, w.r0, w.zp);
+    const double Pxy = cX + nY - yX - cY, Pxz = cX + nZ - zX - cZ, Pyz = cY + yZ - zY - cZ;
+    int wn[3];
+    wn[0] = (int)rint((dX + ndY - ydX - dY + Pxy) * 0.15915494309189535);
+    wn[1] = (int)rint((dX + ndZ - zdX - dZ + Pxz) * 0.15915494309189535);
+    wn[2] = (int)rint((dY + ydZ - zdY - dZ + Pyz) * 0.15915494309189535);
+    if (w.act) emit(w.site(x), wn);
+    cX = nX; cY = nY; cZ = nZ;
+    dX = ndX; dY = ndY; dZ = ndZ;
   }
 }
 
