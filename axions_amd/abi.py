@@ -197,9 +197,9 @@ class AxPoolStats(ctypes.Structure):
 def lattice_bytes(n, precision, pad_bytes=4352):
     """Device bytes the pool holds for one n^3 request: 5 components x 2
     levels of planes, each padded by pad_bytes, then (256 B aligned) the
-    3 x n^3 double edge phases of its winding pass, rounded to 256 B."""
+    3 x n^3 FP32 edge values of its winding pass, rounded to 256 B."""
     es = 8 if precision == 64 else 4
     V = n ** 3
     b = 10 * (V + (pad_bytes + es - 1) // es) * es
-    b = (b + 255) // 256 * 256 + 3 * V * 8
+    b = (b + 255) // 256 * 256 + 3 * V * 4
     return (b + 255) // 256 * 256

@@ -110,7 +110,7 @@ struct ax_lattice {
   double *dres = nullptr;   /* small device results */
   double *hres = nullptr;   /* pinned host copy */
   int8_t *wind = nullptr;
-  double *phase = nullptr;  /* forward edge phases [3][V] of the marching diagnostics */
+  float *phase = nullptr;   /* FP32 edge values [3][V] of the marching diagnostics (ax_wind.h) */
   int *bcount = nullptr;
   long long *boff = nullptr;
   long long *ctotal = nullptr;
@@ -836,10 +836,10 @@ size_t diag_part_blocks(const ax_params &p) {
 
 int ensure_phase_buffer(ax_lattice *L) {
   if (L->phase) return AX_OK;
-  if (hipMalloc((void **)&L->phase, 3 * L->V * sizeof(double)) != hipSuccess) {
+  if (hipMalloc((void **)&L->phase, 3 * L->V * sizeof(float)) != hipSuccess) {
     (void)hipGetLastError();   /* clear the sticky error of the failed call */
     L->phase = nullptr;
-    return fail(AX_ENOMEM, "edge-phase buffer of %zu bytes", 3 * L->V * sizeof(double));
+    return fail(AX_ENOMEM, "edge-phase buffer of %zu bytes", 3 * L->V * sizeof(float));
   }
   return AX_OK;
 }
@@ -847,25 +847,16 @@ int ensure_phase_buffer(ax_lattice *L) {
 /* energy/Gauss (and, with phases, the edge phases) of the current level */
 hipError_t launch_march_diag(ax_lattice *L, bool phases) {
   const DiagConsts k = diag_consts(&L->p);
-  double *ph = phases ? L->phase : nullptr;
+  float *ph = phases ? L->phase : nullptr;
   return L->esize == 8
              ? launch_diag_march<double>(k, levels_of<double>(L, L->timestep), kDiagXseg, ph, L->part, L->dres, L->stream)
              : launch_diag_march<float>(k, levels_of<float>(L, L->timestep), kDiagXseg, ph, L->part, L->dres, L->stream);
 }
 
-/* windings of the current level from the phases launch_march_diag stored */
+/* windings of the current level from the edge values launch_march_diag
+ * stored */
 hipError_t launch_phase_winding(ax_lattice *L, int8_t *out, double *res3) {
-  const ax_params &p = L->p;
-  const int tN = (L->timestep + 1) % 2;
-  auto P = [&](int c) { return (size_t)(c * 2 + tN) * L->pitch; };
-  if (L->esize == 8) {
-    const double *b = (const double *)L->state;
-    return launch_wind_phase<double>(p.nx, p.ny, p.nz, b + P(2), b + P(3), b + P(4), L->phase, out, L->part, res3,
-                                     L->stream);
-  }
-  const float *b = (const float *)L->state;
-  return launch_wind_phase<float>(p.nx, p.ny, p.nz, b + P(2), b + P(3), b + P(4), L->phase, out, L->part, res3,
-                                  L->stream);
+  return launch_wind_phase(L->p.nx, L->p.ny, L->p.nz, L->phase, out, L->part, res3, L->stream);
 }
 
 int ensure_winding_buffer(ax_lattice *L) {

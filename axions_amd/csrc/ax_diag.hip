@@ -416,9 +416,11 @@ __global__ void __launch_bounds__(256) cross_write_kernel(int nx, int ny, int nz
  * carrying the x-backward field, link and link trig from the previous
  * plane, and, with PH, the three forward edge phases
  *   delta_a(s) = arg( conj(phi(s)) exp(-i theta_a(s)) phi(s+a) )
- * from the same link sincos, stored for wind_phase_kernel.  The winding of
- * a plaquette needs four edge phases of three sites; computing each edge
- * once here instead of nine per site in winding_kernel is the saving. */
+ * from the same link sincos, stored as the FP32 edge values
+ * e_a(s) = delta_a(s) + theta_a(s) that wind_march_kernel combines
+ * (ax_wind.h).  The winding of a plaquette needs four edges of three
+ * sites; computing each edge once here instead of nine per site in
+ * winding_kernel is the saving. */
 
 /* link_phase_sc (ax_energy.h): link_phase with the link's sin/cos already at hand */
 
@@ -430,7 +432,7 @@ __global__ void __launch_bounds__(256) cross_write_kernel(int nx, int ny, int nz
  * per 512^3 call); FP32 keeps the compiler's choice (already 3 waves; the
  * same cap there changed its schedule and cost 36 %). */
 template <typename T, bool PH, bool P2>
-__global__ void __launch_bounds__(64, sizeof(T) == 8 ? 3 : 1) diag_march_kernel(DiagConsts k, Levels<T> L, TileGrid g, double *phase,
+__global__ void __launch_bounds__(64, sizeof(T) == 8 ? 3 : 1) diag_march_kernel(DiagConsts k, Levels<T> L, TileGrid g, float *phase,
                                                         double *part) {
   const int w = (int)blockIdx.x;
   int zc, y, seg;
@@ -574,9 +576,9 @@ __global__ void __launch_bounds__(64, sizeof(T) == 8 ? 3 : 1) diag_march_kernel(
       const double dZ = link_phase_sc(pr, pi, PRzp, PIzp, sz, cz);
       if (act) {
         const size_t s = ((size_t)x * NY + y) * NZ + z;
-        phase[s] = dX;
-        phase[V + s] = dY;
-        phase[2 * V + s] = dZ;
+        phase[s] = (float)(dX + X);   /* edge values e_a(s) (ax_wind.h) */
+        phase[V + s] = (float)(dY + Y);
+        phase[2 * V + s] = (float)(dZ + Z);
       }
     }
     if (act) {
@@ -597,22 +599,19 @@ __global__ void __launch_bounds__(64, sizeof(T) == 8 ? 3 : 1) diag_march_kernel(
   }
 }
 
-/* Windings of the current level from the edge phases the marching
+/* Windings of the current level from the edge values the marching
  * diagnostics stored (wind_march, ax_wind.h): one wave per work item of
- * the tile grid.  Per-site kernels read the x+1 plane's theta and phases a
- * second time (512^3 FP64: 12.7 GB fetched for 6.9 GB of operands); the
- * march carries them.  out (the winding field [3][V]) may be null when only
- * the counts are wanted. */
-template <typename T>
-__global__ void __launch_bounds__(64) wind_march_kernel(int nx, int ny, int nz, const T *tx, const T *ty, const T *tz,
-                                                        const double *phase, TileGrid g, int8_t *out, double *part) {
+ * the tile grid, carrying plane x+1's values.  out (the winding field
+ * [3][V]) may be null when only the counts are wanted. */
+__global__ void __launch_bounds__(64) wind_march_kernel(int nx, int ny, int nz, const float *E, TileGrid g, int8_t *out,
+                                                        double *part) {
   const int item = (int)blockIdx.x;
   int zc, y, seg;
   tile_item(g, ny, item, zc, y, seg);
   const int x0 = seg * g.xseg, x1 = min(nx, x0 + g.xseg);
   const WindRow w(nx, ny, nz, y, zc);
   double tot[3] = {0, 0, 0};
-  wind_march<T>(w, tx, ty, tz, phase, x0, x1, [&](size_t s, const int *wn) {
+  wind_march(w, E, x0, x1, [&](size_t s, const int *wn) {
 #pragma unroll
     for (int pl = 0; pl < 3; ++pl) {
       if (out) out[(size_t)pl * w.V + s] = (int8_t)wn[pl];
@@ -668,7 +667,7 @@ hipError_t launch_crossings(int nx, int ny, int nz, double dx, double dy, double
 }
 
 template <typename T>
-hipError_t launch_diag_march(const DiagConsts &k, const Levels<T> &L, int xseg, double *phase, double *part,
+hipError_t launch_diag_march(const DiagConsts &k, const Levels<T> &L, int xseg, float *phase, double *part,
                              double *out3, hipStream_t st) {
   const TileGrid g = make_tile_grid(k.nx, k.ny, k.nz, xseg);
   /* every divisor the P2 kernel multiplies by the reciprocal of must be a
@@ -727,13 +726,11 @@ bool diag_pow2(double dx, double dy, double dz, double gg) {
   return p2;
 }
 
-template <typename T>
-hipError_t launch_wind_phase(int nx, int ny, int nz, const T *tx, const T *ty, const T *tz, const double *phase,
-                             int8_t *out, double *part, double *out3, hipStream_t st) {
+hipError_t launch_wind_phase(int nx, int ny, int nz, const float *E, int8_t *out, double *part, double *out3,
+                             hipStream_t st) {
   /* part holds 3 doubles per work item (diag_part_blocks) */
   const TileGrid g = make_tile_grid(nx, ny, nz, kDiagXseg);
-  hipLaunchKernelGGL((wind_march_kernel<T>), dim3(g.work), dim3(64), 0, st, nx, ny, nz, tx, ty, tz, phase, g, out,
-                     part);
+  hipLaunchKernelGGL(wind_march_kernel, dim3(g.work), dim3(64), 0, st, nx, ny, nz, E, g, out, part);
   hipLaunchKernelGGL(sum3_kernel, dim3(1), dim3(1024), 0, st, part, g.work, out3);
   return hipGetLastError();
 }
@@ -745,10 +742,8 @@ hipError_t launch_wind_phase(int nx, int ny, int nz, const T *tx, const T *ty, c
   template hipError_t launch_crossings<T>(int, int, int, double, double, double, const T *, const T *, int,   \
                                           int, int *, long long *, long long *, double *, long long,          \
                                           hipStream_t, bool);                                               \
-  template hipError_t launch_diag_march<T>(const DiagConsts &, const Levels<T> &, int, double *, double *,    \
-                                           double *, hipStream_t);                                          \
-  template hipError_t launch_wind_phase<T>(int, int, int, const T *, const T *, const T *, const double *,    \
-                                           int8_t *, double *, double *, hipStream_t);
+  template hipError_t launch_diag_march<T>(const DiagConsts &, const Levels<T> &, int, float *, double *,     \
+                                           double *, hipStream_t);
 AX_INST(double)
 AX_INST(float)
 #undef AX_INST

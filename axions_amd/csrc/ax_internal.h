@@ -124,17 +124,17 @@ hipError_t launch_global_energy(const ax_params &p, const void *state, size_t pi
 template <typename T>
 hipError_t launch_diag(const DiagConsts &k, const Levels<T> &L, double *part, double *out3, hipStream_t st);
 /* periodic Abelian-Higgs: energy/Gauss march, with phase != NULL also the
- * forward edge phases [3][V]; launch_wind_phase turns them into windings
- * (the field into out when out != NULL, the counts into out3) */
+ * FP32 edge values [3][V] (forward edge phase + link angle, ax_wind.h);
+ * launch_wind_phase turns them into windings (the field into out when
+ * out != NULL, the counts into out3) */
 constexpr int kDiagXseg = 32;
 bool use_diag_march(const ax_params &p);
 size_t diag_part_blocks(const ax_params &p);
 template <typename T>
-hipError_t launch_diag_march(const DiagConsts &k, const Levels<T> &L, int xseg, double *phase, double *part,
+hipError_t launch_diag_march(const DiagConsts &k, const Levels<T> &L, int xseg, float *phase, double *part,
                              double *out3, hipStream_t st);
-template <typename T>
-hipError_t launch_wind_phase(int nx, int ny, int nz, const T *tx, const T *ty, const T *tz, const double *phase,
-                             int8_t *out, double *part, double *out3, hipStream_t st);
+hipError_t launch_wind_phase(int nx, int ny, int nz, const float *E, int8_t *out, double *part, double *out3,
+                             hipStream_t st);
 template <typename T>
 hipError_t launch_winding(int nx, int ny, int nz, const T *fr, const T *fi, const T *tx, const T *ty, const T *tz,
                           double q, int ex, int ez, int8_t *out, double *part, double *out3, hipStream_t st);

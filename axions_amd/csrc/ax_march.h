@@ -412,7 +412,7 @@ typedef __attribute__((address_space(3))) void lds_void;
  * multiplications by the exact reciprocals). */
 struct FusedDiag {
   double *div;     /* [V] divergence of the links of a level */
-  double *phase;   /* [3][V] forward edge phases */
+  float *phase;    /* [3][V] edge values: forward edge phase + link angle, FP32 (ax_wind.h) */
   double *part;    /* [3] per work item: energy, sum |G|, non-finite count */
   EnergyConsts ek;
 };
@@ -680,9 +680,9 @@ __device__ __forceinline__ void ah_march(const MarchConsts<R> &k, const Levels<R
         const double dZ = link_phase_sc(PRc, PIc, D.PRzp, D.PIzp, tr.sz, tr.cz);
         if (act) {
           const size_t V = (size_t)NX * NY * NZ;
-          fd.phase[si] = dX;
-          fd.phase[V + si] = dY;
-          fd.phase[2 * V + si] = dZ;
+          fd.phase[si] = (float)(dX + TXc);
+          fd.phase[V + si] = (float)(dY + TYc);
+          fd.phase[2 * V + si] = (float)(dZ + TZc);
           rho_sum += rho;
           g_sum += fabs(G);
           const double vals[10] = {PRc, PIc, TXc, TYc, TZc, D.oPR, D.oPI, D.oX, D.oY, D.oZ};

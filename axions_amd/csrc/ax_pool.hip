@@ -67,7 +67,7 @@ struct BatchEntry {
   int32_t steps;        /* TimeSteps to take in this chunk (<= chunk length) */
   int32_t pre;          /* 1: count windings before the first step (t0 == 0) */
   int32_t pad[2];
-  uint64_t phase;       /* device address of the lattice's edge phases [3][n^3] */
+  uint64_t phase;       /* device address of the lattice's FP32 edge values [3][n^3] (ax_wind.h) */
 };
 static_assert(sizeof(BatchEntry) == 64, "entry layout");
 
@@ -192,13 +192,12 @@ __global__ void __launch_bounds__(64) pool_wind_pass_kernel(const BatchHeader *_
     const Levels<T> L = entry_levels<T>(E, E.t0 + (which == 0 ? 0 : E.steps));   /* state after t TimeSteps */
     const WindRow row(E.n, E.n, E.n, y, zc);
     if (WIND) {
-      wind_march<T>(row, L.cur[2], L.cur[3], L.cur[4], (const double *)E.phase, x0, x1,
-                    [&](size_t, const int *wn) {
+      wind_march(row, (const float *)E.phase, x0, x1, [&](size_t, const int *wn) {
 #pragma unroll
                       for (int c = 0; c < 3; ++c) v[c] += abs(wn[c]);
                     });
     } else {
-      v[0] += phase_march<T>(row, L.cur[0], L.cur[1], L.cur[2], L.cur[3], L.cur[4], (double *)E.phase, x0, x1);
+      v[0] += phase_march<T>(row, L.cur[0], L.cur[1], L.cur[2], L.cur[3], L.cur[4], (float *)E.phase, x0, x1);
     }
   }
   flush();

@@ -97,8 +97,8 @@ class Arena {
 
 /* Arena slot of one n^3 Abelian-Higgs lattice: its state (5 components x 2
  * levels of planes, each plane padded by pad_bytes rounded to the element
- * size), then, 256-byte aligned, the forward edge phases of the winding pass
- * (3 x n^3 doubles); the whole block rounded to the arena alignment. */
+ * size), then, 256-byte aligned, the edge values of the winding pass
+ * (3 x n^3 floats); the whole block rounded to the arena alignment. */
 inline size_t lattice_pitch(size_t sites, size_t esize, size_t pad_bytes) {
   return sites + (pad_bytes + esize - 1) / esize;
 }
@@ -109,7 +109,7 @@ inline size_t lattice_phase_offset(int n, size_t esize, size_t pad_bytes) {
 }
 inline size_t lattice_bytes(int n, size_t esize, size_t pad_bytes) {
   const size_t V = (size_t)n * n * n;
-  const size_t b = lattice_phase_offset(n, esize, pad_bytes) + 3 * V * sizeof(double);
+  const size_t b = lattice_phase_offset(n, esize, pad_bytes) + 3 * V * sizeof(float);
   return (b + Arena::kAlign - 1) / Arena::kAlign * Arena::kAlign;
 }
 

@@ -108,15 +108,27 @@ struct WindRow {
   __device__ size_t site(int x) const { return ((size_t)x * ny + y) * nz + z; }
 };
 
-/* The forward edge phases delta_a(s) = link_phase(phi(s), phi(s+a),
+/* The marching passes store, per edge, e_a(s) = delta_a(s) + theta_a(s)
+ * rounded to FP32, and form W from four of them:
+ *   W = rint( (e_a(s) + e_b(s+a) - e_a(s+b) - e_b(s)) / 2pi ).
+ * The sum is site_winding's sum regrouped, and W is the same integer: the
+ * exact phases of phi telescope around the plaquette, so the exact sum is
+ * 2 pi K (K an integer) and the computed one differs from it only by
+ * rounding -- FP64 delta and sin/cos a few ulp of pi each, the FP32 e
+ * half an ulp of |delta + theta|, 2^-24 |e| -- which stays far below pi
+ * for any finite link angle below 2^20.  Both forms round to K.
+ * (Non-finite fields are counted separately and fail the run.)  FP32 e
+ * halves the bytes the step writes and the winding pass reads. */
+
+/* The edge values e_a(s) = fl32(link_phase(phi(s), phi(s+a), theta_a(s)) +
  * theta_a(s)), a = x, y, z (charge 1), of the sites of one wave's march
- * into phase[a V + s]: the edges that wind_march combines.  The arithmetic
- * of link_phase, as the marching diagnostics that store them with the
- * link's sin/cos computed once.  Returns the number of the lane's sites
- * holding a non-finite value in one of the five fields. */
+ * into E[a V + s]: the edges that wind_march combines.  The arithmetic of
+ * link_phase, as the marching diagnostics that store them with the link's
+ * sin/cos computed once.  Returns the number of the lane's sites holding a
+ * non-finite value in one of the five fields. */
 template <typename T>
 __device__ __forceinline__ int phase_march(const WindRow &w, const T *fr, const T *fi, const T *tx, const T *ty,
-                                           const T *tz, double *phase, int x0, int x1) {
+                                           const T *tz, float *E, int x0, int x1) {
   double cr = w.at(fr, x0, w.r0, w.z0), ci = w.at(fi, x0, w.r0, w.z0);
   int bad = 0;
   for (int x = x0; x < x1; ++x) {
@@ -125,14 +137,14 @@ __device__ __forceinline__ int phase_march(const WindRow &w, const T *fr, const 
     const double yr = w.at(fr, x, w.rp, w.z0), yi = w.at(fi, x, w.rp, w.z0);
     const double zr = w.at(fr, x, w.r0, w.zp), zi = w.at(fi, x, w.r0, w.zp);
     const double X = w.at(tx, x, w.r0, w.z0), Y = w.at(ty, x, w.r0, w.z0), Z = w.at(tz, x, w.r0, w.z0);
-    const double dX = link_phase(cr, ci, nr, ni, 1.0 * X);
-    const double dY = link_phase(cr, ci, yr, yi, 1.0 * Y);
-    const double dZ = link_phase(cr, ci, zr, zi, 1.0 * Z);
+    const double dX = link_phase(cr, ci, nr, ni, X);
+    const double dY = link_phase(cr, ci, yr, yi, Y);
+    const double dZ = link_phase(cr, ci, zr, zi, Z);
     if (w.act) {
       const size_t s = w.site(x);
-      phase[s] = dX;
-      phase[w.V + s] = dY;
-      phase[2 * w.V + s] = dZ;
+      E[s] = (float)(dX + X);
+      E[w.V + s] = (float)(dY + Y);
+      E[2 * w.V + s] = (float)(dZ + Z);
       bad += (!isfinite(cr) || !isfinite(ci) || !isfinite(X) || !isfinite(Y) || !isfinite(Z)) ? 1 : 0;
     }
     cr = nr;
@@ -142,36 +154,24 @@ __device__ __forceinline__ int phase_march(const WindRow &w, const T *fr, const 
 }
 
 /* Windings of the xy, xz, yz plaquettes of one wave's march from the stored
- * forward edge phases D[a] = phase + a V of the same level: for axes (a, b)
- *   W = rint( (D_a(s) + D_b(s+a) - D_a(s+b) - D_b(s) + P) / 2pi ),
- *   P = theta_a(s) + theta_b(s+a) - theta_a(s+b) - theta_b(s),
- * the arithmetic of site_winding with q = 1 (phases and angles in its
- * order, so its bits).  emit(s, w) is called for each of the lane's sites. */
-template <typename T, typename Emit>
-__device__ __forceinline__ void wind_march(const WindRow &w, const T *tx, const T *ty, const T *tz,
-                                           const double *phase, int x0, int x1, Emit emit) {
-  const double *DX = phase, *DY = phase + w.V, *DZ = phase + 2 * w.V;
-  double cX = 1.0 * w.at(tx, x0, w.r0, w.z0), cY = 1.0 * w.at(ty, x0, w.r0, w.z0);
-  double cZ = 1.0 * w.at(tz, x0, w.r0, w.z0);
-  double dX = w.at(DX, x0, w.r0, w.z0), dY = w.at(DY, x0, w.r0, w.z0), dZ = w.at(DZ, x0, w.r0, w.z0);
+ * edge values E[a V + s] = e_a(s) of the same level (above).  emit(s, w) is
+ * called for each of the lane's sites. */
+template <typename Emit>
+__device__ __forceinline__ void wind_march(const WindRow &w, const float *E, int x0, int x1, Emit emit) {
+  const float *EX = E, *EY = E + w.V, *EZ = E + 2 * w.V;
+  double cX = w.at(EX, x0, w.r0, w.z0), cY = w.at(EY, x0, w.r0, w.z0), cZ = w.at(EZ, x0, w.r0, w.z0);
   for (int x = x0; x < x1; ++x) {
     const int xp = x == w.nx - 1 ? 0 : x + 1;
     /* plane x+1 (next centre), row y+1, site z+1 */
-    const double nX = 1.0 * w.at(tx, xp, w.r0, w.z0), nY = 1.0 * w.at(ty, xp, w.r0, w.z0);
-    const double nZ = 1.0 * w.at(tz, xp, w.r0, w.z0);
-    const double ndX = w.at(DX, xp, w.r0, w.z0), ndY = w.at(DY, xp, w.r0, w.z0), ndZ = w.at(DZ, xp, w.r0, w.z0);
-    const double yX = 1.0 * w.at(tx, x, w.rp, w.z0), yZ = 1.0 * w.at(tz, x, w.rp, w.z0);
-    const double ydX = w.at(DX, x, w.rp, w.z0), ydZ = w.at(DZ, x, w.rp, w.z0);
-    const double zX = 1.0 * w.at(tx, x, w.r0, w.zp), zY = 1.0 * w.at(ty, x, w.r0, w.zp);
-    const double zdX = w.at(DX, x, w.r0, w.zp), zdY = w.at(DY, x, w.r0, w.zp);
-    const double Pxy = cX + nY - yX - cY, Pxz = cX + nZ - zX - cZ, Pyz = cY + yZ - zY - cZ;
+    const double nX = w.at(EX, xp, w.r0, w.z0), nY = w.at(EY, xp, w.r0, w.z0), nZ = w.at(EZ, xp, w.r0, w.z0);
+    const double yX = w.at(EX, x, w.rp, w.z0), yZ = w.at(EZ, x, w.rp, w.z0);
+    const double zX = w.at(EX, x, w.r0, w.zp), zY = w.at(EY, x, w.r0, w.zp);
     int wn[3];
-    wn[0] = (int)rint((dX + ndY - ydX - dY + Pxy) * 0.15915494309189535);
-    wn[1] = (int)rint((dX + ndZ - zdX - dZ + Pxz) * 0.15915494309189535);
-    wn[2] = (int)rint((dY + ydZ - zdY - dZ + Pyz) * 0.15915494309189535);
+    wn[0] = (int)rint((cX + nY - yX - cY) * 0.15915494309189535);   /* (a, b) = (x, y) */
+    wn[1] = (int)rint((cX + nZ - zX - cZ) * 0.15915494309189535);   /* (x, z) */
+    wn[2] = (int)rint((cY + yZ - zY - cZ) * 0.15915494309189535);   /* (y, z) */
     if (w.act) emit(w.site(x), wn);
     cX = nX; cY = nY; cZ = nZ;
-    dX = ndX; dY = ndY; dZ = ndZ;
   }
 }
 
