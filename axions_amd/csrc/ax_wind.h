@@ -111,14 +111,18 @@ struct WindRow {
 /* The marching passes store, per edge, e_a(s) = delta_a(s) + theta_a(s)
  * rounded to FP32, and form W from four of them:
  *   W = rint( (e_a(s) + e_b(s+a) - e_a(s+b) - e_b(s)) / 2pi ).
- * The sum is site_winding's sum regrouped, and W is the same integer: the
- * exact phases of phi telescope around the plaquette, so the exact sum is
- * 2 pi K (K an integer) and the computed one differs from it only by
- * rounding -- FP64 delta and sin/cos a few ulp of pi each, the FP32 e
- * half an ulp of |delta + theta|, 2^-24 |e| -- which stays far below pi
- * for any finite link angle below 2^20.  Both forms round to K.
- * (Non-finite fields are counted separately and fail the run.)  FP32 e
- * halves the bytes the step writes and the winding pass reads. */
+ * The sum is site_winding's sum regrouped, and W is the same integer.  Each
+ * delta is link_phase's own value (the same FP64 operations as
+ * site_winding), which lies within a few ulp of the edge's exact phase up
+ * to a multiple of 2 pi; the exact phases of phi telescope around the
+ * plaquette, so site_winding's sum lies within rounding of some 2 pi K.
+ * The regrouped sum differs from it by rounding only -- half an ulp of
+ * |delta + theta| per FP32 e, at most 2^-24 |e| -- far below pi for any
+ * link angle below 2^20, so both round to K.  (Non-finite fields are
+ * counted separately and fail the run.)  Computing delta any other way
+ * (another atan2, FP32) would not keep this: an edge phase near +-pi may
+ * then land on the other branch, 2 pi away.  FP32 e halves the bytes the
+ * step writes and the winding pass reads. */
 
 /* The edge values e_a(s) = fl32(link_phase(phi(s), phi(s+a), theta_a(s)) +
  * theta_a(s)), a = x, y, z (charge 1), of the sites of one wave's march

@@ -2,13 +2,16 @@
 rounded to FP32 and form W = rint((e_a(s) + e_b(s+a) - e_a(s+b) - e_b(s)) / 2pi)
 (axions_amd/csrc/ax_wind.h), a regrouping of the per-plaquette sum of the
 oracle's site winding.  The exact sum is 2 pi times an integer (the phases of
-phi telescope around the plaquette), so any evaluation whose error stays
-below pi rounds to the same integer.  Checked here with numpy's own
-sin/cos/arctan2 (not the oracle's or the GPU's bits) against the oracle's
-windings, on random-phase states evolved up to 300 steps, on anisotropic and
-FP32 states, and with link angles shifted by up to 2^16 pi, where FP32
-rounds e to 2^-7 and the integers still agree.  CPU only; the GPU passes are
-compared with the oracle in test_gpu_parity.py, test_gpu_fp32.py and
+phi telescope around the plaquette), so an evaluation whose rounding stays
+below pi gives the same integer.  Checked here with numpy's own
+sin/cos/arctan2, so the check shares no code with the oracle or the GPU
+(numpy's phases could take the other branch for an edge phase within
+rounding of +-pi, which the GPU avoids by computing them with the oracle's
+arithmetic; these states have none), against the oracle's windings: on
+random-phase states evolved up to 300 steps, on anisotropic and FP32
+states, and with link angles shifted by up to 2^16 pi, where FP32 rounds e
+to 2^-7 and the integers still agree.  CPU only; the GPU passes are compared
+with the oracle in test_gpu_parity.py, test_gpu_fp32.py and
 test_gpu_pool.py."""
 import numpy as np
 import pytest
