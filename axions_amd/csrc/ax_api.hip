@@ -696,7 +696,7 @@ int ax_step_diag(ax_lattice *L, int32_t nsteps, int32_t every, int32_t want_wind
       int rc = plain_steps(L, 1);
       if (rc) return rc;
     }
-    return AX_OK;
+    return ax_sync(L);   /* the steps after the last output too */
   }
   int rc = ensure_fused(L, at.size(), want_winding != 0);
   if (rc) return rc;
@@ -726,6 +726,9 @@ int ax_step_diag(ax_lattice *L, int32_t nsteps, int32_t every, int32_t want_wind
     if (e == hipSuccess) e = hipStreamSynchronize(L->stream);
     if (e != hipSuccess) return hip_fail(e, "diagnostic readback");
     for (size_t j = 0; j < k; ++j) fill_diag(L, &h[8 * j], at[j], want_winding != 0, &out[j]);
+  } else {
+    rc = ax_sync(L);
+    if (rc) return rc;
   }
   *count = k;
   return AX_OK;

@@ -242,6 +242,7 @@ def run_lattice(ax, n, precision, steps, warmup, diag_every, seed, log, moore=Fa
                         raise RuntimeError(f"non-finite state at step {d['timestep']}")
                     series.append((d["timestep"], d["energy"], d["gauss_mean"], d["wound_plaquettes"], d["xi"],
                                    d["finite"]))
+                lat.sync()   # ax_step_diag waits at its end; the timed region ends with the device idle
                 return
             done = 0
             while done < nsteps:

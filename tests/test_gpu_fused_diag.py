@@ -112,3 +112,27 @@ def test_fused_diagnostics_512():
     d_fus, st_fus = fused(p, 42, 20, [20, 20], 20)
     assert np.array_equal(st_fus.view(np.uint64), st_sep.view(np.uint64))
     same(d_fus, d_sep)
+
+
+@pytest.mark.parametrize("precision", [64, 32])
+def test_step_diag_waits_for_its_last_steps(precision):
+    """ax_step_diag returns only after every step it launched has run: the
+    steps after its last output too, and when the range holds no output
+    (FP64 takes the fused path, FP32 the separate pass).  Checked by time:
+    the call lasts at least as long as the step kernels it launched."""
+    import time
+    p = ax.ah_params(256, precision=precision)
+    say(f"[step_diag wait] 256^3 fp{precision}")
+    with ax.Lattice(p) as lat:
+        lat.init_random_phase(3)
+        lat.step_diag(2, 20)          # buffers and code paths warm
+        for n, every in ((30, 20), (12, 1000)):
+            lat.kernel_timer(True)
+            lat.kernel_time()
+            t0 = time.perf_counter()
+            lat.step_diag(n, every, winding=True)
+            host_ms = (time.perf_counter() - t0) * 1e3
+            kms, ksteps = lat.kernel_time()
+            lat.kernel_timer(False)
+            assert ksteps >= n - 2 and kms > 1.0, (n, every, kms, ksteps)
+            assert host_ms >= 0.9 * kms, (n, every, host_ms, kms)
