@@ -590,8 +590,10 @@ static bool fused_ok(const ax_lattice *L) {
          (double)L->V * 8.0 < 4294967296.0 && getenv("AX_NO_FUSED_DIAG") == nullptr;
 }
 
-/* device buffers of the fused path for up to npoints diagnostic points */
-static int ensure_fused(ax_lattice *L, size_t npoints, bool winding) {
+/* device buffers of the fused path for up to npoints diagnostic points: the
+ * divergence plane, the partial sums, the results and the edge values (the
+ * output step stores them whether or not windings are wanted) */
+static int ensure_fused(ax_lattice *L, size_t npoints) {
   const size_t items = (size_t)fused_work_items(L->p.nx, L->p.ny, L->p.nz, L->xseg);
   auto grow = [&](double **buf, size_t *have, size_t want, const char *what) -> int {
     if (*have >= want) return AX_OK;
@@ -698,7 +700,7 @@ int ax_step_diag(ax_lattice *L, int32_t nsteps, int32_t every, int32_t want_wind
     }
     return ax_sync(L);   /* the steps after the last output too */
   }
-  int rc = ensure_fused(L, at.size(), want_winding != 0);
+  int rc = ensure_fused(L, at.size());
   if (rc) return rc;
   size_t k = 0;
   int run = 0;   /* plain steps not yet launched */
