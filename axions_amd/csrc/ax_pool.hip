@@ -15,11 +15,12 @@
  *                      chunk after chunk.
  *   pool_wind_pass_kernel
  *                      the winding passes over the same work items, each
- *                      item a march along x (ax_wind.h): first the forward
- *                      edge phases of every resident lattice into its slot,
- *                      plus a non-finite count, then the wound-plaquette
- *                      counts from those phases (the arithmetic of the
- *                      single-lattice diagnostics), summed per lattice
+ *                      item a march along x (ax_wind.h): first the FP32
+ *                      edge values (forward edge phase + link angle) of
+ *                      every resident lattice into its slot, plus a
+ *                      non-finite count, then the wound-plaquette counts
+ *                      from them (as the single-lattice diagnostics form
+ *                      them), summed per lattice
  *                      with integer atomics (order-independent, so
  *                      deterministic).
  *   pool_zero_kernel   clears those counters (a kernel, not a memset, so the
@@ -166,7 +167,7 @@ __global__ void __launch_bounds__(64) pool_wind_pass_kernel(const BatchHeader *_
   const int q = (work + 7) >> 3;
   int ce = -1;
   int v[3] = {0, 0, 0};
-  /* edge phases: v[0] counts sites with a non-finite field (counter 3);
+  /* edge values: v[0] counts sites with a non-finite field (counter 3);
    * windings: v[0..2] sum |W| of the xy, xz, yz plaquettes (counters 0..2) */
   auto flush = [&]() {
     if (ce < 0) return;
