@@ -11,7 +11,9 @@
  * and q the field's charge (1 for Abelian-Higgs).  Each
  * delta lies in (-pi, pi], so W is the integer the gauge-invariant phase
  * winds around the plaquette.  All arithmetic is FP64, whatever the storage
- * type T, so FP32 and FP64 states share one definition.
+ * type T, so FP32 and FP64 states share one definition.  The marching
+ * passes at the end of this file store FP32 edge values e = delta + q theta
+ * and form the same integers from four of them.
  */
 #ifndef AXIONS_AX_WIND_H
 #define AXIONS_AX_WIND_H
