@@ -887,6 +887,20 @@ int launch_winding_field(ax_lattice *L, int field, int8_t *out, double *part, do
   const int ex = !glob && p.boundary != AX_BC_PERIODIC, ez = !glob && p.boundary_z != AX_BC_PERIODIC;
   int rc = prepare_faces(L);
   if (rc) return rc;
+  /* periodic Abelian-Higgs and Moore: the marching edge-value passes
+   * (ax_wind.h; the same integers as the site kernel) */
+  if (!glob && !ex && !ez && getenv("AX_DIAG_SITE_KERNEL") == nullptr && ensure_phase_buffer(L) == AX_OK) {
+    if (L->esize == 8) {
+      const double *b = (const double *)L->state;
+      e = launch_edge_winding<double>(p.nx, p.ny, p.nz, b + P(c0), b + P(c0 + 1), b + P(t0), b + P(t0 + 1),
+                                      b + P(t0 + 2), q, L->phase, out, part, res3, L->stream);
+    } else {
+      const float *b = (const float *)L->state;
+      e = launch_edge_winding<float>(p.nx, p.ny, p.nz, b + P(c0), b + P(c0 + 1), b + P(t0), b + P(t0 + 1),
+                                     b + P(t0 + 2), q, L->phase, out, part, res3, L->stream);
+    }
+    return e == hipSuccess ? AX_OK : hip_fail(e, "winding launch");
+  }
   if (L->esize == 8) {
     const double *b = (const double *)L->state;
     e = launch_winding<double>(p.nx, p.ny, p.nz, b + P(c0), b + P(c0 + 1), b + P(t0), b + P(t0 + 1), b + P(t0 + 2), q,

@@ -666,6 +666,30 @@ hipError_t launch_crossings(int nx, int ny, int nz, double dx, double dy, double
   return hipGetLastError();
 }
 
+/* The FP32 edge values of field (fr, fi) of charge q over the work items of
+ * the tile grid (phase_march, ax_wind.h), for periodic lattices whose
+ * energy and Gauss law take the site kernel (Moore). */
+template <typename T>
+__global__ void __launch_bounds__(64) edge_march_kernel(int nx, int ny, int nz, const T *fr, const T *fi, const T *tx,
+                                                        const T *ty, const T *tz, double q, TileGrid g, float *E) {
+  int zc, y, seg;
+  tile_item(g, ny, (int)blockIdx.x, zc, y, seg);
+  const int x0 = seg * g.xseg, x1 = min(nx, x0 + g.xseg);
+  const WindRow w(nx, ny, nz, y, zc);
+  (void)phase_march<T>(w, fr, fi, tx, ty, tz, E, x0, x1, q);
+}
+
+template <typename T>
+hipError_t launch_edge_winding(int nx, int ny, int nz, const T *fr, const T *fi, const T *tx, const T *ty,
+                               const T *tz, double q, float *E, int8_t *out, double *part, double *out3,
+                               hipStream_t st) {
+  const TileGrid g = make_tile_grid(nx, ny, nz, kDiagXseg);
+  hipLaunchKernelGGL((edge_march_kernel<T>), dim3(g.work), dim3(64), 0, st, nx, ny, nz, fr, fi, tx, ty, tz, q, g,
+                     E);
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? launch_wind_phase(nx, ny, nz, E, out, part, out3, st) : e;
+}
+
 template <typename T>
 hipError_t launch_diag_march(const DiagConsts &k, const Levels<T> &L, int xseg, float *phase, double *part,
                              double *out3, hipStream_t st) {
@@ -743,7 +767,10 @@ hipError_t launch_wind_phase(int nx, int ny, int nz, const float *E, int8_t *out
                                           int, int *, long long *, long long *, double *, long long,          \
                                           hipStream_t, bool);                                               \
   template hipError_t launch_diag_march<T>(const DiagConsts &, const Levels<T> &, int, float *, double *,     \
-                                           double *, hipStream_t);
+                                           double *, hipStream_t);                                          \
+  template hipError_t launch_edge_winding<T>(int, int, int, const T *, const T *, const T *, const T *,      \
+                                             const T *, double, float *, int8_t *, double *, double *,      \
+                                             hipStream_t);
 AX_INST(double)
 AX_INST(float)
 #undef AX_INST

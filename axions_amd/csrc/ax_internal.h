@@ -135,6 +135,12 @@ hipError_t launch_diag_march(const DiagConsts &k, const Levels<T> &L, int xseg, 
                              double *out3, hipStream_t st);
 hipError_t launch_wind_phase(int nx, int ny, int nz, const float *E, int8_t *out, double *part, double *out3,
                              hipStream_t st);
+/* periodic lattices: the FP32 edge values of field (fr, fi) of charge q into
+ * E, then the windings from them, as launch_wind_phase */
+template <typename T>
+hipError_t launch_edge_winding(int nx, int ny, int nz, const T *fr, const T *fi, const T *tx, const T *ty,
+                               const T *tz, double q, float *E, int8_t *out, double *part, double *out3,
+                               hipStream_t st);
 template <typename T>
 hipError_t launch_winding(int nx, int ny, int nz, const T *fr, const T *fi, const T *tx, const T *ty, const T *tz,
                           double q, int ex, int ez, int8_t *out, double *part, double *out3, hipStream_t st);

@@ -126,15 +126,16 @@ struct WindRow {
  * then land on the other branch, 2 pi away.  FP32 e halves the bytes the
  * step writes and the winding pass reads. */
 
-/* The edge values e_a(s) = fl32(link_phase(phi(s), phi(s+a), theta_a(s)) +
- * theta_a(s)), a = x, y, z (charge 1), of the sites of one wave's march
- * into E[a V + s]: the edges that wind_march combines.  The arithmetic of
- * link_phase, as the marching diagnostics that store them with the link's
- * sin/cos computed once.  Returns the number of the lane's sites holding a
- * non-finite value in one of the five fields. */
+/* The edge values e_a(s) = fl32(link_phase(phi(s), phi(s+a), q theta_a(s)) +
+ * q theta_a(s)), a = x, y, z, of the sites of one wave's march into
+ * E[a V + s]: the edges that wind_march combines.  The arithmetic of
+ * link_phase and site_winding (q times the link, q = 1 for Abelian-Higgs),
+ * as the marching diagnostics that store them with the link's sin/cos
+ * computed once.  Returns the number of the lane's sites holding a
+ * non-finite value in phi or a link. */
 template <typename T>
 __device__ __forceinline__ int phase_march(const WindRow &w, const T *fr, const T *fi, const T *tx, const T *ty,
-                                           const T *tz, float *E, int x0, int x1) {
+                                           const T *tz, float *E, int x0, int x1, double q = 1.0) {
   double cr = w.at(fr, x0, w.r0, w.z0), ci = w.at(fi, x0, w.r0, w.z0);
   int bad = 0;
   for (int x = x0; x < x1; ++x) {
@@ -142,7 +143,7 @@ __device__ __forceinline__ int phase_march(const WindRow &w, const T *fr, const 
     const double nr = w.at(fr, xp, w.r0, w.z0), ni = w.at(fi, xp, w.r0, w.z0);
     const double yr = w.at(fr, x, w.rp, w.z0), yi = w.at(fi, x, w.rp, w.z0);
     const double zr = w.at(fr, x, w.r0, w.zp), zi = w.at(fi, x, w.r0, w.zp);
-    const double X = w.at(tx, x, w.r0, w.z0), Y = w.at(ty, x, w.r0, w.z0), Z = w.at(tz, x, w.r0, w.z0);
+    const double X = q * w.at(tx, x, w.r0, w.z0), Y = q * w.at(ty, x, w.r0, w.z0), Z = q * w.at(tz, x, w.r0, w.z0);
     const double dX = link_phase(cr, ci, nr, ni, X);
     const double dY = link_phase(cr, ci, yr, yi, Y);
     const double dZ = link_phase(cr, ci, zr, zi, Z);

@@ -391,6 +391,34 @@ def test_diag_march_equals_site_kernel(shape, precision, dx, g, energy_form):
     assert d_m["finite"] == d_s["finite"] == 1
 
 
+@pytest.mark.parametrize("shape,precision", [((12, 10, 70), 64), ((16, 16, 64), 32), ((9, 12, 130), 64),
+                                             ((10, 8, 33), 32)])
+def test_moore_winding_march_equals_site_kernel(shape, precision):
+    """Periodic Moore windings of both scalars (charges q1 = 2, q2 = 1) from
+    the marching edge-value passes equal the site kernel's bit for bit, and
+    so do the diagnostics' wound counts (z tails, rows not a multiple of 8)."""
+    import os
+    nx, ny, nz = shape
+    p = ax.moore_params(nx, ny=ny, nz=nz, dt=0.2, precision=precision)
+    say(f"[moore winding march] {shape} fp{precision}")
+    with ax.Lattice(p) as lat:
+        lat.init_random_phase(23)
+        lat.step(9)
+        w_m = [lat.winding(f) for f in (0, 1)]
+        d_m = lat.diagnostics(winding=True)
+        os.environ["AX_DIAG_SITE_KERNEL"] = "1"
+        try:
+            w_s = [lat.winding(f) for f in (0, 1)]
+            d_s = lat.diagnostics(winding=True)
+        finally:
+            os.environ.pop("AX_DIAG_SITE_KERNEL", None)
+    for f in (0, 1):
+        assert np.array_equal(w_m[f], w_s[f]), f
+    assert np.abs(w_m[0].astype(np.int64)).sum() > 0
+    assert d_m["wound_plaquettes"] == d_s["wound_plaquettes"]
+    assert d_m["energy"] == d_s["energy"] and d_m["gauss_mean"] == d_s["gauss_mean"]
+
+
 @pytest.mark.parametrize("shape,precision", [((24, 16, 192), 64), ((20, 24, 64), 32), ((12, 10, 128), 64),
                                              ((16, 8, 100), 32)])
 def test_work_orders_give_the_same_state(shape, precision):
