@@ -537,7 +537,7 @@ int ax_diagnostics(ax_lattice *L, int32_t want_winding, ax_diag *out) {
   const ax_params &p = L->p;
   const bool glob = p.model == AX_MODEL_GLOBAL;
   hipError_t e;
-  bool march = false;
+  bool march = false, mwind = false;
   if (glob) {
     /* the energy leapfrog.cpp prints for the last TimeStep (its pre-step
      * field is still in the other buffer); none before the first step */
@@ -551,9 +551,11 @@ int ax_diagnostics(ax_lattice *L, int32_t want_winding, ax_diag *out) {
     int rc = prepare_faces(L);
     if (rc) return rc;
     march = use_diag_march(p);
-    if (march && want_winding && ensure_phase_buffer(L) != AX_OK) march = false;
+    /* the windings come from the march's edge values (periodic), else from
+     * launch_winding_field */
+    mwind = want_winding && march && use_diag_march_windings(p) && ensure_phase_buffer(L) == AX_OK;
     if (march)
-      e = launch_march_diag(L, want_winding != 0);
+      e = launch_march_diag(L, mwind);
     else if (L->esize == 8)
       e = launch_diag<double>(k, levels_of<double>(L, L->timestep), L->part, L->dres, L->stream);
     else
@@ -562,7 +564,7 @@ int ax_diagnostics(ax_lattice *L, int32_t want_winding, ax_diag *out) {
   if (e != hipSuccess) return hip_fail(e, "diag launch");
   if (want_winding) {
     /* only the count is returned: the march stores no winding field */
-    if (march) {
+    if (mwind) {
       e = launch_phase_winding(L, nullptr, L->dres + 4);
       if (e != hipSuccess) return hip_fail(e, "winding launch");
     } else {
@@ -744,7 +746,7 @@ int ax_winding_field(ax_lattice *L, int32_t field, int8_t *out, size_t n) {
   if (field < 0 || field > (L->p.model == AX_MODEL_MOORE ? 1 : 0)) return fail(AX_EINVAL, "field %d", field);
   int rc = ensure_winding_buffer(L);
   if (rc) return rc;
-  if (use_diag_march(L->p) && ensure_phase_buffer(L) == AX_OK) {
+  if (use_diag_march_windings(L->p) && ensure_phase_buffer(L) == AX_OK) {
     hipError_t e = launch_march_diag(L, true);
     if (e == hipSuccess) e = launch_phase_winding(L, L->wind, L->dres + 4);
     if (e != hipSuccess) return hip_fail(e, "winding launch");
@@ -826,9 +828,15 @@ int prepare_faces(ax_lattice *L) {
   return e == hipSuccess ? AX_OK : hip_fail(e, "Neumann faces launch");
 }
 
+/* Abelian-Higgs energy and Gauss law by the diagnostics march (any boundary:
+ * it sums the bulk region where x/y or z are not periodic) */
 bool use_diag_march(const ax_params &p) {
-  return p.model == AX_MODEL_AH && p.boundary == AX_BC_PERIODIC && p.boundary_z == AX_BC_PERIODIC &&
-         getenv("AX_DIAG_SITE_KERNEL") == nullptr;
+  return p.model == AX_MODEL_AH && getenv("AX_DIAG_SITE_KERNEL") == nullptr;
+}
+
+/* ... and the windings from the edge values it stores (periodic lattices) */
+bool use_diag_march_windings(const ax_params &p) {
+  return use_diag_march(p) && p.boundary == AX_BC_PERIODIC && p.boundary_z == AX_BC_PERIODIC;
 }
 
 /* partial-sum blocks any diagnostic pass of this lattice writes */

@@ -409,12 +409,14 @@ __global__ void __launch_bounds__(256) cross_write_kernel(int nx, int ny, int nz
   }
 }
 
-/* --------------- marching diagnostics (periodic Abelian-Higgs) -------------
+/* ------------------- marching diagnostics (Abelian-Higgs) -----------------
  * One 64-lane wave per (64-site z chunk, row, x segment) marching along x,
  * as the step kernel.  Per site it evaluates the energy density and Gauss
- * residual with the expressions of site_energy_ah (the same per-site bits),
+ * residual with the expressions of site_energy_ah (the same per-site bits;
+ * with x/y or z not periodic only the bulk region's terms are summed, as
+ * diag_kernel does),
  * carrying the x-backward field, link and link trig from the previous
- * plane, and, with PH, the three forward edge phases
+ * plane, and, with PH (periodic lattices), the three forward edge phases
  *   delta_a(s) = arg( conj(phi(s)) exp(-i theta_a(s)) phi(s+a) )
  * from the same link sincos, stored as the FP32 edge values
  * e_a(s) = delta_a(s) + theta_a(s) that wind_march_kernel combines
@@ -444,6 +446,7 @@ __global__ void __launch_bounds__(64, sizeof(T) == 8 ? 3 : 1) diag_march_kernel(
   const bool act = zr < NZ;
   const int z = act ? zr : NZ - 1;
   const int ym = y == 0 ? NY - 1 : y - 1, yp = y == NY - 1 ? 0 : y + 1;
+  const bool in_yz = !(k.ex && (y == 0 || y == NY - 1)) && !(k.ez && (z == 0 || z == NZ - 1));
   const uint32_t E = (uint32_t)sizeof(T);
   const uint32_t oz = (uint32_t)z * E;
   const uint32_t ozm = (uint32_t)(z == 0 ? NZ - 1 : z - 1) * E;
@@ -582,8 +585,14 @@ __global__ void __launch_bounds__(64, sizeof(T) == 8 ? 3 : 1) diag_march_kernel(
       }
     }
     if (act) {
-      rho_sum += rho;
-      g_sum += fabs(G);
+      /* energy and Gauss over the bulk region only when x/y or z are not
+       * periodic (the boundary sites' terms, computed from wrapped
+       * neighbours, are dropped), finiteness over every site -- as
+       * diag_kernel */
+      if (in_yz && !(k.ex && (x == 0 || x == NX - 1))) {
+        rho_sum += rho;
+        g_sum += fabs(G);
+      }
       const double vals[10] = {PRc, PIc, X, Y, Z, oPR, oPI, oX, oY, oZ};
 #pragma unroll
       for (int q = 0; q < 10; ++q) bad += isfinite(vals[q]) ? 0.0 : 1.0;

@@ -129,6 +129,7 @@ hipError_t launch_diag(const DiagConsts &k, const Levels<T> &L, double *part, do
  * out != NULL, the counts into out3) */
 constexpr int kDiagXseg = 32;
 bool use_diag_march(const ax_params &p);
+bool use_diag_march_windings(const ax_params &p);
 size_t diag_part_blocks(const ax_params &p);
 template <typename T>
 hipError_t launch_diag_march(const DiagConsts &k, const Levels<T> &L, int xseg, float *phase, double *part,
