@@ -635,15 +635,149 @@ __global__ void __launch_bounds__(64) wind_march_kernel(int nx, int ny, int nz, 
   }
 }
 
+/* ------------------ marching diagnostics (Moore model) -------------------
+ * Energy density and Gauss residual of site_energy_moore (the same
+ * expressions in the same order, so the same per-site terms) by one wave
+ * per (64-site z chunk, row, x segment) marching along x: both scalars and
+ * theta_x (both levels) of plane x-1 are carried from the previous plane,
+ * and plane x+1's current level is loaded once, as the next plane's centre.
+ * The site kernel reads the x-neighbour planes, which another pass over the
+ * lattice had fetched, a second time.  Sums over the bulk region where x/y
+ * or z are not periodic, finiteness over the fourteen stored values of
+ * every site, as diag_kernel.  FP64 is held to 3 waves per SIMD (222 VGPRs
+ * otherwise; 2 spilled with the cap): 256^3 0.83 -> 0.72 ms per call. */
+template <typename T>
+__global__ void __launch_bounds__(64, sizeof(T) == 8 ? 3 : 1) moore_diag_march_kernel(DiagConsts k, Levels<T> L,
+                                                                                   TileGrid g, double *part) {
+  const int w = (int)blockIdx.x;
+  int zc, y, seg;
+  tile_item(g, k.ny, w, zc, y, seg);
+  const int x0 = seg * g.xseg, x1 = min(k.nx, x0 + g.xseg);
+  const int NX = k.nx, NY = k.ny, NZ = k.nz;
+  const int lane = (int)(threadIdx.x & 63);
+  const int zr = zc * 64 + lane;
+  const bool act = zr < NZ;
+  const int z = act ? zr : NZ - 1;
+  const int ym = y == 0 ? NY - 1 : y - 1, yp = y == NY - 1 ? 0 : y + 1;
+  const bool in_yz = !(k.ex && (y == 0 || y == NY - 1)) && !(k.ez && (z == 0 || z == NZ - 1));
+  const uint32_t E = (uint32_t)sizeof(T);
+  const uint32_t oz = (uint32_t)z * E;
+  const uint32_t ozm = (uint32_t)(z == 0 ? NZ - 1 : z - 1) * E;
+  const uint32_t ozp = (uint32_t)(z == NZ - 1 ? 0 : z + 1) * E;
+  const uint32_t R0 = (uint32_t)y * NZ * E, RM = (uint32_t)ym * NZ * E, RP = (uint32_t)yp * NZ * E;
+  const size_t NYNZ = (size_t)NY * NZ;
+  auto C = [&](int c, int x, uint32_t row, uint32_t off) { return (double)ldo(L.cur[c] + (size_t)x * NYNZ, row, off); };
+  auto P = [&](int c, int x, uint32_t row, uint32_t off) { return (double)ldo(L.prv[c] + (size_t)x * NYNZ, row, off); };
+  const double dx = k.dx, dy = k.dy, dz = k.dz, dt = k.dt, gg = k.g;
+  const double q[2] = {k.q1, k.q2};
+  /* plane x-1 (carried): both scalars, theta_x of both levels; plane x */
+  const int xm0 = x0 == 0 ? NX - 1 : x0 - 1;
+  double Fm[4], TXm = C(4, xm0, R0, oz), oXm = P(4, xm0, R0, oz), Fc[4], Tc[3];
+  for (int c = 0; c < 4; ++c) {
+    Fm[c] = C(c, xm0, R0, oz);
+    Fc[c] = C(c, x0, R0, oz);
+  }
+  for (int a = 0; a < 3; ++a) Tc[a] = C(4 + a, x0, R0, oz);
+  double rho_sum = 0, g_sum = 0, bad = 0;
+  for (int x = x0; x < x1; ++x) {
+    const int xp = x == NX - 1 ? 0 : x + 1;
+    double Fn[4], Tn[3], Fo[4], Fy[4], Fz[4];
+    for (int c = 0; c < 4; ++c) {
+      Fn[c] = C(c, xp, R0, oz);
+      Fo[c] = P(c, x, R0, oz);
+      Fy[c] = C(c, x, RM, oz);
+      Fz[c] = C(c, x, R0, ozm);
+    }
+    for (int a = 0; a < 3; ++a) Tn[a] = C(4 + a, xp, R0, oz);
+    const double oX = P(4, x, R0, oz), oY = P(5, x, R0, oz), oZ = P(6, x, R0, oz);
+    const double TYym = C(5, x, RM, oz), oYym = P(5, x, RM, oz);
+    const double TZzm = C(6, x, R0, ozm), oZzm = P(6, x, R0, ozm);
+    const double TXyp = C(4, x, RP, oz), TZyp = C(6, x, RP, oz);
+    const double TXzp = C(4, x, R0, ozp), TYzp = C(5, x, R0, ozp);
+    const double X = Tc[0], Y = Tc[1], Z = Tc[2];
+
+    /* site_energy_moore at s, operands by name */
+    double v[2][2], gx[2][2], gy[2][2], gz[2][2], m[2];
+    for (int f = 0; f < 2; ++f) {
+      const double ar = Fc[2 * f], ai = Fc[2 * f + 1];
+      m[f] = ar * ar + ai * ai;
+      double sxm_, cxm_, sym_, cym_, szm_, czm_;
+      ax_sincos(q[f] * TXm, &sxm_, &cxm_);
+      ax_sincos(q[f] * TYym, &sym_, &cym_);
+      ax_sincos(q[f] * TZzm, &szm_, &czm_);
+      const double rxm = Fm[2 * f], ixm = Fm[2 * f + 1], rym = Fy[2 * f], iym = Fy[2 * f + 1];
+      const double rzm = Fz[2 * f], izm = Fz[2 * f + 1];
+      gx[f][0] = (ar - cxm_ * rxm + sxm_ * ixm) / dx;
+      gy[f][0] = (ar - cym_ * rym + sym_ * iym) / dy;
+      gz[f][0] = (ar - czm_ * rzm + szm_ * izm) / dz;
+      gx[f][1] = (ai - cxm_ * ixm + (-sxm_) * rxm) / dx;
+      gy[f][1] = (ai - cym_ * iym + (-sym_) * rym) / dy;
+      gz[f][1] = (ai - czm_ * izm + (-szm_) * rzm) / dz;
+      v[f][0] = (ar - Fo[2 * f]) / dt;
+      v[f][1] = (ai - Fo[2 * f + 1]) / dt;
+    }
+    double tdc = 0, fcont = 0, G = 0;
+    if (gg != 0) {
+      double divN = (X - TXm) / (dx * dx) + (Y - TYym) / (dy * dy) + (Z - TZzm) / (dz * dz);
+      double divP = (oX - oXm) / (dx * dx) + (oY - oYym) / (dy * dy) + (oZ - oZzm) / (dz * dz);
+      double divT = (divN - divP) / dt;
+      G = divT / gg - 2 * gg * (q[0] * (Fc[1] * v[0][0] - Fc[0] * v[0][1]) +
+                                q[1] * (Fc[3] * v[1][0] - Fc[2] * v[1][1]));
+      double ex = (X - oX) / (gg * dt * dx), ey = (Y - oY) / (gg * dt * dy), ez = (Z - oZ) / (gg * dt * dz);
+      tdc = 0.5 * (ex * ex + ey * ey + ez * ez);
+      double bxy = (1 - ax_cos(X + Tn[1] - TXyp - Y)) / ((gg * dx * dy) * (gg * dx * dy));
+      double bxz = (1 - ax_cos(X + Tn[2] - TXzp - Z)) / ((gg * dx * dz) * (gg * dx * dz));
+      double byz = (1 - ax_cos(Y + TZyp - TYzp - Z)) / ((gg * dy * dz) * (gg * dy * dz));
+      fcont = bxy + bxz + byz;
+    }
+    const double w1 = m[0] - k.eta1 * k.eta1, w2 = m[1] - k.eta2 * k.eta2;
+    double r = v[0][0] * v[0][0] + v[0][1] * v[0][1] + gx[0][0] * gx[0][0] + gx[0][1] * gx[0][1] +
+               gy[0][0] * gy[0][0] + gy[0][1] * gy[0][1] + gz[0][0] * gz[0][0] + gz[0][1] * gz[0][1] +
+               v[1][0] * v[1][0] + v[1][1] * v[1][1] + gx[1][0] * gx[1][0] + gx[1][1] * gx[1][1] +
+               gy[1][0] * gy[1][0] + gy[1][1] * gy[1][1] + gz[1][0] * gz[1][0] + gz[1][1] * gz[1][1] + tdc +
+               fcont + 0.25 * k.lambda1 * (w1 * w1) + 0.25 * k.lambda2 * (w2 * w2) + 0.5 * k.lambda12 * w1 * w2;
+    const double rho = dx * dy * dz * r;
+    const double absG = k.ex ? fabs(G) / k.inner : fabs(G);
+    if (act) {
+      if (in_yz && !(k.ex && (x == 0 || x == NX - 1))) {
+        rho_sum += rho;
+        g_sum += absG;
+      }
+      const double vals[14] = {Fc[0], Fc[1], Fc[2], Fc[3], X, Y, Z, Fo[0], Fo[1], Fo[2], Fo[3], oX, oY, oZ};
+#pragma unroll
+      for (int c = 0; c < 14; ++c) bad += isfinite(vals[c]) ? 0.0 : 1.0;
+    }
+    for (int c = 0; c < 4; ++c) {
+      Fm[c] = Fc[c];
+      Fc[c] = Fn[c];
+    }
+    TXm = X;
+    oXm = oX;
+    for (int a = 0; a < 3; ++a) Tc[a] = Tn[a];
+  }
+  const double a = wave_sum(rho_sum), b = wave_sum(g_sum), c = wave_sum(bad);
+  if (lane == 0) {
+    part[3 * w] = a;
+    part[3 * w + 1] = b;
+    part[3 * w + 2] = c;
+  }
+}
+
 /* ------------------------------ launchers ------------------------------- */
 template <typename T>
 hipError_t launch_diag(const DiagConsts &k, const Levels<T> &L, double *part, double *out3, hipStream_t st) {
   const int V = k.nx * k.ny * k.nz;
-  const int nb = (V + kDiagSitesPerBlock - 1) / kDiagSitesPerBlock;
-  if (k.model == AX_MODEL_AH)
+  int nb = (V + kDiagSitesPerBlock - 1) / kDiagSitesPerBlock;
+  if (k.model == AX_MODEL_AH) {
     hipLaunchKernelGGL((diag_kernel<T, AX_MODEL_AH>), dim3(nb), dim3(256), 0, st, k, L, V, part);
-  else
+  } else if (getenv("AX_DIAG_SITE_KERNEL") == nullptr) {
+    /* part holds 3 doubles per work item (diag_part_blocks) */
+    const TileGrid g = make_tile_grid(k.nx, k.ny, k.nz, kDiagXseg);
+    hipLaunchKernelGGL((moore_diag_march_kernel<T>), dim3(g.work), dim3(64), 0, st, k, L, g, part);
+    nb = g.work;
+  } else {
     hipLaunchKernelGGL((diag_kernel<T, AX_MODEL_MOORE>), dim3(nb), dim3(256), 0, st, k, L, V, part);
+  }
   hipLaunchKernelGGL(sum3_kernel, dim3(1), dim3(1024), 0, st, part, nb, out3);
   return hipGetLastError();
 }
