@@ -9,8 +9,9 @@ sin/cos/arctan2, so the check shares no code with the oracle or the GPU
 rounding of +-pi, which the GPU avoids by computing them with the oracle's
 arithmetic; these states have none), against the oracle's windings: on
 random-phase states evolved up to 300 steps, on anisotropic and FP32
-states, and with link angles shifted by up to 2^16 pi, where FP32 rounds e
-to 2^-7 and the integers still agree.  CPU only; the GPU passes are compared
+states, on both Moore scalars (charges 2 and 1: e = delta + q theta), and
+with link angles shifted by up to 2^16 pi, where FP32 rounds e to 2^-7 and
+the integers still agree.  CPU only; the GPU passes are compared
 with the oracle in test_gpu_parity.py, test_gpu_fp32.py and
 test_gpu_pool.py."""
 import numpy as np
@@ -20,10 +21,14 @@ import axions_amd as ax
 from oracle import oracle_py as orc
 
 
-def edge_value_windings(p, st, n):
-    """W of every xy, xz, yz plaquette from FP32 edge values, numpy trig."""
-    lev = st.reshape(5, 2, p.nx, p.ny, p.nz)[:, (n + 1) % 2].astype(np.float64)
-    pr, pi_, th = lev[0], lev[1], lev[2:5]
+def edge_value_windings(p, st, n, field=0):
+    """W of every xy, xz, yz plaquette from FP32 edge values, numpy trig;
+    Moore: scalar `field` with its charge times the links."""
+    nc = ax.ncomp(p)
+    lev = st.reshape(nc, 2, p.nx, p.ny, p.nz)[:, (n + 1) % 2].astype(np.float64)
+    moore = p.model == ax.AX_MODEL_MOORE
+    c0, q = (2 * field, float(p.q1 if field == 0 else p.q2)) if moore else (0, 1.0)
+    pr, pi_, th = lev[c0], lev[c0 + 1], q * lev[nc - 3:nc]
     e = []
     for a in range(3):
         nr, ni = np.roll(pr, -1, axis=a), np.roll(pi_, -1, axis=a)
@@ -38,8 +43,24 @@ def edge_value_windings(p, st, n):
     return np.stack(w)
 
 
-def oracle_windings(p, st, n):
-    return orc.winding(p, st, n, 0).reshape(3, p.nx, p.ny, p.nz).astype(np.int64)
+def oracle_windings(p, st, n, field=0):
+    return orc.winding(p, st, n, field).reshape(3, p.nx, p.ny, p.nz).astype(np.int64)
+
+
+@pytest.mark.parametrize("field", [0, 1])
+def test_edge_value_windings_moore(field):
+    """Moore: both scalars (charges 2 and 1 times the one link field)."""
+    p = ax.moore_params(12, ny=10, nz=14, dt=0.2)
+    st = orc.init_random_phase(p, 13, 0)
+    done, wound = 0, 0
+    for n in (0, 12, 40):
+        if n > done:
+            orc.step(p, st, done, n - done, 0, 1)
+            done = n
+        w_o = oracle_windings(p, st, n, field)
+        assert np.array_equal(edge_value_windings(p, st, n, field), w_o), (field, n)
+        wound += int(np.abs(w_o).sum())
+    assert wound > 0
 
 
 @pytest.mark.parametrize("name,build,steps", [
