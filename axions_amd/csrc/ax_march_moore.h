@@ -80,7 +80,8 @@ inline void moore_consts_from(const StepConsts &k, MooreConsts<R> &m) {
 /* BUF: the loop's loads and stores are raw buffer accesses, one resource per
  * field spanning both of its levels (planes 2c, 2c+1; the caller checks that
  * two planes fit 32-bit offsets), with the streaming hint on the previous
- * level as in ah_march; otherwise plain global accesses.
+ * level as in ah_march; otherwise plain global accesses from each x plane's
+ * 64-bit address (field planes of any size, 1001^3 FP64 included).
  * FASTDIV (FP64, divisors not powers of two): divisions by dx^2, dy^2,
  * dz^2 through FixedDivisor; the caller has checked the current level's
  * band flag.  CHECK: OR 1 into *bad when a written value lies outside the
@@ -127,39 +128,41 @@ __device__ __forceinline__ void moore_march(const MooreConsts<R> &k, const Level
     offC = curLow ? 0u : lev;
     offP = curLow ? lev : 0u;
   }
-  /* current-level value of field c at (plane/row byte offset ro, lane offset lo) */
-  auto LC = [&](int c, uint32_t ro, uint32_t lo) -> R {
-    if (BUF) return bld<R>(FR[c], lo, offC + ro);
-    return ldo(F[c], ro, lo);
+  /* current-level value of field c in x plane x at (row byte offset ro,
+   * lane offset lo); without BUF the plane's 64-bit address is formed on
+   * the scalar unit, so planes of any size work */
+  const size_t NYNZ = (size_t)NY * NZ;
+  auto LC = [&](int c, int x, uint32_t ro, uint32_t lo) -> R {
+    if (BUF) return bld<R>(FR[c], lo, offC + (uint32_t)x * PLB + ro);
+    return ldo(F[c] + (size_t)x * NYNZ, ro, lo);
   };
+  auto LG = [&](const R *f, int x, uint32_t ro, uint32_t lo) -> R { return ldo(f + (size_t)x * NYNZ, ro, lo); };
 
   /* carry from plane x-1: phi_f(s-x), cos/sin(q_f theta_x(s-x)), sines at s-x */
   const int xm0 = x0 == 0 ? NX - 1 : x0 - 1;
   R Pm[4], cxm[2], sxm[2], Qyx_m, Qzx_m;
   {
-    const uint32_t b = (uint32_t)xm0 * PLB, bn = (uint32_t)x0 * PLB;
-    for (int c = 0; c < 4; ++c) Pm[c] = ldo(F[c], b + R0, oz);
-    const R X = ldo(TX, b + R0, oz), Y = ldo(TY, b + R0, oz), Z = ldo(TZ, b + R0, oz);
+    for (int c = 0; c < 4; ++c) Pm[c] = LG(F[c], xm0, R0, oz);
+    const R X = LG(TX, xm0, R0, oz), Y = LG(TY, xm0, R0, oz), Z = LG(TZ, xm0, R0, oz);
     for (int f = 0; f < 2; ++f) T::sc(q[f] * X, &sxm[f], &cxm[f]);
-    Qyx_m = T::sn(Y + ldo(TX, b + RP, oz) - ldo(TY, bn + R0, oz) - X);
-    Qzx_m = T::sn(Z + ldo(TX, b + R0, ozp) - ldo(TZ, bn + R0, oz) - X);
+    Qyx_m = T::sn(Y + LG(TX, xm0, RP, oz) - LG(TY, x0, R0, oz) - X);
+    Qzx_m = T::sn(Z + LG(TX, xm0, R0, ozp) - LG(TZ, x0, R0, oz) - X);
   }
-  const uint32_t b0 = (uint32_t)x0 * PLB;
   R C[7];
-  for (int c = 0; c < 7; ++c) C[c] = ldo(F[c], b0 + R0, oz);
+  for (int c = 0; c < 7; ++c) C[c] = LG(F[c], x0, R0, oz);
 
   for (int x = x0; x < x1; ++x) {
     const int xp = x == NX - 1 ? 0 : x + 1;
-    const uint32_t b = (uint32_t)x * PLB, bn = (uint32_t)xp * PLB, so = b + R0;
+    const uint32_t so = (uint32_t)x * PLB + R0;   /* BUF only */
     R Nx[7];
-    for (int c = 0; c < 7; ++c) Nx[c] = LC(c, bn + R0, oz);
+    for (int c = 0; c < 7; ++c) Nx[c] = LC(c, xp, R0, oz);
     const R X = C[4], Y = C[5], Z = C[6];
-    const R TXyp = LC(4, b + RP, oz), TZyp = LC(6, b + RP, oz);
-    const R TXzp = LC(4, b + R0, ozp), TYzp = LC(5, b + R0, ozp);
-    const R TXym = LC(4, b + RM, oz), TYym = LC(5, b + RM, oz), TZym = LC(6, b + RM, oz);
-    const R TYxpym = LC(5, bn + RM, oz), TYymzp = LC(5, b + RM, ozp);
-    const R TXzm = LC(4, b + R0, ozm), TYzm = LC(5, b + R0, ozm), TZzm = LC(6, b + R0, ozm);
-    const R TZxpzm = LC(6, bn + R0, ozm), TZypzm = LC(6, b + RP, ozm);
+    const R TXyp = LC(4, x, RP, oz), TZyp = LC(6, x, RP, oz);
+    const R TXzp = LC(4, x, R0, ozp), TYzp = LC(5, x, R0, ozp);
+    const R TXym = LC(4, x, RM, oz), TYym = LC(5, x, RM, oz), TZym = LC(6, x, RM, oz);
+    const R TYxpym = LC(5, xp, RM, oz), TYymzp = LC(5, x, RM, ozp);
+    const R TXzm = LC(4, x, R0, ozm), TYzm = LC(5, x, R0, ozm), TZzm = LC(6, x, R0, ozm);
+    const R TZxpzm = LC(6, xp, R0, ozm), TZypzm = LC(6, x, RP, ozm);
 
     /* plaquette sines (mooreEvol.cpp:310-333 operand order) */
     const R a_xy = X + Nx[5] - TXyp - Y;
@@ -208,10 +211,10 @@ __device__ __forceinline__ void moore_march(const MooreConsts<R> &k, const Level
       T::sc(q[f] * TZzm, &szm, &czm);
       const R ar = C[cr], ai = C[ci];
       const R rxp = Nx[cr], ixp = Nx[ci];
-      const R ryp = LC(cr, b + RP, oz), iyp = LC(ci, b + RP, oz);
-      const R rym = LC(cr, b + RM, oz), iym = LC(ci, b + RM, oz);
-      const R rzp = LC(cr, b + R0, ozp), izp = LC(ci, b + R0, ozp);
-      const R rzm = LC(cr, b + R0, ozm), izm = LC(ci, b + R0, ozm);
+      const R ryp = LC(cr, x, RP, oz), iyp = LC(ci, x, RP, oz);
+      const R rym = LC(cr, x, RM, oz), iym = LC(ci, x, RM, oz);
+      const R rzp = LC(cr, x, R0, ozp), izp = LC(ci, x, R0, ozp);
+      const R rzm = LC(cr, x, R0, ozm), izm = LC(ci, x, R0, ozm);
       {
         R lxx = dvx(cx * rxp + sx * ixp - 2 * ar + cxm[f] * Pm[cr] - sxm[f] * Pm[ci]);
         R lyy = dvy(cy * ryp + sy * iyp - 2 * ar + cym * rym - sym_ * iym);
@@ -236,7 +239,7 @@ __device__ __forceinline__ void moore_march(const MooreConsts<R> &k, const Level
     /* previous level at s: read once, overwritten with level n+1 */
     auto LP = [&](int c) -> R {
       if (BUF) return bld<R, kStream>(FR[c], oz, offP + so);
-      return ldo(O[c], so, oz);
+      return ldo((const R *)O[c] + (size_t)x * NYNZ, R0, oz);
     };
     R Onew[7];
 #pragma unroll
@@ -247,7 +250,7 @@ __device__ __forceinline__ void moore_march(const MooreConsts<R> &k, const Level
     if (act) {
       for (int c = 0; c < 7; ++c) {
         if (BUF) bst<R, kStream>(FR[c], oz, offP + so, Onew[c]);
-        else sto(O[c], so, oz, Onew[c]);
+        else sto(O[c] + (size_t)x * NYNZ, R0, oz, Onew[c]);
       }
       if constexpr (CHECK)
         for (int c = 0; c < 7; ++c) inband = inband && in_moore_band((double)Onew[c]);

@@ -79,7 +79,8 @@ hipError_t launch_step_tiled_moore(const StepConsts &k, const Levels<R> &L, int 
   const bool pow2 = is_pow2(k.dx2) && is_pow2(k.dy2) && is_pow2(k.dz2);
   /* buffer path when both levels of a field lie within 4 GiB of its lower plane */
   const size_t lev = (size_t)std::llabs((const char *)L.cur[0] - (const char *)L.prv[0]);
-  const bool buf = lev + (size_t)k.nx * k.ny * k.nz * sizeof(R) < ((size_t)1 << 32);
+  const bool buf = lev + (size_t)k.nx * k.ny * k.nz * sizeof(R) < ((size_t)1 << 32) &&
+                   getenv("AX_BIG_PLANES") == nullptr;   /* forces the global path, for tests */
   /* FP64 divisions by dx^2, dy^2, dz^2 through FixedDivisor, guarded by the
    * band flags (every site written by this kernel: periodic lattices only) */
   const bool fd = sizeof(R) == 8 && !pow2 && band != nullptr && !k.ex && !k.ez && fixed_divisor_range(k.dx2) &&
@@ -106,13 +107,11 @@ template hipError_t launch_step_tiled_moore<float>(const StepConsts &, const Lev
                                                    int, bool, bool *);
 
 /* Every Abelian-Higgs and Moore lattice marches, whatever its z tail (a
- * last chunk of 1..63 sites runs with its other lanes idle) and, for
- * Abelian-Higgs, whatever its plane size (planes of 4 GiB or more take the
- * 64-bit plane addressing of ah_march).  The Moore march keeps 32-bit plane
- * offsets: its planes must stay below 4 GiB. */
+ * last chunk of 1..63 sites runs with its other lanes idle) and whatever
+ * its plane size (planes of 4 GiB or more take the 64-bit plane addressing
+ * of ah_march and moore_march). */
 bool use_tiled_ah(const ax_params &p) {
-  const bool big = (double)p.nx * p.ny * p.nz * (p.precision == 64 ? 8.0 : 4.0) >= 4294967296.0;
-  return (p.model == AX_MODEL_AH || (p.model == AX_MODEL_MOORE && !big)) && p.nx >= 3 && p.ny >= 3 &&
+  return (p.model == AX_MODEL_AH || p.model == AX_MODEL_MOORE) && p.nx >= 3 && p.ny >= 3 &&
          getenv("AX_FORCE_SITE_KERNEL") == nullptr;
 }
 

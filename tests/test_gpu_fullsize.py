@@ -188,3 +188,8 @@ def test_big_planes_march_equals_site_kernel():
         assert lat.tile()[1]
         lat.step(1)
         lat.sync()
+    say("[fullsize] Moore 1001^3 fp64 (112 GB, moore_mpi_evolution.cpp's lattice): marching kernel")
+    with ax.Lattice(ax.moore_params(1001)) as lat:
+        assert lat.tile()[1]
+        lat.step(2)
+        lat.sync()

@@ -280,10 +280,12 @@ def test_short_z_tails_march_bitexact_vs_oracle(name, monkeypatch):
             assert bits_equal(lat.get_state(), st)
 
 
-@pytest.mark.parametrize("name", ["ah_plain_64", "tiled_aniso_damped", "ah_odd_sizes", "tiled_rect_damped"])
+@pytest.mark.parametrize("name", ["ah_plain_64", "tiled_aniso_damped", "ah_odd_sizes", "tiled_rect_damped",
+                                  "moore_tiled_64", "moore_tiled_96_aniso"])
 def test_big_plane_addressing_bitexact_vs_oracle(name):
-    """The 64-bit plane addressing of lattices whose field planes reach 4 GiB,
-    forced on small lattices (AX_BIG_PLANES): the same bits as the oracle."""
+    """The 64-bit plane addressing of lattices whose field planes reach 4 GiB
+    (ah_march's, and moore_march's global path), forced on small lattices
+    (AX_BIG_PLANES): the same bits as the oracle."""
     import os
     build, nsteps = CASES[name]
     p = build()
