@@ -82,8 +82,11 @@ hipError_t launch_step_tiled_moore(const StepConsts &k, const Levels<R> &L, int 
   const bool buf = lev + (size_t)k.nx * k.ny * k.nz * sizeof(R) < ((size_t)1 << 32) &&
                    getenv("AX_BIG_PLANES") == nullptr;   /* forces the global path, for tests */
   /* FP64 divisions by dx^2, dy^2, dz^2 through FixedDivisor, guarded by the
-   * band flags (every site written by this kernel: periodic lattices only) */
-  const bool fd = sizeof(R) == 8 && !pow2 && band != nullptr && !k.ex && !k.ez && fixed_divisor_range(k.dx2) &&
+   * band flags (every site written by this kernel: periodic lattices only);
+   * with the buffer path only: on the global path (planes of 4 GiB or
+   * more) it spills 67 VGPRs at 3 waves per SIMD and is no faster than the
+   * site kernel, the plain divisions spill none */
+  const bool fd = sizeof(R) == 8 && buf && !pow2 && band != nullptr && !k.ex && !k.ez && fixed_divisor_range(k.dx2) &&
                   fixed_divisor_range(k.dy2) && fixed_divisor_range(k.dz2) && moore_band_charge((int)k.q1) &&
                   moore_band_charge((int)k.q2) && getenv("AX_MOORE_PLAIN_DIV") == nullptr;
   *fd_used = fd;
